@@ -1,0 +1,171 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+One step = one render call of the headline workload: the 983,040-triangle
+sphere field (12 icospheres at subdivision 6 under a 2048x1024 procedural HDR
+sky) at 1920x1080, 64 spp, max depth 16, through generate -> extend -> shade
+-> connect -> accumulate -> tone map.  Scene upload and the BVH build happen
+before the timed region (the build time is reported separately).
+
+Prints one JSON line with Mpaths/s, the traversal roofline of the extend
+kernel (algorithmic bytes from an untimed counting pass of the same frames,
+kernel time from HIP events on the library's stream during the timed steps)
+and the CPU oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 32, 36, 48   # 32-B node record, 36-B triangle, 32-B ray in + 16-B hit out
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--subdiv", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, args):
+    """The CPU oracle on the same workload: full frames, as many spp as fit ~10 s."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib  # checker only: the baseline leg, never the measured path
+
+    threads = min(os.cpu_count() or 1, 16)
+    W, H, depth = args.width, args.height, args.depth
+    t0 = time.perf_counter()
+    oracle_lib.render_stats(scene, W, H, 1, max_depth=depth, threads=threads)
+    first = time.perf_counter() - t0
+    spp = int(max(1, min(args.spp, 10.0 // max(first, 1e-3))))
+    t0 = time.perf_counter()
+    stats = oracle_lib.render_stats(scene, W, H, spp, max_depth=depth, threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(W * H * spp / dt / 1e6, 4),
+        "unit": "Mpaths/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{W}x{H} at {spp} spp (frames 0..{spp - 1}, depth {depth}) of the {spp_total(args)} spp workload; "
+                  f"includes the oracle's CPU LBVH build and env cache; {stats['extension_rays']} extension rays",
+        "seconds": round(dt, 2),
+    }
+
+
+def spp_total(args):
+    return args.spp
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is "
+                                   "design-only, see DESIGN.md)"}))
+        sys.exit(2)
+    from optixrender_amd import abi, scenes
+    from optixrender_amd.renderer import Context
+
+    t0 = time.perf_counter()
+    scene = scenes.sphere_field(seed=1, subdiv=args.subdiv)
+    gen_s = time.perf_counter() - t0
+    ctx = Context(0)
+    sid = ctx.upload(scene)
+    build_ms = ctx.build_bvh(sid)
+    ctx.select(sid)
+    ctx.resize(args.width, args.height)
+    ctx.set_camera(scene.camera)
+    W, H, spp, depth = args.width, args.height, args.spp, args.depth
+    print(f"[bench] scene {scene.n_tris} tris generated in {gen_s:.1f}s, BVH build {build_ms:.2f} ms", file=sys.stderr,
+          flush=True)
+
+    def step(flags=0):
+        return ctx.render(spp=spp, max_depth=depth, rr_start=0, progressive=False, flags=flags)
+
+    counted = None
+    if not args.profile_steps:
+        counted = step(abi.OXR_FLAG_COUNT_TRAVERSAL)   # untimed: algorithmic bytes of the same frames
+        print(f"[bench] counting pass done: {counted.extension_rays} ext rays, {counted.shadow_rays} shadow rays",
+              file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
+    ctx_ms, ext_ms, conn_ms, shade_ms = [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = step()
+        ctx_ms.append(st.render_ms)
+        ext_ms.append(st.extend_ms)
+        conn_ms.append(st.connect_ms)
+        shade_ms.append(st.shade_ms)
+        print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms (extend {st.extend_ms:.1f}, shade "
+              f"{st.shade_ms:.1f}, connect {st.connect_ms:.1f})", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    paths = W * H * spp
+    value = paths / (wall / args.steps) / 1e6
+    out = {
+        "metric": "Mpaths/s 1080p 1M-tri; p50/p99 render latency at equal RMSE vs plain",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"sphere_field {scene.n_tris} tris, {W}x{H}, {spp} spp, depth {depth}, "
+                               f"2048x1024 HDR sky (seed 1)",
+                   "bvh_build_ms": round(build_ms, 3), "device_ms_per_step": round(float(np.mean(ctx_ms)), 3),
+                   "extend_ms_per_step": round(float(np.mean(ext_ms)), 3),
+                   "shade_ms_per_step": round(float(np.mean(shade_ms)), 3),
+                   "connect_ms_per_step": round(float(np.mean(conn_ms)), 3)},
+    }
+    if counted is not None:
+        ext_bytes = NODE_BYTES * counted.ext_node_fetches + TRI_BYTES * counted.ext_tri_tests + \
+            RAY_BYTES * counted.extension_rays
+        conn_nodes = counted.node_fetches - counted.ext_node_fetches
+        conn_tris = counted.tri_tests - counted.ext_tri_tests
+        conn_bytes = NODE_BYTES * conn_nodes + TRI_BYTES * conn_tris + RAY_BYTES * counted.shadow_rays
+        ext_s = float(np.mean(ext_ms)) / 1e3
+        achieved = ext_bytes / ext_s / 1e9
+        both = (ext_bytes + conn_bytes) / ((float(np.mean(ext_ms)) + float(np.mean(conn_ms))) / 1e3) / 1e9
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_extend (all bounces of one step)",
+            "bytes_per_step": int(ext_bytes),
+            "per_ray_bytes": round(ext_bytes / max(counted.extension_rays, 1), 1),
+            "extend_plus_connect_GBs": round(both, 1),
+        }
+        out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
+                              "rays_per_s": round((counted.extension_rays + counted.shadow_rays) /
+                                                  (ms_per_step / 1e3), 0)})
+    if not args.no_cpu_baseline and not args.profile_steps:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(scene, args)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * optixrender_amd -- C ABI of the MI355X path-tracing hot path.
+ *
+ * This is the drop-in boundary that replaces the reference renderer's OptiX
+ * path (GraphicsEnthusiast/OptiXRender).  Plain C types only: opaque context,
+ * plain pointers and sizes, int status codes (0 = OK, message through
+ * oxr_last_error).  The C++ facade in optixrender_amd/cpp/ (Renderer, Scene,
+ * SceneParser, CUDABuffer) is built on these entry points and keeps the
+ * reference's class interfaces.
+ *
+ * Threading: one context is driven by one host thread at a time.
+ */
+#ifndef OXR_H
+#define OXR_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OXR_OK 0
+#define OXR_ERR_INVALID 1
+#define OXR_ERR_HIP 2
+#define OXR_ERR_OOM 3
+#define OXR_ERR_STATE 4
+
+typedef struct oxr_ctx oxr_ctx;
+
+/* Material, one per surface slot (reference: Material, core/include/Scene.h:70-116,
+   plus the mesh's in/out medium from TriangleMeshSBTData, LaunchParams.h:16-26).
+   type: 0 Diffuse 1 Conductor 2 Dielectric 3 Plastic 4 MetalWorkflow
+         5 ThinDielectric 6 ClearCoatedConductor 7 DiffuseTransmitter (Scene.h:59-68).
+   tex[8]: texture ids (albedo, roughness, anisotropy, specular, metallic,
+   coat_roughness_u, coat_roughness_v, normal); -1 = none. */
+typedef struct oxr_material {
+    int32_t type;
+    float albedo[3];
+    float roughness;
+    float anisotropy;
+    float eta[3];
+    float k[3];
+    float int_ior, ext_ior;
+    int32_t nonlinear;
+    float specular[3];
+    float metallic;
+    float coat_weight;
+    float coat_roughness_u, coat_roughness_v;
+    int32_t in_medium, out_medium;
+    int32_t tex[8];
+} oxr_material;
+
+/* Analytic light (reference: Light, Scene.h:143-155). type 0 quad, 1 sphere. */
+typedef struct oxr_light {
+    int32_t type;
+    int32_t double_side;
+    float radiance[3];
+    float position[3];
+    float u[3];
+    float v[3];
+    float radius;
+    int32_t medium;
+} oxr_light;
+
+/* Participating medium (reference: Medium/Phase, Scene.h:33-55). */
+typedef struct oxr_medium {
+    int32_t type;        /* 0 homogeneous */
+    int32_t phase_type;  /* 0 isotropic, 1 Henyey-Greenstein */
+    float g[3];
+    float sigma_s[3];
+    float sigma_a[3];
+    float scale;
+} oxr_medium;
+
+/* Camera (reference: Camera, Scene.h:11-20; cosFovy fixed at 0.66 in
+   Renderer::SetCamera, Renderer.cpp:803 -- here a field, 0 means 0.66). */
+typedef struct oxr_camera {
+    float from[3], at[3], up[3];
+    int32_t medium;
+    float cos_fovy;
+} oxr_camera;
+
+/* Scene upload (replaces Scene + BuildAccel's per-mesh buffers,
+   Renderer.cpp:146-285).  Triangles are de-indexed as TriangleMesh::LoadMesh
+   leaves them (Scene.cpp:167-174): 9 floats of positions and 9 of normals per
+   triangle, one material slot per triangle.  uvs (6 floats per triangle) may
+   be NULL.  env_rgba: env_w*env_h RGBA32F, row 0 at v ~ 0 (stbi flipped load,
+   Scene.cpp:197-206).  Host pointers are only read during the call. */
+typedef struct oxr_scene_desc {
+    int64_t n_tris;
+    const float* positions;
+    const float* normals;
+    const float* uvs;
+    const uint32_t* mat_ids;
+    int32_t n_materials;
+    const oxr_material* materials;
+    int32_t n_lights;
+    const oxr_light* lights;
+    int32_t n_media;
+    const oxr_medium* media;
+    int32_t env_w, env_h;
+    const float* env_rgba;
+} oxr_scene_desc;
+
+#define OXR_FLAG_KEEP_PATH_RADIANCE 1u  /* keep per-(frame, pixel) radiance for oxr_download_path_radiance */
+#define OXR_FLAG_COUNT_TRAVERSAL 2u     /* count node fetches / triangle tests (slower; measurement only) */
+
+/* One render call = spp progressive frames of one sample each (reference
+   Render() with numPixelSamples = 1, Renderer.cpp:635-793). */
+typedef struct oxr_render_params {
+    int32_t spp;          /* frames rendered by this call (>= 1) */
+    int32_t max_depth;    /* maxBounce (LaunchParams.h:30); 0 means 25 */
+    int32_t rr_start;     /* first bounce that plays Russian roulette (reference: 0) */
+    int32_t progressive;  /* 1: continue frameID; 0: frameID restarts at 0 (Renderer.cpp:642-644) */
+    int32_t filter;       /* 0 box 1 tent 2 triangle 3 gaussian (reference default 3) */
+    int32_t denoise;      /* 0 plain accumulation */
+    uint32_t flags;       /* OXR_FLAG_* */
+    int32_t batch_frames; /* frames per wavefront batch, 0 = automatic */
+} oxr_render_params;
+
+typedef struct oxr_render_stats {
+    double render_ms;        /* whole oxr_render wall time on the device stream */
+    double extend_ms, shade_ms, connect_ms, generate_ms, accumulate_ms, tonemap_ms;
+    uint64_t extension_rays, shadow_rays;
+    uint64_t node_fetches, tri_tests;   /* all traversals; only with OXR_FLAG_COUNT_TRAVERSAL */
+    uint64_t ext_node_fetches, ext_tri_tests;  /* extension rays only; same condition */
+    uint64_t paths;
+    int32_t frame_id;                   /* next frameID */
+    int32_t kernel_launches;
+} oxr_render_stats;
+
+/* ---- context and memory (CUDABuffer-style API, common/CUDABuffer.h:26-81) ---- */
+int oxr_create(int device, oxr_ctx** out);
+int oxr_destroy(oxr_ctx* ctx);
+const char* oxr_last_error(const oxr_ctx* ctx);
+int oxr_malloc(oxr_ctx* ctx, size_t bytes, void** dptr);
+int oxr_free(oxr_ctx* ctx, void* dptr);
+int oxr_memcpy_h2d(oxr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int oxr_memcpy_d2h(oxr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int oxr_memcpy_d2d(oxr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int oxr_synchronize(oxr_ctx* ctx);
+
+/* ---- scenes (Scene / BuildAccel / CreateTextures / BuildSBT) ---- */
+int oxr_scene_upload(oxr_ctx* ctx, const oxr_scene_desc* desc, uint32_t* scene_id);
+/* LBVH build on the device (replaces optixAccelBuild/Compact, Renderer.cpp:247-274). */
+int oxr_scene_build_bvh(oxr_ctx* ctx, uint32_t scene_id, float* build_ms);
+int oxr_scene_release(oxr_ctx* ctx, uint32_t scene_id);
+int oxr_scene_select(oxr_ctx* ctx, uint32_t scene_id);
+/* BVH of a built scene in Karras layout: codes/order: n; left/right: n-1
+   (bit 31 = leaf, low bits = sorted position); lo/hi: 3*(2n-1) floats
+   (internal nodes then leaves). Any pointer may be NULL. */
+int oxr_scene_download_bvh(oxr_ctx* ctx, uint32_t scene_id, uint32_t* codes, uint32_t* order,
+                           uint32_t* left, uint32_t* right, float* lo, float* hi);
+/* Environment importance cache built on the device (HdrTexture::CalculateHdrCache,
+   Scene.cpp:217-313): env_w*env_h RGBA32F. */
+int oxr_scene_download_env_cache(oxr_ctx* ctx, uint32_t scene_id, float* cache_rgba);
+
+/* ---- ray queries against the selected scene (optixTrace, devicePrograms.cu:258-269,303-320) ----
+   rays: n*8 floats (ox oy oz tmin dx dy dz tmax), host memory.
+   closest: hits n*4 words (t, triangle id bits, u, v); id 0xffffffff = miss.
+   any: occluded n bytes.  counts (optional): n*2 uint32 (node records, triangle tests). */
+int oxr_trace_rays(oxr_ctx* ctx, const float* rays, int64_t n, int any_hit, float* hits,
+                   uint8_t* occluded, uint32_t* counts);
+
+/* ---- renderer (Renderer::Resize/SetCamera/Render/DownloadPixels, Renderer.h:13-25) ---- */
+int oxr_resize(oxr_ctx* ctx, int width, int height);
+int oxr_set_camera(oxr_ctx* ctx, const oxr_camera* cam);
+int oxr_render(oxr_ctx* ctx, const oxr_render_params* params);
+int oxr_get_stats(oxr_ctx* ctx, oxr_render_stats* out);
+int oxr_download_rgba8(oxr_ctx* ctx, uint32_t* out, size_t count);
+/* which: 0 colour, 1 albedo, 2 normal; count = width*height float4. */
+int oxr_download_aov(oxr_ctx* ctx, int which, float* out_rgba, size_t count);
+/* radiance of every path of the last render call: spp*width*height*3 floats,
+   frame-major.  Needs OXR_FLAG_KEEP_PATH_RADIANCE. */
+int oxr_download_path_radiance(oxr_ctx* ctx, float* out, size_t count);
+
+/* ---- building blocks exposed for parity tests ---- */
+/* ComputeFinalPixelColorsKernel (PostProcessing.cu:26-48) over n host float4. */
+int oxr_tonemap_host(oxr_ctx* ctx, const float* rgba, int64_t n, uint32_t* out);
+/* Kulla-Conty tables as uploaded (ComputeKullaConty, Microfacet.h:231-291). */
+int oxr_kulla_conty(float* bsdf_avg /* 64*64 */, float* albedo_avg /* 64 */);
+/* Batched EvaluateMaterial / SampleMaterial (Material.h:783-845) on the device.
+   in: n*12 floats (N xyz, V xyz, L xyz, front_face, rng_state bits, unused).
+   out: n*8 floats (f xyz, pdf, L xyz (sample), rng_state bits). mode 0 eval, 1 sample. */
+int oxr_material_batch(oxr_ctx* ctx, const oxr_material* m, int mode, const float* in, int64_t n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,769 @@
+// C ABI of the MI355X path tracer (include/oxr.h): context, device memory,
+// scene upload + LBVH build, the wavefront render loop, downloads and the
+// test entry points.  Host-side counterpart of the reference's Renderer
+// (core/src/Renderer.cpp) with OptiX replaced by the HIP kernels in
+// bvh.hip / render.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+using namespace oxr;
+
+namespace {
+
+// ------------------------------------------------------------ Kulla-Conty
+// ComputeKullaConty (core/cuda/Microfacet.h:220-291), run on the host at
+// context creation exactly as Renderer::BuildSBT does (Renderer.cpp:546-551).
+struct HV { float x, y, z; };
+inline HV hsub(HV a, HV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline HV hmul(HV a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline HV hmul(float s, HV a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float hdot(HV a, HV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline HV hnorm(HV v) { float s = sqrtf(hdot(v, v)); return {v.x * 1.f / s, v.y * 1.f / s, v.z * 1.f / s}; }
+inline HV hreflect(HV v, HV n) { return hnorm(hsub(v, hmul(2.0f * hdot(v, n), n))); }
+float hg1(HV V, HV H, HV N, float a) {  // GeometrySmith_1, isotropic branch (Microfacet.h:117-134)
+    float c = hdot(V, N);
+    if (c * hdot(V, H) <= 0.0f) return 0.0f;
+    if (fabsf(c - 1.0f) < 1e-4f) return 1.0f;
+    float c2 = c * c, t2 = (1.0f - c2) / c2, a2 = a * a;
+    return 2.0f / (1.0f + sqrtf(1.0f + a2 * t2));
+}
+HV hsample_ggx(float a, float s0, float s1) {  // SampleGGX, isotropic branch (Microfacet.h:166-186)
+    float phi = 6.28318530717958647692f * s1;
+    float cp = cosf(phi), sp = sinf(phi), a2 = a * a;
+    float t2 = a2 * s0 / (1.0f - s0), ct = 1.0f / sqrtf(1.0f + t2), st = sqrtf(1.0f - ct * ct);
+    return hnorm({st * cp, st * sp, ct});
+}
+void hammersley(uint32_t i, uint32_t n, float& a, float& b) {  // Microfacet.h:220-229
+    uint32_t v = (i << 16u) | (i >> 16u);
+    v = ((v & 0x55555555u) << 1u) | ((v & 0xAAAAAAAAu) >> 1u);
+    v = ((v & 0x33333333u) << 2u) | ((v & 0xCCCCCCCCu) >> 2u);
+    v = ((v & 0x0F0F0F0Fu) << 4u) | ((v & 0xF0F0F0F0u) >> 4u);
+    v = ((v & 0x00FF00FFu) << 8u) | ((v & 0xFF00FF00u) >> 8u);
+    a = (float)i / (float)n;
+    b = (float)v * 2.3283064365386963e-10f;
+}
+void compute_kulla_conty(float* bsdf, float* alb) {
+    const HV N{0.0f, 0.0f, 1.0f};
+    const int R = 64;
+    const float step = 1.0f / R, sstep = 1.0f / 128;
+    for (int i = R - 1; i >= 0; --i) {
+        float acc = 0.0f;
+        float r = step * ((float)i + 0.5f);
+        for (int j = R - 1; j >= 0; --j) {
+            float NdotV = step * ((float)j + 0.5f);
+            HV V{sqrtf(1.0f - NdotV * NdotV), 0.0f, NdotV};
+            float b = 0.0f;
+            for (uint32_t k = 0; k < 128; ++k) {
+                float s0, s1;
+                hammersley(k + 1, 129, s0, s1);
+                HV H = hsample_ggx(r, s0, s1);
+                HV L = hreflect(hmul(V, -1.0f), H);
+                float G = hg1(L, H, N, r) * hg1(V, H, N, r);
+                float nv = hdot(N, V), nl = hdot(N, L), nh = hdot(N, H), hv = hdot(H, V);
+                if (nl > 0.0f && nh > 0.0f && hv > 0.0f) b += (hv * G) / (nv * nh);
+            }
+            b = std::min(b * sstep, 1.0f);
+            bsdf[i * R + j] = b;
+            float a = 0.0f;
+            for (uint32_t k = 0; k < 128; ++k) {
+                float s0, s1;
+                hammersley(k + 1, 129, s0, s1);
+                HV H = hsample_ggx(r, s0, s1);
+                HV L = hreflect(hmul(V, -1.0f), H);
+                float hv = hdot(H, V), nl = hdot(N, L), nh = hdot(N, H);
+                if (nl > 0.0f && nh > 0.0f && hv > 0.0f) a += b * nl;
+            }
+            acc += a * 2.0f * sstep;
+        }
+        alb[i] = acc * step;
+    }
+}
+struct KcTable {
+    float bsdf[64 * 64];
+    float albedo[64];
+    KcTable() { compute_kulla_conty(bsdf, albedo); }
+};
+const KcTable& kc_table() {
+    static KcTable t;
+    return t;
+}
+
+// ------------------------------------------------------------ camera
+// Renderer::SetCamera (Renderer.cpp:796-813).
+struct CamBasis { float pos[3], dir[3], h[3], v[3]; };
+inline HV hcross(HV a, HV b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+CamBasis camera_basis(const oxr_camera& c, int w, int h) {
+    HV from{c.from[0], c.from[1], c.from[2]}, at{c.at[0], c.at[1], c.at[2]}, up{c.up[0], c.up[1], c.up[2]};
+    HV dir = hnorm(hsub(at, from));
+    float cf = c.cos_fovy > 0.0f ? c.cos_fovy : 0.66f;
+    float aspect = (float)w / (float)h;
+    HV hor = hmul(cf * aspect, hnorm(hcross(dir, up)));
+    HV ver = hmul(cf, hnorm(hcross(hor, dir)));
+    CamBasis b;
+    b.pos[0] = from.x; b.pos[1] = from.y; b.pos[2] = from.z;
+    b.dir[0] = dir.x; b.dir[1] = dir.y; b.dir[2] = dir.z;
+    b.h[0] = hor.x; b.h[1] = hor.y; b.h[2] = hor.z;
+    b.v[0] = ver.x; b.v[1] = ver.y; b.v[2] = ver.z;
+    return b;
+}
+
+struct SceneDev {
+    int64_t n = 0;
+    float* positions = nullptr;
+    float* normals = nullptr;
+    uint32_t* mat_ids = nullptr;
+    float* normals_sorted = nullptr;
+    uint32_t* mat_sorted = nullptr;
+    oxr_material* mats = nullptr;
+    int n_mats = 0;
+    oxr_light* lights = nullptr;
+    int n_lights = 0;
+    oxr_medium* media = nullptr;
+    int n_media = 0;
+    float4* env = nullptr;
+    float4* env_cache = nullptr;
+    int env_w = 0, env_h = 0;
+    BvhBuffers bvh;
+    bool built = false;
+    float build_ms = 0.0f;
+};
+
+}  // namespace
+
+struct oxr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    std::string err;
+    std::map<uint32_t, SceneDev*> scenes;
+    uint32_t next_id = 1;
+    uint32_t cur = 0;
+    float* lut_bsdf = nullptr;
+    float* lut_albedo = nullptr;
+    int width = 0, height = 0;
+    float4* color = nullptr;
+    float4* albedo = nullptr;
+    float4* normal = nullptr;
+    uint32_t* rgba = nullptr;
+    oxr_camera cam;
+    int frame_id = 0;
+    // wavefront buffers
+    int64_t cap = 0;
+    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
+    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr;
+    int* ctr = nullptr;
+    unsigned long long* trav = nullptr;
+    uint32_t* gstack = nullptr;
+    int* overflow = nullptr;
+    int trace_grid = 0, shade_grid = 0;
+    float* path_radiance = nullptr;
+    size_t path_radiance_floats = 0;
+    size_t path_radiance_valid = 0;
+    oxr_render_stats stats;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+int fail(oxr_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define OXR_HIP(ctx, expr)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? OXR_ERR_OOM : OXR_ERR_HIP,                     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                             \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_scene(SceneDev* s) {
+    dfree(s->positions); dfree(s->normals); dfree(s->mat_ids); dfree(s->normals_sorted); dfree(s->mat_sorted);
+    dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache);
+    dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
+    dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
+    delete s;
+}
+
+void free_wave(oxr_ctx* c) {
+    dfree(c->ray_o); dfree(c->ray_d); dfree(c->thr); dfree(c->rad); dfree(c->aov_alb); dfree(c->aov_nrm);
+    dfree(c->hits); dfree(c->sh_env); dfree(c->sh_light); dfree(c->q0); dfree(c->q1);
+    c->cap = 0;
+}
+
+int ensure_capacity(oxr_ctx* c, int64_t paths) {
+    if (paths <= c->cap) return OXR_OK;
+    free_wave(c);
+    OXR_HIP(c, dalloc(&c->ray_o, paths));
+    OXR_HIP(c, dalloc(&c->ray_d, paths));
+    OXR_HIP(c, dalloc(&c->thr, paths));
+    OXR_HIP(c, dalloc(&c->rad, paths));
+    OXR_HIP(c, dalloc(&c->aov_alb, paths));
+    OXR_HIP(c, dalloc(&c->aov_nrm, paths));
+    OXR_HIP(c, dalloc(&c->hits, paths));
+    OXR_HIP(c, dalloc(&c->sh_env, 3 * paths));
+    OXR_HIP(c, dalloc(&c->sh_light, 3 * paths));
+    OXR_HIP(c, dalloc(&c->q0, paths));
+    OXR_HIP(c, dalloc(&c->q1, paths));
+    c->cap = paths;
+    return OXR_OK;
+}
+
+SceneDev* selected(oxr_ctx* c) {
+    auto it = c->scenes.find(c->cur);
+    return it == c->scenes.end() ? nullptr : it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oxr_create(int device, oxr_ctx** out) {
+    if (!out) return OXR_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return OXR_ERR_HIP;
+    if (device < 0 || device >= n) return OXR_ERR_INVALID;
+    oxr_ctx* c = new oxr_ctx();
+    c->device = device;
+    std::memset(&c->cam, 0, sizeof(c->cam));
+    c->cam.at[2] = -1.0f;
+    c->cam.up[1] = 1.0f;
+    c->cam.medium = -1;
+    c->cam.cos_fovy = 0.66f;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    auto bail = [&](int code) { oxr_destroy(c); return code; };
+    if (hipSetDevice(device) != hipSuccess) return bail(OXR_ERR_HIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP);
+    const KcTable& t = kc_table();
+    if (dalloc(&c->lut_bsdf, 64 * 64) != hipSuccess || dalloc(&c->lut_albedo, 64) != hipSuccess) return bail(OXR_ERR_OOM);
+    if (hipMemcpy(c->lut_bsdf, t.bsdf, sizeof(t.bsdf), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->lut_albedo, t.albedo, sizeof(t.albedo), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(OXR_ERR_HIP);
+    c->trace_grid = c->num_cu * 4;
+    c->shade_grid = c->num_cu * 8;
+    if (dalloc(&c->ctr, kCtrCount) != hipSuccess || dalloc(&c->trav, 8) != hipSuccess ||
+        dalloc(&c->overflow, 4) != hipSuccess ||
+        dalloc(&c->gstack, (size_t)kStackGlobal * c->trace_grid * kTraceBlock) != hipSuccess)
+        return bail(OXR_ERR_OOM);
+    if (hipMemset(c->ctr, 0, kCtrCount * sizeof(int)) != hipSuccess ||
+        hipMemset(c->overflow, 0, 4 * sizeof(int)) != hipSuccess)
+        return bail(OXR_ERR_HIP);
+    c->events.resize(1024);
+    for (auto& e : c->events)
+        if (hipEventCreate(&e) != hipSuccess) return bail(OXR_ERR_HIP);
+    *out = c;
+    return OXR_OK;
+}
+
+int oxr_destroy(oxr_ctx* c) {
+    if (!c) return OXR_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->scenes) free_scene(kv.second);
+    c->scenes.clear();
+    free_wave(c);
+    dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
+    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance);
+    for (auto& e : c->events)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OXR_OK;
+}
+
+const char* oxr_last_error(const oxr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int oxr_malloc(oxr_ctx* c, size_t bytes, void** dptr) {
+    if (!c || !dptr) return OXR_ERR_INVALID;
+    *dptr = nullptr;
+    if (bytes == 0) return OXR_OK;
+    OXR_HIP(c, hipMalloc(dptr, bytes));
+    return OXR_OK;
+}
+int oxr_free(oxr_ctx* c, void* p) {
+    if (!c) return OXR_ERR_INVALID;
+    if (p) OXR_HIP(c, hipFree(p));
+    return OXR_OK;
+}
+int oxr_memcpy_h2d(oxr_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return OXR_ERR_INVALID;
+    if (bytes) OXR_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return OXR_OK;
+}
+int oxr_memcpy_d2h(oxr_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return OXR_ERR_INVALID;
+    if (bytes) OXR_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+int oxr_memcpy_d2d(oxr_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return OXR_ERR_INVALID;
+    if (bytes) OXR_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return OXR_OK;
+}
+int oxr_synchronize(oxr_ctx* c) {
+    if (!c) return OXR_ERR_INVALID;
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    return OXR_OK;
+}
+
+int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
+    if (!c || !d || !scene_id) return fail(c, OXR_ERR_INVALID, "null argument");
+    if (d->n_tris < 0 || d->n_tris > 0x7fffffffLL) return fail(c, OXR_ERR_INVALID, "n_tris out of range");
+    if (d->n_tris > 0 && (!d->positions || !d->normals || !d->mat_ids))
+        return fail(c, OXR_ERR_INVALID, "positions, normals and mat_ids are required");
+    if (d->n_materials < 0 || (d->n_materials > 0 && !d->materials)) return fail(c, OXR_ERR_INVALID, "materials");
+    if (d->n_tris > 0 && d->n_materials == 0) return fail(c, OXR_ERR_INVALID, "triangles need materials");
+    if (d->n_lights < 0 || (d->n_lights > 0 && !d->lights)) return fail(c, OXR_ERR_INVALID, "lights");
+    if (d->n_media < 0 || (d->n_media > 0 && !d->media)) return fail(c, OXR_ERR_INVALID, "media");
+    if ((d->env_w > 0) != (d->env_h > 0) || (d->env_w > 0 && !d->env_rgba) || d->env_w < 0)
+        return fail(c, OXR_ERR_INVALID, "environment map: width, height and data must all be given");
+    for (int64_t i = 0; i < d->n_tris; i++)
+        if (d->mat_ids[i] >= (uint32_t)d->n_materials)
+            return fail(c, OXR_ERR_INVALID, "mat_ids[" + std::to_string(i) + "] out of range");
+    for (int i = 0; i < d->n_materials; i++) {
+        const oxr_material& m = d->materials[i];
+        if (m.type < 0 || m.type > 7) return fail(c, OXR_ERR_INVALID, "material type out of range");
+        if (m.in_medium >= d->n_media || m.out_medium >= d->n_media || m.in_medium < -1 || m.out_medium < -1)
+            return fail(c, OXR_ERR_INVALID, "material medium index out of range");
+    }
+    for (int i = 0; i < d->n_lights; i++)
+        if (d->lights[i].medium >= d->n_media || d->lights[i].medium < -1)
+            return fail(c, OXR_ERR_INVALID, "light medium index out of range");
+    OXR_HIP(c, hipSetDevice(c->device));
+    SceneDev* s = new SceneDev();
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    const int64_t n = d->n_tris;
+    s->n = n;
+    s->n_mats = d->n_materials;
+    s->n_lights = d->n_lights;
+    s->n_media = d->n_media;
+    s->env_w = d->env_w;
+    s->env_h = d->env_h;
+    const size_t env_px = (size_t)d->env_w * d->env_h;
+    if ((e = dalloc(&s->positions, 9 * n)) == hipSuccess && (e = dalloc(&s->normals, 9 * n)) == hipSuccess &&
+        (e = dalloc(&s->mat_ids, n)) == hipSuccess && (e = dalloc(&s->mats, d->n_materials)) == hipSuccess &&
+        (e = dalloc(&s->lights, d->n_lights)) == hipSuccess && (e = dalloc(&s->media, d->n_media)) == hipSuccess &&
+        (e = dalloc(&s->env, env_px)) == hipSuccess && (e = dalloc(&s->env_cache, env_px)) == hipSuccess &&
+        (e = up(s->positions, d->positions, 36 * n)) == hipSuccess &&
+        (e = up(s->normals, d->normals, 36 * n)) == hipSuccess && (e = up(s->mat_ids, d->mat_ids, 4 * n)) == hipSuccess &&
+        (e = up(s->mats, d->materials, sizeof(oxr_material) * d->n_materials)) == hipSuccess &&
+        (e = up(s->lights, d->lights, sizeof(oxr_light) * d->n_lights)) == hipSuccess &&
+        (e = up(s->media, d->media, sizeof(oxr_medium) * d->n_media)) == hipSuccess &&
+        (e = up(s->env, d->env_rgba, 16 * env_px)) == hipSuccess) {
+    }
+    if (e != hipSuccess) {
+        free_scene(s);
+        return fail(c, e == hipErrorOutOfMemory ? OXR_ERR_OOM : OXR_ERR_HIP,
+                    std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    uint32_t id = c->next_id++;
+    c->scenes[id] = s;
+    if (c->cur == 0) c->cur = id;
+    *scene_id = id;
+    return OXR_OK;
+}
+
+int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
+    if (!c) return OXR_ERR_INVALID;
+    auto it = c->scenes.find(id);
+    if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    SceneDev* s = it->second;
+    OXR_HIP(c, hipSetDevice(c->device));
+    const int64_t n = s->n;
+    if (!s->built) {
+        BvhBuffers& b = s->bvh;
+        int64_t ni = n > 1 ? n - 1 : 0;
+        int64_t nodes = n > 0 ? 2 * n - 1 : 0;
+        OXR_HIP(c, dalloc(&b.codes, n));
+        OXR_HIP(c, dalloc(&b.order, n));
+        OXR_HIP(c, dalloc(&b.left, ni));
+        OXR_HIP(c, dalloc(&b.right, ni));
+        OXR_HIP(c, dalloc(&b.parent, nodes));
+        OXR_HIP(c, dalloc(&b.lo, 3 * nodes));
+        OXR_HIP(c, dalloc(&b.hi, 3 * nodes));
+        OXR_HIP(c, dalloc(&b.pairs, 4 * ni));
+        OXR_HIP(c, dalloc(&b.tris, 3 * n));
+        OXR_HIP(c, dalloc(&s->normals_sorted, 9 * n));
+        OXR_HIP(c, dalloc(&s->mat_sorted, n));
+    }
+    void* scratch = nullptr;
+    size_t sb = bvh_scratch_bytes(n);
+    size_t eb = s->env_w > 0 ? env_scratch_floats(s->env_w, s->env_h) * sizeof(float) : 0;
+    OXR_HIP(c, hipMalloc(&scratch, std::max<size_t>(std::max(sb, eb), 256)));
+    hipEvent_t e0 = c->events[0], e1 = c->events[1];
+    OXR_HIP(c, hipEventRecord(e0, c->stream));
+    hipError_t e = hipSuccess;
+    if (n > 0) {
+        e = bvh_build(s->positions, n, s->bvh, scratch, c->stream);
+        if (e == hipSuccess)
+            e = gather_sorted_attributes(s->normals, s->mat_ids, s->bvh.order, n, s->normals_sorted, s->mat_sorted,
+                                         c->stream);
+    }
+    OXR_HIP(c, hipEventRecord(e1, c->stream));
+    if (e == hipSuccess && s->env_w > 0)
+        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_cache, (float*)scratch, c->stream);
+    hipError_t es = hipStreamSynchronize(c->stream);
+    uint32_t bad = 0;
+    if (e == hipSuccess && es == hipSuccess && n > 0)
+        es = hipMemcpy(&bad, bvh_error_word(scratch, n), sizeof(bad), hipMemcpyDeviceToHost);
+    (void)hipFree(scratch);
+    if (bad) return fail(c, OXR_ERR_STATE, "bvh build produced a malformed hierarchy");
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("bvh build: ") + hipGetErrorString(e));
+    if (es != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("bvh build sync: ") + hipGetErrorString(es));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    s->build_ms = ms;
+    s->built = true;
+    if (build_ms) *build_ms = ms;
+    return OXR_OK;
+}
+
+int oxr_scene_release(oxr_ctx* c, uint32_t id) {
+    if (!c) return OXR_ERR_INVALID;
+    auto it = c->scenes.find(id);
+    if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    (void)hipStreamSynchronize(c->stream);
+    free_scene(it->second);
+    c->scenes.erase(it);
+    if (c->cur == id) c->cur = 0;
+    return OXR_OK;
+}
+
+int oxr_scene_select(oxr_ctx* c, uint32_t id) {
+    if (!c) return OXR_ERR_INVALID;
+    if (!c->scenes.count(id)) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    if (c->cur != id) c->frame_id = 0;
+    c->cur = id;
+    return OXR_OK;
+}
+
+int oxr_scene_download_bvh(oxr_ctx* c, uint32_t id, uint32_t* codes, uint32_t* order, uint32_t* left, uint32_t* right,
+                           float* lo, float* hi) {
+    if (!c) return OXR_ERR_INVALID;
+    auto it = c->scenes.find(id);
+    if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    SceneDev* s = it->second;
+    if (!s->built) return fail(c, OXR_ERR_STATE, "bvh not built");
+    const int64_t n = s->n, ni = n > 1 ? n - 1 : 0, nodes = n > 0 ? 2 * n - 1 : 0;
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    if (codes && n) OXR_HIP(c, hipMemcpy(codes, s->bvh.codes, 4 * n, hipMemcpyDeviceToHost));
+    if (order && n) OXR_HIP(c, hipMemcpy(order, s->bvh.order, 4 * n, hipMemcpyDeviceToHost));
+    if (left && ni) OXR_HIP(c, hipMemcpy(left, s->bvh.left, 4 * ni, hipMemcpyDeviceToHost));
+    if (right && ni) OXR_HIP(c, hipMemcpy(right, s->bvh.right, 4 * ni, hipMemcpyDeviceToHost));
+    if (lo && nodes) OXR_HIP(c, hipMemcpy(lo, s->bvh.lo, 12 * nodes, hipMemcpyDeviceToHost));
+    if (hi && nodes) OXR_HIP(c, hipMemcpy(hi, s->bvh.hi, 12 * nodes, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_scene_download_env_cache(oxr_ctx* c, uint32_t id, float* out) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    auto it = c->scenes.find(id);
+    if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    SceneDev* s = it->second;
+    if (!s->built) return fail(c, OXR_ERR_STATE, "scene not built");
+    if (s->env_w <= 0) return fail(c, OXR_ERR_STATE, "scene has no environment");
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    OXR_HIP(c, hipMemcpy(out, s->env_cache, 16 * (size_t)s->env_w * s->env_h, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_trace_rays(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float* hits, uint8_t* occ,
+                   uint32_t* counts) {
+    if (!c || (n > 0 && !rays) || n < 0) return fail(c, OXR_ERR_INVALID, "invalid rays");
+    if (n > 0 && any_hit && !occ) return fail(c, OXR_ERR_INVALID, "occluded output required");
+    if (n > 0 && !any_hit && !hits) return fail(c, OXR_ERR_INVALID, "hits output required");
+    SceneDev* s = selected(c);
+    if (!s || !s->built) return fail(c, OXR_ERR_STATE, "no built scene selected");
+    if (n == 0) return OXR_OK;
+    OXR_HIP(c, hipSetDevice(c->device));
+    float* dr = nullptr;
+    float4* dh = nullptr;
+    uint8_t* dOcc = nullptr;
+    uint32_t* dc = nullptr;
+    auto cleanup = [&] { dfree(dr); dfree(dh); dfree(dOcc); dfree(dc); };
+    hipError_t e;
+    if ((e = dalloc(&dr, 8 * n)) != hipSuccess || (e = dalloc(&dh, n)) != hipSuccess ||
+        (e = dalloc(&dOcc, n)) != hipSuccess || (e = dalloc(&dc, 2 * n)) != hipSuccess ||
+        (e = hipMemcpy(dr, rays, 32 * n, hipMemcpyHostToDevice)) != hipSuccess) {
+        cleanup();
+        return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
+    }
+    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
+    e = trace_rays(bv, dr, n, any_hit, dh, dOcc, dc, c->gstack, c->trace_grid, c->overflow, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && !any_hit) e = hipMemcpy(hits, dh, 16 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && any_hit) e = hipMemcpy(occ, dOcc, n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && counts) e = hipMemcpy(counts, dc, 8 * n, hipMemcpyDeviceToHost);
+    int ov = 0;
+    if (e == hipSuccess) e = hipMemcpy(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
+    if (ov) return fail(c, OXR_ERR_STATE, ov == 2 ? "traversal visited more nodes than the tree holds"
+                                                  : "traversal stack overflow");
+    return OXR_OK;
+}
+
+int oxr_resize(oxr_ctx* c, int w, int h) {
+    if (!c || w <= 0 || h <= 0 || (int64_t)w * h > (1LL << 30)) return fail(c, OXR_ERR_INVALID, "invalid size");
+    OXR_HIP(c, hipSetDevice(c->device));
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
+    const size_t np = (size_t)w * h;
+    OXR_HIP(c, dalloc(&c->color, np));
+    OXR_HIP(c, dalloc(&c->albedo, np));
+    OXR_HIP(c, dalloc(&c->normal, np));
+    OXR_HIP(c, dalloc(&c->rgba, np));
+    OXR_HIP(c, hipMemset(c->color, 0, 16 * np));
+    OXR_HIP(c, hipMemset(c->albedo, 0, 16 * np));
+    OXR_HIP(c, hipMemset(c->normal, 0, 16 * np));
+    OXR_HIP(c, hipMemset(c->rgba, 0, 4 * np));
+    c->width = w;
+    c->height = h;
+    c->frame_id = 0;  // Resize calls SetCamera, which resets accumulation (Renderer.cpp:866, :799)
+    c->path_radiance_valid = 0;
+    return OXR_OK;
+}
+
+int oxr_set_camera(oxr_ctx* c, const oxr_camera* cam) {
+    if (!c || !cam) return OXR_ERR_INVALID;
+    c->cam = *cam;
+    c->frame_id = 0;  // Renderer.cpp:799
+    return OXR_OK;
+}
+
+int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
+    if (!c || !prm) return OXR_ERR_INVALID;
+    if (c->width == 0) return fail(c, OXR_ERR_STATE, "resize before render");
+    SceneDev* s = selected(c);
+    if (!s || !s->built) return fail(c, OXR_ERR_STATE, "no built scene selected");
+    if (prm->spp < 1) return fail(c, OXR_ERR_INVALID, "spp must be >= 1");
+    if (prm->filter < 0 || prm->filter > 3) return fail(c, OXR_ERR_INVALID, "filter out of range");
+    if (prm->max_depth < 0) return fail(c, OXR_ERR_INVALID, "max_depth must be >= 0");
+    if (prm->denoise != 0) return fail(c, OXR_ERR_INVALID, "denoise needs the denoiser entry points");
+    if (c->cam.medium < -1 || c->cam.medium >= s->n_media) return fail(c, OXR_ERR_INVALID, "camera medium out of range");
+    OXR_HIP(c, hipSetDevice(c->device));
+    if (!prm->progressive) c->frame_id = 0;  // Renderer.cpp:642-644
+    const int W = c->width, H = c->height;
+    const int64_t npix = (int64_t)W * H;
+    int F = prm->batch_frames > 0 ? prm->batch_frames : (int)std::max<int64_t>(1, (int64_t)(1 << 24) / npix);
+    F = std::min(F, prm->spp);
+    int rc = ensure_capacity(c, (int64_t)F * npix);
+    if (rc != OXR_OK) return rc;
+    const bool keep = (prm->flags & OXR_FLAG_KEEP_PATH_RADIANCE) != 0;
+    const bool count = (prm->flags & OXR_FLAG_COUNT_TRAVERSAL) != 0;
+    if (keep) {
+        size_t need = (size_t)3 * prm->spp * npix;
+        if (need > c->path_radiance_floats) {
+            dfree(c->path_radiance);
+            OXR_HIP(c, dalloc(&c->path_radiance, need));
+            c->path_radiance_floats = need;
+        }
+    }
+    CamBasis cb = camera_basis(c->cam, W, H);
+    WaveParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
+    p.normals = s->normals_sorted;
+    p.mat_ids = s->mat_sorted;
+    p.mats = s->mats;
+    p.lights = s->lights;
+    p.n_lights = s->n_lights;
+    p.media = s->media;
+    p.n_media = s->n_media;
+    p.env = s->env;
+    p.env_cache = s->env_cache;
+    p.env_w = s->env_w;
+    p.env_h = s->env_h;
+    p.lut_bsdf = c->lut_bsdf;
+    p.lut_albedo = c->lut_albedo;
+    std::memcpy(p.cam_pos, cb.pos, 12);
+    std::memcpy(p.cam_dir, cb.dir, 12);
+    std::memcpy(p.cam_h, cb.h, 12);
+    std::memcpy(p.cam_v, cb.v, 12);
+    p.cam_medium = c->cam.medium;
+    p.width = W;
+    p.height = H;
+    p.filter = prm->filter;
+    p.max_depth = prm->max_depth > 0 ? prm->max_depth : 25;
+    p.rr_start = prm->rr_start;
+    p.ray_o = c->ray_o; p.ray_d = c->ray_d; p.thr = c->thr; p.rad = c->rad;
+    p.aov_alb = c->aov_alb; p.aov_nrm = c->aov_nrm;
+    p.queue[0] = c->q0; p.queue[1] = c->q1;
+    p.hits = c->hits; p.sh_env = c->sh_env; p.sh_light = c->sh_light;
+    p.ctr = c->ctr; p.traversal = c->trav; p.gstack = c->gstack; p.overflow = c->overflow;
+    p.color = c->color; p.albedo = c->albedo; p.normal = c->normal;
+    p.path_radiance = keep ? c->path_radiance : nullptr;
+
+    OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
+    // stage timing: one event between consecutive launches
+    std::vector<int> stage;   // 0 generate 1 extend 2 shade 3 connect 4 accumulate 5 tonemap
+    size_t ev = 0;
+    auto mark = [&](int st) -> hipError_t {
+        if (ev + 1 >= c->events.size()) return hipSuccess;
+        stage.push_back(st);
+        return hipEventRecord(c->events[++ev], c->stream);
+    };
+    OXR_HIP(c, hipEventRecord(c->events[0], c->stream));
+    int launches = 0;
+    for (int f0 = 0; f0 < prm->spp; f0 += F) {
+        int nf = std::min(F, prm->spp - f0);
+        int frame0 = c->frame_id + f0;
+        OXR_HIP(c, wave_generate(p, frame0, nf, c->stream));
+        OXR_HIP(c, mark(0));
+        launches++;
+        for (int b = 0; b < p.max_depth; b++) {
+            OXR_HIP(c, wave_extend(p, b, c->trace_grid, count, c->stream));
+            OXR_HIP(c, mark(1));
+            OXR_HIP(c, wave_shade(p, b, c->shade_grid, c->stream));
+            OXR_HIP(c, mark(2));
+            OXR_HIP(c, wave_connect(p, 0, c->trace_grid, count, c->stream));
+            OXR_HIP(c, wave_connect(p, 1, c->trace_grid, count, c->stream));
+            OXR_HIP(c, mark(3));
+            launches += 4;
+        }
+        OXR_HIP(c, wave_accumulate(p, frame0, nf, f0, c->stream));
+        OXR_HIP(c, mark(4));
+        launches++;
+    }
+    OXR_HIP(c, tonemap(c->color, c->rgba, npix, c->stream));
+    OXR_HIP(c, mark(5));
+    launches++;
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    unsigned long long tv[8];
+    OXR_HIP(c, hipMemcpy(tv, c->trav, sizeof(tv), hipMemcpyDeviceToHost));
+    int ov = 0;
+    OXR_HIP(c, hipMemcpy(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost));
+    oxr_render_stats& st = c->stats;
+    std::memset(&st, 0, sizeof(st));
+    float total = 0.0f;
+    (void)hipEventElapsedTime(&total, c->events[0], c->events[ev]);
+    st.render_ms = total;
+    for (size_t i = 0; i < stage.size(); i++) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->events[i], c->events[i + 1]);
+        double* dst[] = {&st.generate_ms, &st.extend_ms, &st.shade_ms, &st.connect_ms, &st.accumulate_ms, &st.tonemap_ms};
+        *dst[stage[i]] += ms;
+    }
+    st.ext_node_fetches = tv[0];
+    st.ext_tri_tests = tv[1];
+    st.node_fetches = tv[0] + tv[4];
+    st.tri_tests = tv[1] + tv[5];
+    st.extension_rays = tv[2];
+    st.shadow_rays = tv[3];
+    st.paths = (uint64_t)prm->spp * npix;
+    st.kernel_launches = launches;
+    c->frame_id += prm->spp;
+    st.frame_id = c->frame_id;
+    c->path_radiance_valid = keep ? (size_t)3 * prm->spp * npix : 0;
+    if (ov) return fail(c, OXR_ERR_STATE, "traversal stack overflow");
+    return OXR_OK;
+}
+
+int oxr_get_stats(oxr_ctx* c, oxr_render_stats* out) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    *out = c->stats;
+    return OXR_OK;
+}
+
+int oxr_download_rgba8(oxr_ctx* c, uint32_t* out, size_t count) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    if (count != (size_t)c->width * c->height) return fail(c, OXR_ERR_INVALID, "count must equal width*height");
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    OXR_HIP(c, hipMemcpy(out, c->rgba, 4 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_download_aov(oxr_ctx* c, int which, float* out, size_t count) {
+    if (!c || !out || which < 0 || which > 2) return OXR_ERR_INVALID;
+    if (count != (size_t)c->width * c->height) return fail(c, OXR_ERR_INVALID, "count must equal width*height");
+    float4* src = which == 0 ? c->color : (which == 1 ? c->albedo : c->normal);
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    OXR_HIP(c, hipMemcpy(out, src, 16 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_download_path_radiance(oxr_ctx* c, float* out, size_t count) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    if (c->path_radiance_valid == 0) return fail(c, OXR_ERR_STATE, "last render did not keep path radiance");
+    if (count != c->path_radiance_valid) return fail(c, OXR_ERR_INVALID, "count must equal spp*width*height*3");
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    OXR_HIP(c, hipMemcpy(out, c->path_radiance, 4 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_tonemap_host(oxr_ctx* c, const float* rgba, int64_t n, uint32_t* out) {
+    if (!c || n < 0 || (n > 0 && (!rgba || !out))) return OXR_ERR_INVALID;
+    if (n == 0) return OXR_OK;
+    float4* di = nullptr;
+    uint32_t* dout = nullptr;
+    hipError_t e;
+    if ((e = dalloc(&di, n)) == hipSuccess && (e = dalloc(&dout, n)) == hipSuccess &&
+        (e = hipMemcpy(di, rgba, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = tonemap(di, dout, n, c->stream)) == hipSuccess && (e = hipStreamSynchronize(c->stream)) == hipSuccess)
+        e = hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost);
+    dfree(di);
+    dfree(dout);
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("tonemap: ") + hipGetErrorString(e));
+    return OXR_OK;
+}
+
+int oxr_kulla_conty(float* bsdf_avg, float* albedo_avg) {
+    if (!bsdf_avg || !albedo_avg) return OXR_ERR_INVALID;
+    const KcTable& t = kc_table();
+    std::memcpy(bsdf_avg, t.bsdf, sizeof(t.bsdf));
+    std::memcpy(albedo_avg, t.albedo, sizeof(t.albedo));
+    return OXR_OK;
+}
+
+int oxr_material_batch(oxr_ctx* c, const oxr_material* m, int mode, const float* in, int64_t n, float* out) {
+    if (!c || !m || n < 0 || (n > 0 && (!in || !out)) || mode < 0 || mode > 1) return OXR_ERR_INVALID;
+    if (m->type < 0 || m->type > 7) return fail(c, OXR_ERR_INVALID, "material type out of range");
+    if (n == 0) return OXR_OK;
+    oxr_material* dm = nullptr;
+    float *di = nullptr, *dout = nullptr;
+    hipError_t e;
+    if ((e = dalloc(&dm, 1)) == hipSuccess && (e = dalloc(&di, 12 * n)) == hipSuccess &&
+        (e = dalloc(&dout, 8 * n)) == hipSuccess &&
+        (e = hipMemcpy(dm, m, sizeof(oxr_material), hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = hipMemcpy(di, in, 48 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = material_batch(dm, c->lut_bsdf, c->lut_albedo, mode, di, n, dout, c->stream)) == hipSuccess &&
+        (e = hipStreamSynchronize(c->stream)) == hipSuccess)
+        e = hipMemcpy(out, dout, 32 * n, hipMemcpyDeviceToHost);
+    dfree(dm);
+    dfree(di);
+    dfree(dout);
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("material batch: ") + hipGetErrorString(e));
+    return OXR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,357 @@
+// LBVH build on gfx950 (replaces optixAccelBuild/optixAccelCompact,
+// core/src/Renderer.cpp:146-285).
+//   1. centroid bounds: block min/max reduction, order-preserving integer
+//      atomics (exact).
+//   2. 30-bit Morton codes of triangle centroids ((v0+v1+v2)*(1/3)),
+//      quantised as min(max(n*1024, 0), 1023) per axis.
+//   3. LSD radix sort of (code, triangle id) in 4 passes of 8 bits: per-tile
+//      LDS histograms, one scan, stable scatter ranked with wave64 ballots.
+//   4. Karras 2012 hierarchy, one thread per internal node; equal codes are
+//      split on the sorted position (augmented key).
+//   5. bottom-up AABB refit with per-node arrival counters; each finished box
+//      is also written into its parent's 64-byte child-pair record used by
+//      traversal.
+#include <algorithm>
+#include "common.h"
+
+namespace oxr {
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;  // keys per block
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+__device__ __forceinline__ uint32_t float_to_ordered(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ordered_to_float(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ __forceinline__ void centroid(const float* __restrict__ p, float c[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) c[a] = ((p[a] + p[3 + a]) + p[6 + a]) * (1.0f / 3.0f);
+}
+
+__global__ void k_init_bounds(uint32_t* b) {
+    if (threadIdx.x < 3) b[threadIdx.x] = 0xffffffffu;      // min (ordered)
+    else if (threadIdx.x < 8) b[threadIdx.x] = 0u;          // max (ordered), error word, pad
+}
+
+__global__ __launch_bounds__(256) void k_centroid_bounds(const float* __restrict__ pos, int64_t n, uint32_t* bounds) {
+    __shared__ uint32_t smin[3][256], smax[3][256];
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float c[3];
+        centroid(pos + 9 * i, c);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            uint32_t o = float_to_ordered(c[a]);
+            lo[a] = min(lo[a], o);
+            hi[a] = max(hi[a], o);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) { smin[a][threadIdx.x] = lo[a]; smax[a][threadIdx.x] = hi[a]; }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                smin[a][threadIdx.x] = min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+                smax[a][threadIdx.x] = max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) atomicMin(&bounds[threadIdx.x], smin[threadIdx.x][0]);
+    else if (threadIdx.x < 6) atomicMax(&bounds[threadIdx.x], smax[threadIdx.x - 3][0]);
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quantize(float c, float lo, float ext) {
+    float nrm = ext > 0.0f ? (c - lo) / ext : 0.0f;
+    float q = fminf(fmaxf(nrm * 1024.0f, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ pos, int64_t n, const uint32_t* bounds,
+                                                 uint32_t* keys, uint32_t* vals) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], ext[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = ordered_to_float(bounds[a]);
+        ext[a] = ordered_to_float(bounds[3 + a]) - lo[a];
+    }
+    float c[3];
+    centroid(pos + 9 * i, c);
+    keys[i] = (expand10(quantize(c[0], lo[0], ext[0])) << 2) | (expand10(quantize(c[1], lo[1], ext[1])) << 1) |
+              expand10(quantize(c[2], lo[2], ext[2]));
+    vals[i] = (uint32_t)i;
+}
+
+// ---- radix sort ----
+// hist layout: [digit][block] so that one exclusive scan gives every
+// (digit, block) its global output offset.
+__global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                            uint32_t* hist, int nblocks) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        int64_t i = base + j * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m uint32 in place by one block of 1024 threads:
+// each thread owns a contiguous chunk.
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, int64_t m) {
+    __shared__ uint32_t part[1024];
+    int64_t chunk = (m + 1023) / 1024;
+    int64_t b = threadIdx.x * chunk, e = min(b + chunk, m);
+    uint32_t s = 0;
+    for (int64_t i = b; i < e; i++) s += data[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (int64_t i = b; i < e; i++) {
+        uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin, int64_t n, int shift,
+                                                               const uint32_t* __restrict__ offs, int nblocks,
+                                                               uint32_t* kout, uint32_t* vout) {
+    constexpr int kWaves = kSortBlock / 64;
+    __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wcount[kWaves][kRadix];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    base[threadIdx.x] = offs[(size_t)threadIdx.x * nblocks + blockIdx.x];
+    run[threadIdx.x] = 0;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t tile = (int64_t)blockIdx.x * kSortTile;
+    for (int j = 0; j < kSortItems; j++) {
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) wcount[w][threadIdx.x] = 0;
+        __syncthreads();
+        int64_t i = tile + j * kSortBlock + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+        uint32_t d = (key >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; b++) {
+            uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        uint32_t lrank = __popcll(peers & lt);
+        if (valid && lrank == 0) wcount[wave][d] = __popcll(peers);
+        __syncthreads();
+        uint32_t rank = 0;
+        if (valid) {
+            uint32_t before = run[d];
+            for (int w = 0; w < wave; w++) before += wcount[w][d];
+            rank = before + lrank;
+        }
+        __syncthreads();
+        {
+            uint32_t tot = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) tot += wcount[w][threadIdx.x];
+            run[threadIdx.x] += tot;
+        }
+        if (valid) {
+            uint32_t dst = base[d] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- Karras hierarchy ----
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ c, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t ci = c[i], cj = c[j];
+    if (ci != cj) return __clz((int)(ci ^ cj));
+    return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, int64_t n, uint32_t* left,
+                                                 uint32_t* right, uint32_t* parent) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = n - 1;
+    if (i >= ni) return;
+    int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(codes, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = delta(codes, n, i, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + min(d, 0);
+    uint32_t L = (min(i, j) == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t R = (max(i, j) == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    left[i] = L;
+    right[i] = R;
+    parent[(L & kLeafBit) ? ni + (L & ~kLeafBit) : L] = (uint32_t)i;
+    parent[(R & kLeafBit) ? ni + (R & ~kLeafBit) : R] = (uint32_t)i;
+}
+
+__device__ __forceinline__ void write_record(float4* pairs, const uint32_t* left, uint32_t parent, uint32_t ref,
+                                             const float lo[3], const float hi[3]) {
+    int side = (left[parent] == ref) ? 0 : 1;
+    float4* r = pairs + 4 * (size_t)parent + 2 * side;
+    r[0] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(ref));
+    r[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
+// One thread per leaf: leaf box, sorted triangle record, then climb while
+// this thread is the second to reach a node.
+__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, int64_t n,
+                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ left,
+                                                const uint32_t* __restrict__ right, const uint32_t* __restrict__ parent,
+                                                uint32_t* flags, float* lo, float* hi, float4* pairs, float4* tris,
+                                                uint32_t* error) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t ni = n - 1;
+    uint32_t tid = order[k];
+    const float* p = pos + 9 * (size_t)tid;
+    float bl[3], bh[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        bl[a] = fminf(fminf(p[a], p[3 + a]), p[6 + a]);
+        bh[a] = fmaxf(fmaxf(p[a], p[3 + a]), p[6 + a]);
+    }
+    tris[3 * k + 0] = make_float4(p[0], p[1], p[2], __uint_as_float(tid));
+    tris[3 * k + 1] = make_float4(p[3], p[4], p[5], 0.0f);
+    tris[3 * k + 2] = make_float4(p[6], p[7], p[8], 0.0f);
+    int64_t node = ni + k;
+#pragma unroll
+    for (int a = 0; a < 3; a++) { lo[3 * node + a] = bl[a]; hi[3 * node + a] = bh[a]; }
+    if (ni == 0) return;
+    uint32_t ref = kLeafBit | (uint32_t)k;
+    uint32_t par = parent[node];
+    write_record(pairs, left, par, ref, bl, bh);
+    for (int depth = 0;; depth++) {
+        if (depth > 256 || par >= (uint32_t)ni) {  // LBVH depth <= 30 + log2(n): a malformed tree
+            atomicOr(error, 1u);
+            return;
+        }
+        __threadfence();
+        if (atomicAdd(&flags[par], 1u) == 0u) return;  // sibling not finished yet
+        __threadfence();
+        uint32_t cl = left[par], cr = right[par];
+        int64_t il = (cl & kLeafBit) ? ni + (cl & ~kLeafBit) : cl;
+        int64_t ir = (cr & kLeafBit) ? ni + (cr & ~kLeafBit) : cr;
+        float nl[3], nh[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float l0 = __hip_atomic_load(&lo[3 * il + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            float l1 = __hip_atomic_load(&lo[3 * ir + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            float h0 = __hip_atomic_load(&hi[3 * il + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            float h1 = __hip_atomic_load(&hi[3 * ir + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            nl[a] = fminf(l0, l1);
+            nh[a] = fmaxf(h0, h1);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            __hip_atomic_store(&lo[3 * (size_t)par + a], nl[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hi[3 * (size_t)par + a], nh[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (par == 0) return;
+        uint32_t gp = parent[par];
+        write_record(pairs, left, gp, par, nl, nh);
+        par = gp;
+    }
+}
+
+inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+uint32_t* bvh_error_word(void* scratch, int64_t n) {
+    int64_t nb = (n + kSortTile - 1) / kSortTile;
+    return static_cast<uint32_t*>(scratch) + 4 * n + (size_t)kRadix * nb + 6;
+}
+
+size_t bvh_scratch_bytes(int64_t n) {
+    int64_t nb = (n + kSortTile - 1) / kSortTile;
+    // keys/vals ping-pong (4n words) + histogram + bounds + flags
+    return (size_t)(4 * n + (int64_t)kRadix * nb + 8 + n + 64) * sizeof(uint32_t);
+}
+
+hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    uint32_t* w = static_cast<uint32_t*>(scratch);
+    uint32_t* k0 = w;
+    uint32_t* v0 = k0 + n;
+    uint32_t* k1 = v0 + n;
+    uint32_t* v1 = k1 + n;
+    int nb = (int)((n + kSortTile - 1) / kSortTile);
+    uint32_t* hist = v1 + n;
+    uint32_t* bounds = hist + (size_t)kRadix * nb;
+    uint32_t* flags = bounds + 8;
+
+    k_init_bounds<<<1, 64, 0, s>>>(bounds);
+    int gb = std::min(blocks_for(n, 256), 4096);
+    k_centroid_bounds<<<gb, 256, 0, s>>>(pos, n, bounds);
+    k_morton<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, bounds, k0, v0);
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (int shift = 0; shift < 30; shift += kRadixBits) {
+        k_radix_hist<<<nb, kSortBlock, 0, s>>>(ki, n, shift, hist, nb);
+        k_scan_single<<<1, 1024, 0, s>>>(hist, (int64_t)kRadix * nb);
+        k_radix_scatter<<<nb, kSortBlock, 0, s>>>(ki, vi, n, shift, hist, nb, ko, vo);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    hipError_t e = hipMemcpyAsync(o.codes, ki, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(o.order, vi, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    if (n > 1) {
+        k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(o.codes, n, o.left, o.right, o.parent);
+        e = hipMemsetAsync(flags, 0, (size_t)(n - 1) * sizeof(uint32_t), s);
+        if (e != hipSuccess) return e;
+    }
+    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.left, o.right, o.parent, flags, o.lo, o.hi,
+                                               o.pairs, o.tris, bounds + 6);
+    return hipGetLastError();
+}
+
+}  // namespace oxr

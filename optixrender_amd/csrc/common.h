@@ -1,0 +1,125 @@
+// Shared host/device declarations of the HIP library: device scene layout,
+// launch helpers and the kernel launchers called by the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/oxr.h"
+
+namespace oxr {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kNoHit = 0xffffffffu;
+constexpr int kStackLds = 32;     // traversal stack entries held in LDS per thread
+constexpr int kStackGlobal = 32;  // further entries spilled to HBM (LBVH depth <= 30 + log2 n < 64)
+constexpr int kTraceBlock = 256;
+
+// BVH in traversal layout.  pairs: for internal node i, the two children
+// records at pairs[4i .. 4i+3]: (lo.xyz, ref), (hi.xyz, 0) for the left
+// child then the right child; ref = internal id, or kLeafBit | sorted leaf.
+// tris: 3 float4 per sorted leaf: (v0, triangle id bits), (v1, 0), (v2, 0).
+struct BvhView {
+    const float4* pairs;
+    const float4* tris;
+    int64_t n;
+};
+
+struct BvhBuffers {
+    // Karras layout kept for parity checks and refits
+    uint32_t* codes = nullptr;     // sorted codes, n
+    uint32_t* order = nullptr;     // sorted position -> triangle id, n
+    uint32_t* left = nullptr;      // n-1
+    uint32_t* right = nullptr;     // n-1
+    uint32_t* parent = nullptr;    // 2n-1 (internal, then leaves)
+    float* lo = nullptr;           // 3*(2n-1)
+    float* hi = nullptr;           // 3*(2n-1)
+    float4* pairs = nullptr;       // 4*(n-1)
+    float4* tris = nullptr;        // 3n
+};
+
+struct Counters64 {
+    unsigned long long nodes, tris, rays;
+};
+
+// ---- launchers (bvh.hip) ----
+// Builds all BvhBuffers from de-indexed positions (device, 9 floats/triangle).
+// scratch must hold bvh_scratch_bytes(n).
+size_t bvh_scratch_bytes(int64_t n);
+// Device word set non-zero by the build if the hierarchy came out malformed.
+uint32_t* bvh_error_word(void* d_scratch, int64_t n);
+hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void* d_scratch, hipStream_t s);
+
+// ---- launchers (render.hip) ----
+// gstack: kStackGlobal * grid * kTraceBlock words; overflow: one int set to
+// 1 if a traversal ever needed more than kStackLds + kStackGlobal entries.
+hipError_t trace_rays(const BvhView& b, const float* d_rays, int64_t n, int any, float4* d_hits, uint8_t* d_occ,
+                      uint32_t* d_counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s);
+hipError_t env_cache_build(const float4* d_env, int w, int h, float4* d_cache, float* d_scratch, hipStream_t s);
+size_t env_scratch_floats(int w, int h);
+hipError_t tonemap(const float4* d_in, uint32_t* d_out, int64_t n, hipStream_t s);
+hipError_t material_batch(const oxr_material* d_mat, const float* d_lut_bsdf, const float* d_lut_albedo, int mode,
+                          const float* d_in, int64_t n, float* d_out, hipStream_t s);
+hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_mat_ids, const uint32_t* d_order,
+                                    int64_t n, float* d_normals_sorted, uint32_t* d_mat_sorted, hipStream_t s);
+
+// Wavefront integrator state for one context.
+struct WaveParams {
+    // scene
+    BvhView bvh;
+    const float* normals;          // 9 floats per sorted triangle
+    const uint32_t* mat_ids;       // per sorted triangle
+    const oxr_material* mats;
+    const oxr_light* lights;
+    int n_lights;
+    const oxr_medium* media;
+    int n_media;
+    const float4* env;
+    const float4* env_cache;
+    int env_w, env_h;
+    const float* lut_bsdf;
+    const float* lut_albedo;
+    // camera (Renderer::SetCamera, Renderer.cpp:796-813)
+    float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
+    int cam_medium;
+    int width, height;
+    int filter, max_depth, rr_start;
+    // path state, capacity paths
+    float4* ray_o;     // origin, pre_pdf
+    float4* ray_d;     // direction, rng state bits
+    float4* thr;       // history (throughput)
+    float4* rad;       // radiance
+    float4* aov_alb;
+    float4* aov_nrm;
+    uint32_t* queue[2];
+    float4* hits;
+    float4* sh_env;    // 3 float4 per shadow ray
+    float4* sh_light;
+    int* ctr;          // device counters
+    unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
+    uint32_t* gstack;               // traversal stack spill, kStackGlobal * grid * kTraceBlock
+    int* overflow;
+    // accumulation targets
+    float4* color;
+    float4* albedo;
+    float4* normal;
+    float* path_radiance;  // optional, frame-major 3 floats per path
+};
+
+enum CtrSlot {
+    kCtrQueue0 = 0,
+    kCtrQueue1 = 1,
+    kCtrShEnv = 2,
+    kCtrShLight = 3,
+    kCtrFetchExtend = 4,
+    kCtrFetchShade = 5,
+    kCtrFetchConnEnv = 6,
+    kCtrFetchConnLight = 7,
+    kCtrCount = 16
+};
+
+hipError_t wave_generate(const WaveParams& p, int frame0, int frames, hipStream_t s);
+hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s);
+hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s);
+hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
+hipError_t wave_accumulate(const WaveParams& p, int frame0, int frames, int path_frame_offset, hipStream_t s);
+
+}  // namespace oxr

@@ -1,0 +1,672 @@
+// Wavefront path tracer on gfx950: generate -> [extend -> shade -> connect]
+// x depth -> accumulate, plus the environment importance cache, the tone map
+// and the test entry points.  Replaces __raygen__renderFrame and its
+// closest-hit/miss programs (core/cuda/devicePrograms.cu:68-532) and
+// ComputeFinalPixelColorsKernel (core/cuda/PostProcessing.cu:26-48).
+//
+// A path is (pixel, frame): RNG init(ix + W*iy, frameID), one sample per
+// frame -- the reference with numPixelSamples = 1 in progressive mode.  Path
+// state is SoA in HBM (64 B/path: origin+pre_pdf, direction+rng, throughput,
+// radiance); queues hold path ids and shrink as paths end.  The stage
+// kernels are persistent: each wave grabs 64 queue slots at a time from an
+// atomic counter, so one fixed launch shape serves every bounce.
+#include <algorithm>
+#include "common.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace oxr {
+namespace {
+
+OXR_D int lane_id() { return (int)(threadIdx.x & 63); }
+OXR_D uint64_t lanes_below() { int l = lane_id(); return l == 0 ? 0ull : (~0ull >> (64 - l)); }
+
+// A wave takes the next 64 slots of a queue.
+OXR_D int wave_fetch(int* ctr) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, 64);
+    return __shfl(base, 0);
+}
+
+// Wave-aggregated append: returns this lane's slot or -1.
+OXR_D int wave_append(int* ctr, bool want) {
+    uint64_t m = __ballot(want);
+    if (m == 0) return -1;
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, __popcll(m));
+    base = __shfl(base, 0);
+    return want ? base + (int)__popcll(m & lanes_below()) : -1;
+}
+
+OXR_D void wave_count_add(unsigned long long* dst, uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
+// ------------------------------------------------------------ ray queries
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(BvhView b, const float* __restrict__ rays, int64_t n,
+                                                             float4* hits, uint8_t* occ, uint32_t* counts,
+                                                             uint32_t* gstack, int* overflow) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+    Stack st{lds + threadIdx.x, gstack + gt, gstride, 0};
+    for (int64_t i = gt; i < n; i += gstride) {
+        const float* r = rays + 8 * i;
+        uint32_t nn = 0, nt = 0;
+        Hit h = traverse<ANY, true>(b, f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7], st, nn, nt, overflow);
+        if (ANY) occ[i] = h.leaf != kNoHit;
+        else hits[i] = make_float4(h.t, __uint_as_float(h.id), h.u, h.v);
+        if (counts) { counts[2 * i] = nn; counts[2 * i + 1] = nt; }
+    }
+}
+
+// ------------------------------------------------------------ environment cache
+// HdrTexture::CalculateHdrCache (core/src/Scene.cpp:217-313) with every sum
+// taken in the reference's order, so the cache is bit-identical to the
+// host routine: the global luminance sum runs in row-major order on one
+// lane, the per-column marginal and conditional CDFs run down each column on
+// one lane per column, the marginal CDF runs across columns on one lane.
+__global__ void k_env_lum(const float4* __restrict__ env, int64_t m, float* lum) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float4 t = env[i];
+    lum[i] = 0.299f * t.x + 0.587f * t.y + 0.114f * t.z;  // Utils.h:76-78
+}
+__global__ void k_env_sum(const float* __restrict__ lum, int64_t m, float* out) {
+    if (threadIdx.x != 0) return;
+    float s = 0.0f;
+    int64_t i = 0;
+    for (; i + 8 <= m; i += 8) {
+        float v0 = lum[i], v1 = lum[i + 1], v2 = lum[i + 2], v3 = lum[i + 3];
+        float v4 = lum[i + 4], v5 = lum[i + 5], v6 = lum[i + 6], v7 = lum[i + 7];
+        s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+    }
+    for (; i < m; i++) s += lum[i];
+    *out = s;
+}
+__global__ void k_env_columns(const float* __restrict__ lum, const float* __restrict__ sum, int w, int h, float* pdf,
+                              float* marg, float* cdfy) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= w) return;
+    float total = *sum;
+    float m = 0.0f;
+    for (int i = 0; i < h; i++) {
+        float p = lum[(size_t)i * w + j] / total;
+        pdf[(size_t)i * w + j] = p;
+        m += p;
+    }
+    marg[j] = m;
+    float acc = 0.0f;
+    for (int i = 0; i < h; i++) {
+        float c = pdf[(size_t)i * w + j] / m;
+        acc = (i == 0) ? c : acc + c;
+        cdfy[(size_t)j * h + i] = acc;
+    }
+}
+__global__ void k_env_cdfx(const float* __restrict__ marg, int w, float* cdfx) {
+    if (threadIdx.x != 0) return;
+    float acc = marg[0];
+    cdfx[0] = acc;
+    for (int j = 1; j < w; j++) {
+        acc = acc + marg[j];
+        cdfx[j] = acc;
+    }
+}
+// std::lower_bound as libstdc++ implements it (so NaN entries behave alike).
+OXR_D int lower_bound_f(const float* a, int n, float v) {
+    int first = 0, len = n;
+    while (len > 0) {
+        int half = len >> 1;
+        if (a[first + half] < v) { first = first + half + 1; len = len - half - 1; }
+        else len = half;
+    }
+    return first;
+}
+__global__ void k_env_search(const float* __restrict__ pdf, const float* __restrict__ cdfx,
+                             const float* __restrict__ cdfy, int w, int h, float4* cache) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)w * h) return;
+    int i = (int)(t / w), j = (int)(t % w);
+    float xi1 = (float)i / (float)h, xi2 = (float)j / (float)w;
+    int x = lower_bound_f(cdfx, w, xi1);
+    int y = x < w ? lower_bound_f(cdfy + (size_t)x * h, h, xi2) : 0;
+    cache[t] = make_float4((float)x / (float)w, (float)y / (float)h, pdf[t], 0.0f);
+}
+
+// ------------------------------------------------------------ tone map
+// ComputeFinalPixelColorsKernel (PostProcessing.cu:26-48).
+__global__ void k_tonemap(const float4* __restrict__ in, uint32_t* out, int64_t n) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = in[i];
+    float lum = 0.299f * c.x + 0.587f * c.y + 0.114f * c.z;
+    float f = 1.0f / (1.0f + lum / 1.5f);
+    float r = clampf(powf(c.x * f, 1.0f / 2.2f), 0.0f, 1.0f);
+    float g = clampf(powf(c.y * f, 1.0f / 2.2f), 0.0f, 1.0f);
+    float b = clampf(powf(c.z * f, 1.0f / 2.2f), 0.0f, 1.0f);
+    out[i] = (uint32_t)(r * 255.9f) | ((uint32_t)(g * 255.9f) << 8) | ((uint32_t)(b * 255.9f) << 16) | (255u << 24);
+}
+
+// ------------------------------------------------------------ material batch
+__global__ void k_material_batch(const oxr_material* m, Lut lut, int mode, const float* __restrict__ in, int64_t n,
+                                 float* out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* a = in + 12 * i;
+    Surf s{f3(a[0], a[1], a[2]), a[9] != 0.0f, m};
+    f3 V(a[3], a[4], a[5]);
+    float pdf = 0.0f;
+    f3 f, L(a[6], a[7], a[8]);
+    Rng rng{__float_as_uint(a[10])};
+    if (mode == 0) f = eval_material(s, lut, V, L, pdf);
+    else f = sample_material(s, lut, rng, V, L, pdf);
+    float* o = out + 8 * i;
+    o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = pdf;
+    o[4] = L.x; o[5] = L.y; o[6] = L.z; o[7] = __uint_as_float(rng.s);
+}
+
+__global__ void k_gather(const float* __restrict__ nrm, const uint32_t* __restrict__ mid,
+                         const uint32_t* __restrict__ order, int64_t n, float* nrm_s, uint32_t* mid_s) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t t = order[k];
+#pragma unroll
+    for (int c = 0; c < 9; c++) nrm_s[9 * k + c] = nrm[9 * (size_t)t + c];
+    mid_s[k] = mid[t];
+}
+
+// ------------------------------------------------------------ wavefront
+__global__ __launch_bounds__(256) void k_generate(WaveParams p, int frame0, int64_t npaths) {
+    int64_t pid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pid == 0) {
+        p.ctr[kCtrQueue0] = (int)npaths;
+        p.ctr[kCtrQueue1] = 0;
+        p.ctr[kCtrFetchExtend] = 0;
+    }
+    if (pid >= npaths) return;
+    const int npix = p.width * p.height;
+    int s = (int)(pid / npix), pix = (int)(pid % npix);
+    int ix = pix % p.width, iy = pix / p.width;
+    Rng rng{lcg_seed((uint32_t)(ix + p.width * iy), (uint32_t)(frame0 + s))};
+    float j0 = rng(), j1 = rng();
+    f2 jit = filter_jitter(f2{j0, j1}, p.filter);                         // devicePrograms.cu:220-227
+    float sx = ((float)ix + 0.5f + jit.x) / (float)p.width;
+    float sy = ((float)iy + 0.5f + jit.y) / (float)p.height;
+    f3 dir = normalize(make3(p.cam_dir) + (sx - 0.5f) * make3(p.cam_h) + (sy - 0.5f) * make3(p.cam_v));
+    p.ray_o[pid] = make_float4(p.cam_pos[0], p.cam_pos[1], p.cam_pos[2], 0.0f);
+    p.ray_d[pid] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
+    p.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.queue[0][pid] = (uint32_t)pid;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int bounce) {
+    uint32_t* gstack = p.gstack;
+    int* overflow = p.overflow;
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int qi = bounce & 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.ctr[qi ^ 1] = 0;
+        p.ctr[kCtrShEnv] = 0;
+        p.ctr[kCtrShLight] = 0;
+        p.ctr[kCtrFetchShade] = 0;
+        p.ctr[kCtrFetchConnEnv] = 0;
+        p.ctr[kCtrFetchConnLight] = 0;
+    }
+    const int count = p.ctr[qi];
+    const uint32_t* q = p.queue[qi];
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    Stack st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
+    uint32_t nn = 0, nt = 0, nr = 0;
+    while (true) {
+        int base = wave_fetch(&p.ctr[kCtrFetchExtend]);
+        if (base >= count) break;
+        int slot = base + lane_id();
+        if (slot < count) {
+            uint32_t pid = q[slot];
+            float4 o = p.ray_o[pid], d = p.ray_d[pid];
+            Hit h = traverse<false, COUNT>(p.bvh, xyz(o), xyz(d), kEps, kFltMax, st, nn, nt, overflow);
+            p.hits[slot] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
+            nr++;
+        }
+    }
+    if (COUNT) {
+        wave_count_add(&p.traversal[0], nn);
+        wave_count_add(&p.traversal[1], nt);
+    }
+    wave_count_add(&p.traversal[2], nr);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which) {
+    uint32_t* gstack = p.gstack;
+    int* overflow = p.overflow;
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int count = p.ctr[which == 0 ? kCtrShEnv : kCtrShLight];
+    const float4* rec = which == 0 ? p.sh_env : p.sh_light;
+    int* fetch = &p.ctr[which == 0 ? kCtrFetchConnEnv : kCtrFetchConnLight];
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    Stack st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
+    uint32_t nn = 0, nt = 0, nr = 0;
+    while (true) {
+        int base = wave_fetch(fetch);
+        if (base >= count) break;
+        int slot = base + lane_id();
+        if (slot < count) {
+            float4 a = rec[3 * (size_t)slot], b = rec[3 * (size_t)slot + 1];
+            Hit h = traverse<true, COUNT>(p.bvh, xyz(a), xyz(b), kEps, a.w, st, nn, nt, overflow);
+            nr++;
+            if (h.leaf == kNoHit) {
+                float4 c = rec[3 * (size_t)slot + 2];
+                uint32_t pid = __float_as_uint(b.w);
+                float4 r = p.rad[pid];
+                r.x = r.x + c.x;
+                r.y = r.y + c.y;
+                r.z = r.z + c.z;
+                p.rad[pid] = r;
+            }
+        }
+    }
+    if (COUNT) {
+        wave_count_add(&p.traversal[4], nn);
+        wave_count_add(&p.traversal[5], nt);
+    }
+    wave_count_add(&p.traversal[3], nr);
+}
+
+struct ShadowOut {
+    bool emit;
+    f3 o, d, c;
+    float tmax;
+};
+
+// lightTrace lambda (devicePrograms.cu:192-211): the nearest analytic light
+// closer than `closest` wins; light_pdf is overwritten by every light tried.
+OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& lpdf, float& closest) {
+    bool hit = false;
+    for (int i = 0; i < p.n_lights; i++) {
+        const oxr_light* l = p.lights + i;
+        float ld = 0.0f;
+        f3 Li = eval_light(l, o, d, closest, lpdf, ld);
+        if (lpdf == 0.0f) continue;
+        hit = true;
+        Lr = Li;
+        closest = ld;
+        mi = l->medium;
+    }
+    return hit;
+}
+
+// One bounce of __raygen__renderFrame's loop body (devicePrograms.cu:270-496)
+// for one path, after its radiance ray was traced.  Returns whether the path
+// continues; NEE shadow rays are returned for the connect stage with their
+// contribution precomputed (it is added only if the ray is unoccluded).
+OXR_D bool shade_path(const WaveParams& p, const Lut& lut, int bounce, uint32_t pid, float4 hit, ShadowOut& sh_env,
+                      ShadowOut& sh_light) {
+    sh_env.emit = false;
+    sh_light.emit = false;
+    float4 ro = p.ray_o[pid], rd = p.ray_d[pid], th = p.thr[pid];
+    f3 org = xyz(ro), dir = xyz(rd);
+    float pre_pdf = ro.w;
+    Rng rng{__float_as_uint(rd.w)};
+    f3 history = xyz(th);
+    float4 rad4 = p.rad[pid];
+    f3 radiance = xyz(rad4);
+    f3 V = -dir, L = dir;
+    float mult_trans_pdf = 1.0f;
+
+    // __closesthit__radiance / __miss__radiance (devicePrograms.cu:71-159)
+    uint32_t leaf = __float_as_uint(hit.y);
+    Surf s;
+    s.m = nullptr;
+    s.front = true;
+    s.n = f3(0.0f);
+    f3 pos = org;
+    float distance;
+    int mi_in, mi_out;
+    if (leaf != kNoHit) {
+        const float* nb = p.normals + 9 * (size_t)leaf;
+        const float4* tb = p.bvh.tris + 3 * (size_t)leaf;
+        float u = hit.z, v = hit.w, w = 1.0f - u - v;
+        f3 Ns = normalize(w * f3(nb[0], nb[1], nb[2]) + u * f3(nb[3], nb[4], nb[5]) + v * f3(nb[6], nb[7], nb[8]));
+        f3 P = w * xyz(tb[0]) + u * xyz(tb[1]) + v * xyz(tb[2]);
+        s.m = p.mats + p.mat_ids[leaf];
+        s.front = dot(dir, Ns) < 0.0f;                                      // Utils.h:70-73
+        s.n = s.front ? Ns : -Ns;
+        distance = length(P - org);
+        pos = P;
+        mi_in = s.m->in_medium;
+        mi_out = s.m->out_medium;
+        if (bounce == 0) {
+            p.aov_nrm[pid] = make_float4(Ns.x, Ns.y, Ns.z, 1.0f);
+            p.aov_alb[pid] = make_float4(s.m->albedo[0], s.m->albedo[1], s.m->albedo[2], 1.0f);
+        }
+    } else {
+        distance = kFltMax;
+        mi_in = mi_out = p.cam_medium;
+        if (bounce == 0) {
+            p.aov_nrm[pid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.aov_alb[pid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        }
+    }
+    float closest = distance;
+    f3 light_radiance(0.0f);
+    float light_pdf = 0.0f;
+    int mi_light = 0;
+    bool hitLight = light_trace(p, org, dir, mi_light, light_radiance, light_pdf, closest);
+    if (hitLight) mi_in = mi_out = mi_light;
+
+    bool scattered = false;
+    int medium_index = -1;
+    bool alive = true;
+    if (p.n_media != 0) {                                                   // devicePrograms.cu:276-338
+        medium_index = (hitLight || s.front) ? mi_out : mi_in;
+        if (medium_index != -1) {
+            const oxr_medium* md = p.media + medium_index;
+            float dist = 0.0f, trans_pdf = 0.0f;
+            f3 T;
+            scattered = sample_medium_distance(md, closest, dist, trans_pdf, T, rng);
+            history = history * (T / f3(trans_pdf));
+            mult_trans_pdf *= trans_pdf;
+            if (scattered) {
+                pos = org + dist * L;
+                if (p.n_lights != 0) {
+                    int idx = min(max((int)((float)p.n_lights * rng()), 0), p.n_lights - 1);
+                    const oxr_light* l = p.lights + idx;
+                    float ld = 0.0f;
+                    float u0 = rng(), u1 = rng();
+                    f3 sd;
+                    light_radiance = sample_light(l, pos, f2{u0, u1}, sd, ld, light_pdf);
+                    light_pdf *= (1.0f / (float)p.n_lights);
+                    f3 tv; float tp; int tm;
+                    float cd = ld - kEps;
+                    if (!light_trace(p, pos, sd, tm, tv, tp, cd)) {
+                        float tpdf2 = 0.0f, phase_pdf = 0.0f;
+                        f3 T2 = eval_medium_distance(md, false, cd, tpdf2);
+                        f3 att = eval_phase(md, V, sd, phase_pdf);
+                        float mw = power_heuristic(light_pdf, phase_pdf * tpdf2);
+                        sh_light.emit = true;
+                        sh_light.o = pos;
+                        sh_light.d = sd;
+                        sh_light.tmax = ld - kEps;
+                        sh_light.c = mw * (T2 / f3(tpdf2)) * light_radiance * att * history / f3(light_pdf);
+                    }
+                }
+                float phase_pdf = 0.0f;
+                f3 att = sample_phase(md, rng, V, L, phase_pdf);
+                history = history * (att / f3(phase_pdf));
+                pre_pdf = phase_pdf;
+            }
+        }
+    }
+
+    if (!scattered) {
+        if (!hitLight && s.m == nullptr && closest == kFltMax && medium_index != -1) {
+            // a ray inside a medium that left the scene unscattered: the
+            // reference would shade stale surface data with zero throughput
+            alive = false;
+        } else if (hitLight) {                                              // devicePrograms.cu:342-351
+            float mw = 1.0f;
+            if (bounce != 0) {
+                pre_pdf *= mult_trans_pdf;
+                mw = power_heuristic(pre_pdf, light_pdf);
+            }
+            radiance = radiance + mw * history * light_radiance;
+            alive = false;
+        } else if (closest == kFltMax && medium_index == -1) {              // devicePrograms.cu:354-373
+            if (p.env_w > 0) {
+                f2 uv = sphere_to_plane(dir);
+                light_pdf = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
+                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                light_radiance = f3(e.x, e.y, e.z);
+                light_pdf *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
+                float mw = 1.0f;
+                if (bounce != 0) {
+                    pre_pdf *= mult_trans_pdf;
+                    mw = power_heuristic(pre_pdf, light_pdf);
+                }
+                radiance = radiance + mw * history * light_radiance;
+            }
+            alive = false;
+        } else {
+            if (p.env_w > 0) {                                              // devicePrograms.cu:376-424
+                float x0 = rng(), x1 = rng();
+                float4 uvt = bilinear(p.env_cache, p.env_w, p.env_h, x0, x1);
+                f2 uv{uvt.x, uvt.y};
+                f3 sd = plane_to_sphere(uv);
+                float lp = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
+                lp *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
+                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                f3 lr(e.x, e.y, e.z);
+                f3 tv; float tp; int tm;
+                float cd = kFltMax;
+                if (!light_trace(p, pos, sd, tm, tv, tp, cd)) {
+                    float bpdf = 0.0f;
+                    f3 f = eval_material(s, lut, V, sd, bpdf);
+                    float ct = fabsf(dot(s.n, sd));
+                    float mw = power_heuristic(lp, bpdf);
+                    f3 thh(1.0f);
+                    if (medium_index != -1) {
+                        float tpdf2 = 0.0f;
+                        f3 T2 = eval_medium_distance(p.media + medium_index, false, cd, tpdf2);
+                        thh = T2 / f3(tpdf2);
+                        mw = power_heuristic(lp, bpdf * tpdf2);
+                    }
+                    sh_env.emit = true;
+                    sh_env.o = pos;
+                    sh_env.d = sd;
+                    sh_env.tmax = kFltMax;
+                    sh_env.c = mw * thh * lr * f * ct * history / f3(lp);
+                }
+            }
+            if (p.n_lights != 0) {                                          // devicePrograms.cu:426-471
+                int idx = min(max((int)((float)p.n_lights * rng()), 0), p.n_lights - 1);
+                const oxr_light* l = p.lights + idx;
+                float ld = 0.0f;
+                float u0 = rng(), u1 = rng();
+                f3 sd;
+                f3 lr = sample_light(l, pos, f2{u0, u1}, sd, ld, light_pdf);
+                light_pdf *= (1.0f / (float)p.n_lights);
+                f3 tv; float tp; int tm;
+                float cd = ld - kEps;
+                if (!light_trace(p, pos, sd, tm, tv, tp, cd)) {
+                    float bpdf = 0.0f;
+                    f3 f = eval_material(s, lut, V, sd, bpdf);
+                    float ct = fabsf(dot(s.n, sd));
+                    float mw = power_heuristic(light_pdf, bpdf);
+                    f3 thh(1.0f);
+                    if (medium_index != -1) {
+                        float tpdf2 = 0.0f;
+                        f3 T2 = eval_medium_distance(p.media + medium_index, false, cd, tpdf2);
+                        thh = T2 / f3(tpdf2);
+                        mw = power_heuristic(light_pdf, bpdf * tpdf2);
+                    }
+                    sh_light.emit = true;
+                    sh_light.o = pos;
+                    sh_light.d = sd;
+                    sh_light.tmax = ld - kEps;
+                    sh_light.c = mw * thh * lr * f * ct * history / f3(light_pdf);
+                }
+            }
+            float bpdf = 0.0f;                                              // devicePrograms.cu:474-477
+            f3 f = sample_material(s, lut, rng, V, L, bpdf);
+            float ct = fabsf(dot(s.n, L));
+            history = history * (f * ct / f3(bpdf));
+            pre_pdf = bpdf;
+        }
+    }
+    if (alive) {
+        if (bounce >= p.rr_start) {                                         // devicePrograms.cu:481-485
+            float prr = fminf((history.x + history.y + history.z) / 3.0f, 0.95f);
+            if (rng() > prr) alive = false;
+            else history = history / f3(prr);
+        }
+        if (alive && any_nan(history)) alive = false;                       // devicePrograms.cu:487-489
+    }
+    p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+    if (alive) {
+        p.ray_o[pid] = make_float4(pos.x, pos.y, pos.z, pre_pdf);
+        p.ray_d[pid] = make_float4(L.x, L.y, L.z, __uint_as_float(rng.s));
+        p.thr[pid] = make_float4(history.x, history.y, history.z, 0.0f);
+    }
+    return alive && bounce + 1 < p.max_depth;
+}
+
+OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) {
+    int slot = wave_append(ctr, s.emit);
+    if (slot >= 0) {
+        rec[3 * (size_t)slot] = make_float4(s.o.x, s.o.y, s.o.z, s.tmax);
+        rec[3 * (size_t)slot + 1] = make_float4(s.d.x, s.d.y, s.d.z, __uint_as_float(pid));
+        rec[3 * (size_t)slot + 2] = make_float4(s.c.x, s.c.y, s.c.z, 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shade(WaveParams p, int bounce) {
+    const int qi = bounce & 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.ctr[kCtrFetchExtend] = 0;
+    const int count = p.ctr[qi];
+    const uint32_t* q = p.queue[qi];
+    uint32_t* qo = p.queue[qi ^ 1];
+    Lut lut{p.lut_bsdf, p.lut_albedo};
+    while (true) {
+        int base = wave_fetch(&p.ctr[kCtrFetchShade]);
+        if (base >= count) break;
+        int slot = base + lane_id();
+        bool active = slot < count;
+        uint32_t pid = 0;
+        bool cont = false;
+        ShadowOut se, sl;
+        se.emit = sl.emit = false;
+        if (active) {
+            pid = q[slot];
+            cont = shade_path(p, lut, bounce, pid, p.hits[slot], se, sl);
+        }
+        int o = wave_append(&p.ctr[qi ^ 1], cont);
+        if (o >= 0) qo[o] = pid;
+        emit_shadow(p.sh_env, &p.ctr[kCtrShEnv], se, pid);
+        emit_shadow(p.sh_light, &p.ctr[kCtrShLight], sl, pid);
+    }
+}
+
+// devicePrograms.cu:499-531 with numPixelSamples = 1: a NaN radiance drops
+// the frame's sample (all three outputs become 0 for that frame).
+__global__ void k_accumulate(WaveParams p, int frame0, int frames, int path_frame_offset) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int npix = p.width * p.height;
+    if (pix >= npix) return;
+    float4 c = p.color[pix], a = p.albedo[pix], nr = p.normal[pix];
+    for (int s = 0; s < frames; s++) {
+        size_t pid = (size_t)s * npix + pix;
+        int frame = frame0 + s;
+        float4 r = p.rad[pid];
+        float4 al = p.aov_alb[pid], no = p.aov_nrm[pid];
+        float4 x = make_float4(r.x, r.y, r.z, 1.0f);
+        if (isnan(r.x) || isnan(r.y) || isnan(r.z)) {
+            x = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            al = x;
+            no = x;
+        }
+        if (p.path_radiance) {
+            float* pr = p.path_radiance + 3 * ((size_t)(path_frame_offset + s) * npix + pix);
+            pr[0] = r.x; pr[1] = r.y; pr[2] = r.z;
+        }
+        if (frame > 0) {
+            float f = (float)frame, g = (float)frame + 1.0f;
+            x.x = (x.x + f * c.x) / g; x.y = (x.y + f * c.y) / g; x.z = (x.z + f * c.z) / g; x.w = (x.w + f * c.w) / g;
+            al.x = (al.x + f * a.x) / g; al.y = (al.y + f * a.y) / g; al.z = (al.z + f * a.z) / g; al.w = (al.w + f * a.w) / g;
+            no.x = (no.x + f * nr.x) / g; no.y = (no.y + f * nr.y) / g; no.z = (no.z + f * nr.z) / g; no.w = (no.w + f * nr.w) / g;
+        }
+        c = x;
+        a = al;
+        nr = no;
+    }
+    p.color[pix] = c;
+    p.albedo[pix] = a;
+    p.normal[pix] = nr;
+}
+
+inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t trace_rays(const BvhView& b, const float* rays, int64_t n, int any, float4* hits, uint8_t* occ,
+                      uint32_t* counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (any) k_trace_rays<true><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, counts, gstack, overflow);
+    else k_trace_rays<false><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, counts, gstack, overflow);
+    return hipGetLastError();
+}
+
+size_t env_scratch_floats(int w, int h) { return (size_t)3 * w * h + 2 * (size_t)w + 16; }
+
+hipError_t env_cache_build(const float4* env, int w, int h, float4* cache, float* scratch, hipStream_t s) {
+    int64_t m = (int64_t)w * h;
+    float* lum = scratch;
+    float* pdf = lum + m;
+    float* cdfy = pdf + m;
+    float* marg = cdfy + m;
+    float* cdfx = marg + w;
+    float* sum = cdfx + w;
+    k_env_lum<<<blocks(m, 256), 256, 0, s>>>(env, m, lum);
+    k_env_sum<<<1, 64, 0, s>>>(lum, m, sum);
+    k_env_columns<<<blocks(w, 64), 64, 0, s>>>(lum, sum, w, h, pdf, marg, cdfy);
+    k_env_cdfx<<<1, 64, 0, s>>>(marg, w, cdfx);
+    k_env_search<<<blocks(m, 256), 256, 0, s>>>(pdf, cdfx, cdfy, w, h, cache);
+    return hipGetLastError();
+}
+
+hipError_t tonemap(const float4* in, uint32_t* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_tonemap<<<blocks(n, 256), 256, 0, s>>>(in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t material_batch(const oxr_material* m, const float* lb, const float* la, int mode, const float* in,
+                          int64_t n, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_material_batch<<<blocks(n, 128), 128, 0, s>>>(m, Lut{lb, la}, mode, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const uint32_t* order, int64_t n,
+                                    float* nrm_s, uint32_t* mid_s, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_gather<<<blocks(n, 256), 256, 0, s>>>(nrm, mid, order, n, nrm_s, mid_s);
+    return hipGetLastError();
+}
+
+hipError_t wave_generate(const WaveParams& p, int frame0, int frames, hipStream_t s) {
+    int64_t np = (int64_t)frames * p.width * p.height;
+    k_generate<<<blocks(np, 256), 256, 0, s>>>(p, frame0, np);
+    return hipGetLastError();
+}
+
+hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s) {
+    if (count) k_extend<true><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+    else k_extend<false><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+    return hipGetLastError();
+}
+
+hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s) {
+    k_shade<<<grid, 256, 0, s>>>(p, bounce);
+    return hipGetLastError();
+}
+
+hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s) {
+    if (count) k_connect<true><<<grid, kTraceBlock, 0, s>>>(p, which);
+    else k_connect<false><<<grid, kTraceBlock, 0, s>>>(p, which);
+    return hipGetLastError();
+}
+
+hipError_t wave_accumulate(const WaveParams& p, int frame0, int frames, int path_frame_offset, hipStream_t s) {
+    int npix = p.width * p.height;
+    k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, frame0, frames, path_frame_offset);
+    return hipGetLastError();
+}
+
+}  // namespace oxr

@@ -1,0 +1,162 @@
+// BVH traversal on gfx950: the build's replacement for optixTrace
+// (core/cuda/devicePrograms.cu:258-269 closest hit, :303-320/:389-405/:437-453
+// any hit with TERMINATE_ON_FIRST_HIT).
+//
+// One ray per lane.  Each visited internal node costs one 64-byte fetch of
+// its two child records (4 x dwordx4), leaves are tested at once in
+// (left, right) order with a watertight ray/triangle test (Woop, Benthin,
+// Wald 2013) on a 48-byte triangle record, then the nearer hit internal
+// child is descended and the farther pushed.  The stack lives in LDS
+// (kStackLds entries per lane, [entry][lane] so a wave touches consecutive
+// banks) and spills the deeper entries to a per-lane HBM region.
+// Closest-hit ties in t go to the lower triangle id, so the result does not
+// depend on visit order.
+#pragma once
+#include "common.h"
+#include "dmath.h"
+
+namespace oxr {
+
+struct RayPre {
+    f3 o, inv;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+OXR_D float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+OXR_D RayPre prep_ray(f3 o, f3 d) {
+    RayPre r;
+    r.o = o;
+    r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int kz = 0;
+    if (fabsf(d.y) > fabsf(comp(d, kz))) kz = 1;
+    if (fabsf(d.z) > fabsf(comp(d, kz))) kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    if (comp(d, kz) < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    float dz = comp(d, kz);
+    r.Sx = comp(d, kx) / dz;
+    r.Sy = comp(d, ky) / dz;
+    r.Sz = 1.0f / dz;
+    return r;
+}
+
+OXR_D bool tri_test(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float& t, float& u, float& v) {
+    f3 A = xyz(a) - r.o, B = xyz(b) - r.o, C = xyz(c) - r.o;
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
+    float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
+    float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    t = T / det;
+    if (!(t > tmin)) return false;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+OXR_D bool box_test(const RayPre& r, float4 lo, float4 hi, float tmin, float tbest, float& tn) {
+    float tx0 = (lo.x - r.o.x) * r.inv.x, tx1 = (hi.x - r.o.x) * r.inv.x;
+    float ty0 = (lo.y - r.o.y) * r.inv.y, ty1 = (hi.y - r.o.y) * r.inv.y;
+    float tz0 = (lo.z - r.o.z) * r.inv.z, tz1 = (hi.z - r.o.z) * r.inv.z;
+    tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000024f;
+    return tn <= tf && tf >= tmin && tn <= tbest;
+}
+
+struct Hit {
+    float t;
+    uint32_t leaf;  // sorted leaf position, kNoHit on miss
+    uint32_t id;    // triangle id
+    float u, v;
+};
+
+struct Stack {
+    uint32_t* lds;   // this lane's first entry; entries kTraceBlock apart
+    uint32_t* gbl;   // this lane's first spill entry; entries gstride apart
+    int64_t gstride;
+    int sp;
+    OXR_D bool push(uint32_t x) {
+        if (sp < kStackLds) lds[sp * kTraceBlock] = x;
+        else if (sp < kStackLds + kStackGlobal) gbl[(int64_t)(sp - kStackLds) * gstride] = x;
+        else return false;
+        sp++;
+        return true;
+    }
+    OXR_D uint32_t pop() {
+        sp--;
+        return sp < kStackLds ? lds[sp * kTraceBlock] : gbl[(int64_t)(sp - kStackLds) * gstride];
+    }
+};
+
+template <bool ANY, bool COUNT>
+OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, Stack& st, uint32_t& nn, uint32_t& nt,
+                   int* overflow) {
+    Hit h;
+    h.t = tmax;
+    h.leaf = kNoHit;
+    h.id = kNoHit;
+    h.u = h.v = 0.0f;
+    if (b.n <= 0) return h;
+    RayPre r = prep_ray(o, d);
+    auto leaf_test = [&](uint32_t k) -> bool {
+        if (k >= (uint64_t)b.n) { *overflow = 2; return false; }
+        const float4* tr = b.tris + 3 * (size_t)k;
+        float4 a = tr[0], bb = tr[1], c = tr[2];
+        if (COUNT) nt++;
+        float t, u, v;
+        if (!tri_test(r, a, bb, c, tmin, t, u, v)) return false;
+        uint32_t id = __float_as_uint(a.w);
+        bool take = ANY ? (t < tmax) : (t < h.t || (t == h.t && h.leaf != kNoHit && id < h.id));
+        if (take) { h.t = t; h.leaf = k; h.id = id; h.u = u; h.v = v; }
+        return take;
+    };
+    if (b.n == 1) {
+        leaf_test(0);
+        return h;
+    }
+    st.sp = 0;
+    uint32_t node = 0;
+    // A tree traversal visits each internal node at most once; the cap only
+    // guards against a malformed hierarchy.
+    int64_t budget = b.n + 64;
+    while (true) {
+        if (--budget < 0 || node >= (uint32_t)(b.n - 1)) { *overflow = 2; break; }
+        const float4* pr = b.pairs + 4 * (size_t)node;
+        float4 lo0 = pr[0], hi0 = pr[1], lo1 = pr[2], hi1 = pr[3];
+        if (COUNT) nn += 2;
+        uint32_t c0 = __float_as_uint(lo0.w), c1 = __float_as_uint(lo1.w);
+        float tn0, tn1;
+        bool h0 = box_test(r, lo0, hi0, tmin, h.t, tn0);
+        bool h1 = box_test(r, lo1, hi1, tmin, h.t, tn1);
+        if (h0 && (c0 & kLeafBit)) {
+            h0 = false;
+            if (leaf_test(c0 & ~kLeafBit) && ANY) break;
+        }
+        if (h1 && (c1 & kLeafBit)) {
+            h1 = false;
+            if (leaf_test(c1 & ~kLeafBit) && ANY) break;
+        }
+        if (h0 && h1) {
+            bool second = tn1 < tn0;
+            if (!st.push(second ? c0 : c1)) { *overflow = 1; break; }
+            node = second ? c1 : c0;
+        } else if (h0) {
+            node = c0;
+        } else if (h1) {
+            node = c1;
+        } else {
+            if (st.sp == 0) break;
+            node = st.pop();
+        }
+    }
+    return h;
+}
+
+}  // namespace oxr

@@ -1,0 +1,169 @@
+"""Seeded synthetic scenes for the benchmark configurations.
+
+Every scene is built in float64 with numpy and stored as float32, so the GPU
+path and the oracle receive bit-identical inputs.  Random draws use numpy's
+PCG64 seeded per scene.
+
+  cornell_spheres (256x256, 16 spp, depth 8, RR from bounce 3):
+      Cornell box of 6 diffuse quads (grey 0.7 / red / green), one quad light of
+      radiance 15, a GGX conductor icosphere (alpha 0.2) and a GGX dielectric
+      icosphere (IOR 1.5), both subdivision 3 (1280 triangles each).
+  sphere_field (1920x1080, 64 spp, depth 16): 12 icospheres at subdivision 6
+      (81,920 triangles each, 983,040 total), centres U[-10,10]^3, radii
+      U[0.5,2], materials uniform over conductor/dielectric/plastic with
+      roughness U[0.05,0.8], lit by a procedural 2048x1024 HDR sky with a
+      Gaussian sun of peak 5e4.
+  heightfield_medium (1920x1080, 32 spp, depth 32): fBm heightfield with
+      interleaved per-triangle materials and a 4x4x4 homogeneous-medium box
+      (sigma_t 0.5, albedo 0.9, HG g 0.3) around a dielectric sphere.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import abi
+from .renderer import SceneData
+
+_PHI = (1.0 + math.sqrt(5.0)) / 2.0
+
+
+def icosphere(subdiv: int) -> np.ndarray:
+    """Unit icosphere as a triangle soup (20*4^s, 3, 3) float64, CCW outward."""
+    v = np.array([[-1, _PHI, 0], [1, _PHI, 0], [-1, -_PHI, 0], [1, -_PHI, 0], [0, -1, _PHI], [0, 1, _PHI],
+                  [0, -1, -_PHI], [0, 1, -_PHI], [_PHI, 0, -1], [_PHI, 0, 1], [-_PHI, 0, -1], [-_PHI, 0, 1]],
+                 dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+    tris = v[f]
+    for _ in range(subdiv):
+        a, b, c = tris[:, 0], tris[:, 1], tris[:, 2]
+
+        def mid(p, q):
+            m = p + q  # symmetric in (p, q): shared edges give identical vertices
+            return m / np.linalg.norm(m, axis=1, keepdims=True)
+
+        ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+        tris = np.stack([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1), np.stack([ca, bc, c], 1),
+                         np.stack([ab, bc, ca], 1)], 1).reshape(-1, 3, 3)
+    return tris
+
+
+def flat_normals(tris: np.ndarray) -> np.ndarray:
+    """Geometric normals as TriangleMesh::LoadMesh adds them (Scene.cpp:150-158)."""
+    e1 = tris[:, 1] - tris[:, 0]
+    e2 = tris[:, 2] - tris[:, 0]
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 /= np.linalg.norm(e2, axis=1, keepdims=True)
+    n = np.cross(e1, e2)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return np.repeat(n[:, None, :], 3, axis=1)
+
+
+def quad(p0, p1, p2, p3) -> np.ndarray:
+    """Two triangles (p0,p1,p2), (p0,p2,p3)."""
+    p = np.array([p0, p1, p2, p3], dtype=np.float64)
+    return np.stack([p[[0, 1, 2]], p[[0, 2, 3]]])
+
+
+def _soup(parts):
+    pos = np.concatenate([p for p, _, _ in parts]).astype(np.float32)
+    nrm = np.concatenate([n for _, n, _ in parts]).astype(np.float32)
+    mid = np.concatenate([np.full(len(p), m, np.uint32) for p, _, m in parts])
+    return pos, nrm, mid
+
+
+def cornell_spheres(subdiv: int = 3) -> SceneData:
+    """Cornell box + conductor and dielectric icospheres; 2,572 triangles."""
+    x0, x1, y0, y1, z0, z1 = -1.0, 1.0, 0.0, 2.0, -1.0, 3.2
+    walls = [
+        (quad((x0, y0, z0), (x0, y0, z1), (x1, y0, z1), (x1, y0, z0)), 0),  # floor, normal +y
+        (quad((x0, y1, z0), (x1, y1, z0), (x1, y1, z1), (x0, y1, z1)), 0),  # ceiling, -y
+        (quad((x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0)), 0),  # back wall, +z
+        (quad((x0, y0, z1), (x0, y1, z1), (x1, y1, z1), (x1, y0, z1)), 0),  # front wall, -z
+        (quad((x0, y0, z0), (x0, y1, z0), (x0, y1, z1), (x0, y0, z1)), 1),  # left (red), +x
+        (quad((x1, y0, z0), (x1, y0, z1), (x1, y1, z1), (x1, y1, z0)), 2),  # right (green), -x
+    ]
+    parts = [(q, flat_normals(q), m) for q, m in walls]
+    ico = icosphere(subdiv)
+    for centre, radius, m in (((-0.45, 0.42, -0.2), 0.42, 3), ((0.45, 0.42, 0.4), 0.42, 4)):
+        t = ico * radius + np.asarray(centre)
+        parts.append((t, flat_normals(t), m))
+    pos, nrm, mid = _soup(parts)
+    mats = [
+        abi.material(abi.DIFFUSE, albedo=(0.7, 0.7, 0.7), roughness=0.0),
+        abi.material(abi.DIFFUSE, albedo=(0.7, 0.1, 0.1), roughness=0.0),
+        abi.material(abi.DIFFUSE, albedo=(0.1, 0.7, 0.1), roughness=0.0),
+        abi.material(abi.CONDUCTOR, roughness=math.sqrt(0.2)),
+        abi.material(abi.DIELECTRIC, int_ior=1.5, ext_ior=1.0),
+    ]
+    # Emitter just below the ceiling, u x v = +y.  Double-sided: with a
+    # one-sided quad every point above its plane gets 0/0 from light NEE
+    # (Light.h:63-66 then devicePrograms.cu:469), which drops the sample.
+    lights = [abi.light(0, double_side=1, radiance=(15.0, 15.0, 15.0), position=(-0.3, 1.98, 0.3),
+                        u=(0.6, 0.0, 0.0), v=(0.0, 0.0, -0.6))]
+    cam = abi.camera((0.0, 1.0, 3.0), (0.0, 1.0, 0.0))
+    return SceneData(pos, nrm, mid, mats, lights, [], None, cam)
+
+
+def sky_env(w: int = 2048, h: int = 1024, sun_peak: float = 5e4, sun_sigma_deg: float = 0.5,
+            sun_dir=(0.35, 0.65, -0.45)) -> np.ndarray:
+    """Procedural equirect sky: gradient plus a Gaussian sun, (h, w, 4) float32.
+
+    Texel (i, j) is centred at uv = ((j+0.5)/w, (i+0.5)/h) and the direction
+    follows PlaneToSphere (core/cuda/Sampling.h:12-23)."""
+    u = (np.arange(w) + 0.5) / w
+    v = (np.arange(h) + 0.5) / h
+    phi = 2.0 * math.pi * (u - 0.5)
+    theta = math.pi * (v - 0.5)
+    ct, st = np.cos(theta)[:, None], np.sin(theta)[:, None]
+    d = np.stack([ct * np.cos(phi)[None, :], np.broadcast_to(st, (h, w)), ct * np.sin(phi)[None, :]], -1)
+    y = d[..., 1:2]
+    horizon = np.array([1.0, 0.85, 0.7])
+    zenith = np.array([0.15, 0.35, 0.9])
+    ground = np.array([0.25, 0.22, 0.2])
+    t = np.clip(y, 0.0, 1.0) ** 0.5
+    sky = horizon * (1 - t) + zenith * t
+    rgb = np.where(y >= 0, sky, ground * (1.0 - 0.5 * np.clip(-y, 0, 1)))
+    s = np.asarray(sun_dir, np.float64)
+    s /= np.linalg.norm(s)
+    ang = np.arccos(np.clip((d * s).sum(-1), -1.0, 1.0))
+    sig = math.radians(sun_sigma_deg)
+    rgb = rgb + sun_peak * np.exp(-0.5 * (ang / sig) ** 2)[..., None] * np.array([1.0, 0.95, 0.85])
+    out = np.ones((h, w, 4), np.float32)
+    out[..., :3] = rgb
+    return out
+
+
+def sphere_field(seed: int = 1, subdiv: int = 6, count: int = 12, env_w: int = 2048, env_h: int = 1024) -> SceneData:
+    """12 icospheres (983,040 triangles at subdivision 6) under a procedural HDR sky."""
+    rng = np.random.default_rng(seed)
+    ico = icosphere(subdiv)
+    nrm_unit = ico / np.linalg.norm(ico, axis=2, keepdims=True)
+    parts, mats = [], []
+    for i in range(count):
+        c = rng.uniform(-10.0, 10.0, 3)
+        r = rng.uniform(0.5, 2.0)
+        kind = int(rng.integers(0, 3))
+        rough = float(rng.uniform(0.05, 0.8))
+        col = rng.uniform(0.2, 0.9, 3)
+        if kind == 0:
+            mats.append(abi.material(abi.CONDUCTOR, roughness=rough))
+        elif kind == 1:
+            mats.append(abi.material(abi.DIELECTRIC, roughness=rough, int_ior=1.5, ext_ior=1.0))
+        else:
+            mats.append(abi.material(abi.PLASTIC, roughness=rough, albedo=col, int_ior=1.5, ext_ior=1.0))
+        parts.append((ico * r + c, nrm_unit, i))
+    pos, nrm, mid = _soup(parts)
+    env = sky_env(env_w, env_h)
+    cam = abi.camera((0.0, 0.0, 30.0), (0.0, 0.0, 0.0))
+    return SceneData(pos, nrm, mid, mats, [], [], env, cam)
+
+
+CONFIGS = {
+    "cornell_spheres": dict(width=256, height=256, spp=16, max_depth=8, rr_start=3),
+    "sphere_field": dict(width=1920, height=1080, spp=64, max_depth=16, rr_start=0),
+}
