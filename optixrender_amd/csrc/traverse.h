@@ -28,7 +28,10 @@ OXR_D float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 OXR_D RayPre prep_ray(f3 o, f3 d) {
     RayPre r;
     r.o = o;
-    r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    // A zero component becomes +-1e-20 so that the slab products are never
+    // 0 * inf = NaN for a ray lying in a box face.
+    auto safe = [](float x) { return fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x); };
+    r.inv = f3(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
     int kz = 0;
     if (fabsf(d.y) > fabsf(comp(d, kz))) kz = 1;
     if (fabsf(d.z) > fabsf(comp(d, kz))) kz = 2;

@@ -1325,7 +1325,10 @@ RayPre prep_ray(V3 o, V3 d) {
     RayPre r;
     r.o = o;
     r.d = d;
-    r.inv = V3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    // reciprocal direction for the slab test; a zero component becomes
+    // +-1e-20 so that (bound - origin) * inv is never 0 * inf = NaN
+    auto safe = [](float x) { return fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x); };
+    r.inv = V3(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
     int kz = 0;
     if (fabsf(d.y) > fabsf(d[kz])) kz = 1;
     if (fabsf(d.z) > fabsf(d[kz])) kz = 2;

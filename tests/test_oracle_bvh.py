@@ -1,0 +1,204 @@
+"""Pin the oracle's Morton codes, LBVH (Karras 2012) and traversal.
+
+Morton codes are recomputed here with numpy float32 in the same operation
+order; the hierarchy is checked against the binary radix tree built top-down
+by splitting at the highest differing key bit (what Karras' bottom-up
+construction must reproduce); traversal is checked against brute force over
+every triangle, and against closed-form axis-aligned hits."""
+import numpy as np
+import pytest
+
+import oracle_lib
+
+LEAF = 0x80000000
+
+
+def soup(n, seed, spread=10.0, size=0.5):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (n, 1, 3))
+    return (c + rng.uniform(-size, size, (n, 3, 3))).astype(np.float32)
+
+
+def expand10(v):
+    v = v.astype(np.uint64) & 0x3FF
+    v = (v | (v << 16)) & 0x030000FF
+    v = (v | (v << 8)) & 0x0300F00F
+    v = (v | (v << 4)) & 0x030C30C3
+    v = (v | (v << 2)) & 0x09249249
+    return v.astype(np.uint32)
+
+
+def morton_numpy(pos):
+    p = pos.astype(np.float32)
+    c = ((p[:, 0] + p[:, 1]) + p[:, 2]) * np.float32(1.0 / 3.0)
+    lo, hi = c.min(0), c.max(0)
+    ext = hi - lo
+    with np.errstate(invalid="ignore", divide="ignore"):
+        n = np.where(ext > 0, (c - lo) / np.where(ext > 0, ext, 1), np.float32(0))
+    q = np.minimum(np.maximum(n * np.float32(1024.0), np.float32(0)), np.float32(1023)).astype(np.uint32)
+    return (expand10(q[:, 0]) << 2) | (expand10(q[:, 1]) << 1) | expand10(q[:, 2])
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (7, 1), (1000, 2), (5000, 3)])
+def test_morton_matches_numpy_restatement(n, seed):
+    pos = soup(n, seed)
+    codes, cb = oracle_lib.morton(pos)
+    assert np.array_equal(codes, morton_numpy(pos))
+    assert codes.max() < (1 << 30)
+
+
+def radix_tree(keys):
+    """Top-down binary radix tree over (key, index) augmented keys; returns
+    {(first, last): (left, right)} with children as ("leaf", i) or (first, last)."""
+    aug = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    out = {}
+
+    def build(a, b):
+        if a == b:
+            return ("leaf", a)
+        x = aug[a] ^ aug[b]
+        bit = x.bit_length() - 1
+        split = a
+        while split < b and not ((aug[split + 1] >> bit) & 1):
+            split += 1
+        left = build(a, split)
+        right = build(split + 1, b)
+        out[(a, b)] = (left, right)
+        return (a, b)
+
+    build(0, len(keys) - 1)
+    return out
+
+
+def karras_to_ranges(left, right, n):
+    """Map each Karras internal node to the (first, last) leaf range it covers."""
+    ranges = {}
+
+    def cover(ref):
+        if ref & LEAF:
+            k = ref & ~LEAF
+            return (k, k)
+        if ref in ranges:
+            return ranges[ref]
+        a = cover(int(left[ref]))
+        b = cover(int(right[ref]))
+        ranges[ref] = (a[0], b[1])
+        return ranges[ref]
+
+    for i in range(n - 1):
+        cover(i)
+    return ranges
+
+
+@pytest.mark.parametrize("n,seed,dup", [(2, 1, 1), (3, 2, 1), (64, 3, 1), (500, 4, 1), (60, 5, 4), (300, 6, 10)])
+def test_lbvh_equals_binary_radix_tree(n, seed, dup):
+    pos = np.repeat(soup(n // dup, seed), dup, axis=0) if dup > 1 else soup(n, seed)
+    n = pos.shape[0]
+    r = oracle_lib.lbvh(pos)
+    codes, order = r["codes"], r["order"]
+    raw = morton_numpy(pos)
+    # stable sort by code, ties by triangle index
+    assert np.array_equal(order, np.argsort(raw, kind="stable"))
+    assert np.array_equal(codes, raw[order])
+    ranges = karras_to_ranges(r["left"], r["right"], n)
+    tree = radix_tree(codes)
+    assert sorted(ranges.values()) == sorted(tree.keys())
+    inv = {v: k for k, v in ranges.items()}
+    for i in range(n - 1):
+        rng = ranges[i]
+        tl, tr = tree[rng]
+
+        def ref_of(t):
+            return (LEAF | t[1]) if t[0] == "leaf" else inv[t]
+
+        assert int(r["left"][i]) == ref_of(tl)
+        assert int(r["right"][i]) == ref_of(tr)
+    assert ranges[0] == (0, n - 1)  # node 0 is the root
+
+
+def test_lbvh_boxes_contain_children_and_triangles():
+    pos = soup(3000, 9)
+    r = oracle_lib.lbvh(pos)
+    n = pos.shape[0]
+    lo, hi = r["lo"], r["hi"]
+    tri = pos[r["order"]]
+    assert np.array_equal(lo[n - 1:], tri.min(1))
+    assert np.array_equal(hi[n - 1:], tri.max(1))
+    for i in range(n - 1):
+        for c in (int(r["left"][i]), int(r["right"][i])):
+            j = (n - 1 + (c & ~LEAF)) if c & LEAF else c
+            assert np.all(lo[i] <= lo[j]) and np.all(hi[i] >= hi[j])
+            assert int(r["parent"][j]) == i
+    # each internal box is exactly the union of its children
+    for i in range(n - 1):
+        js = [((n - 1 + (c & ~LEAF)) if c & LEAF else c) for c in (int(r["left"][i]), int(r["right"][i]))]
+        assert np.array_equal(lo[i], np.minimum(lo[js[0]], lo[js[1]]))
+        assert np.array_equal(hi[i], np.maximum(hi[js[0]], hi[js[1]]))
+
+
+def rays_for(pos, n, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = pos.reshape(-1, 3).min(0), pos.reshape(-1, 3).max(0)
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3], r[:, 3], r[:, 4:7], r[:, 7] = o, 1e-4, d, 3.4e38
+    r[::5, 7] = 1.5
+    return r
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (50, 3), (2000, 4)])
+def test_traversal_equals_brute_force(n, seed):
+    pos = soup(n, seed, spread=3.0, size=1.0)
+    rays = rays_for(pos, 3000, seed + 100)
+    bvh = oracle_lib.Bvh(pos)
+    hits, counts = bvh.closest(rays)
+    brute = oracle_lib.brute_closest(pos, rays)
+    assert np.array_equal(hits.view(np.uint32), brute.view(np.uint32))
+    occ, _ = bvh.any(rays)
+    assert np.array_equal(occ, brute[:, 1].view(np.uint32) != 0xFFFFFFFF)
+    # canonical counts: 2 records per visited internal node, bounded by the tree
+    assert np.all(counts[:, 0] % 2 == 0)
+    assert np.all(counts[:, 0] <= 2 * max(n - 1, 0))
+    assert np.all(counts[:, 1] <= n)
+
+
+def test_axis_aligned_closed_form():
+    # unit square in the plane z = 2 made of two triangles; rays along +z
+    pos = np.array([[[0, 0, 2], [1, 0, 2], [1, 1, 2]], [[0, 0, 2], [1, 1, 2], [0, 1, 2]]], np.float32)
+    xs = np.array([0.75, 0.25, 0.5, 1.5, -0.1], np.float32)
+    ys = np.array([0.1, 0.6, 0.5, 0.5, 0.5], np.float32)
+    rays = np.zeros((5, 8), np.float32)
+    rays[:, 0], rays[:, 1], rays[:, 2] = xs, ys, 0.0
+    rays[:, 3], rays[:, 6], rays[:, 7] = 1e-4, 1.0, 3.4e38
+    hits, _ = oracle_lib.Bvh(pos).closest(rays)
+    ids = hits[:, 1].view(np.uint32)
+    assert np.all(hits[:3, 0] == 2.0)
+    assert ids[0] == 0 and ids[1] == 1
+    assert ids[2] == 0  # on the shared diagonal both triangles hit at t = 2: lower id wins
+    assert ids[3] == 0xFFFFFFFF and ids[4] == 0xFFFFFFFF
+    # barycentrics reproduce the hit point: (1-u-v) v0 + u v1 + v v2
+    for i in range(3):
+        t = pos[ids[i]]
+        u, v = hits[i, 2], hits[i, 3]
+        p = (1 - u - v) * t[0] + u * t[1] + v * t[2]
+        assert np.allclose(p[:2], [xs[i], ys[i]], atol=1e-6)
+
+
+def test_watertight_shared_edges():
+    # rays through the shared edges/vertex of a fan must hit exactly one or more
+    # triangles, never fall through the cracks
+    ang = np.linspace(0, 2 * np.pi, 9)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), np.zeros(8)], 1)
+    pos = np.array([[[0, 0, 0], ring[i], ring[(i + 1) % 8]] for i in range(8)], np.float32)
+    rng = np.random.default_rng(0)
+    t = rng.uniform(0, 1, 4000).astype(np.float32)
+    k = rng.integers(0, 8, 4000)
+    pts = (ring[k] * t[:, None]).astype(np.float32)  # points on the spokes
+    rays = np.zeros((4000, 8), np.float32)
+    rays[:, :2] = pts[:, :2]
+    rays[:, 2] = -1.0
+    rays[:, 3], rays[:, 6], rays[:, 7] = 1e-4, 1.0, 3.4e38
+    hits, _ = oracle_lib.Bvh(pos).closest(rays)
+    assert np.all(hits[:, 1].view(np.uint32) != 0xFFFFFFFF)
