@@ -69,6 +69,21 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+FACADE = ["Scene.cpp", "SceneParser.cpp", "Renderer.cpp", "headless_main.cpp"]
+HEADLESS = PKG / "headless"
+
+
+def build_facade(verbose: bool = False) -> Path:
+    """The C++ host facade (Renderer/Scene/SceneParser/CUDABuffer) and the
+    headless driver, linked against liboxr.so."""
+    cpp = PKG / "cpp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(cpp), *[str(cpp / f) for f in FACADE], "-L", str(PKG),
+          "-loxr", "-Wl,-rpath,$ORIGIN", "-o", str(HEADLESS)])
+    if verbose:
+        print(f"built {HEADLESS}")
+    return HEADLESS
+
+
 def build_oracle(verbose: bool = False) -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     if Path("/root/reference/common/gdt/gdt/random/random.h").exists():
@@ -79,6 +94,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def build_all(verbose: bool = False) -> None:
     build_library(verbose)
+    build_facade(verbose)
     build_oracle(verbose)
 
 

@@ -1,0 +1,50 @@
+// Renderer with the reference's interface (core/include/Renderer.h:10-25,
+// 139-143): Renderer(const Scene*), Render(), Resize(vec2i),
+// DownloadPixels(uint32_t[]), SetCamera(Camera) and the public fields
+// denoiserOn, progressive and launchParams (numPixelSamples, maxBounce,
+// filterType, frame.frameID, frame.size).  OptiX is replaced by the HIP
+// library behind include/oxr.h; errors throw std::runtime_error.
+#pragma once
+#include <cstdint>
+
+#include "CUDABuffer.h"
+#include "Scene.h"
+
+struct LaunchParams {
+    int numPixelSamples = 1;
+    int maxBounce = 25;
+    int rrStart = 0;  // first bounce with Russian roulette (the reference: 0)
+    struct {
+        int frameID = 0;
+        vec2i size;
+    } frame;
+    FilterType filterType = FilterType::Gaussian;
+};
+
+class Renderer {
+public:
+    Renderer(const Scene* scene);
+    ~Renderer();
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    void Render();
+    void Resize(const vec2i& newSize);
+    void DownloadPixels(uint32_t h_pixels[]);
+    void SetCamera(const Camera& camera);
+
+    // statistics of the last Render() (device time, rays, per-stage times)
+    oxr_render_stats LastStats() const;
+    float BvhBuildMs() const { return buildMs; }
+
+    bool denoiserOn = false;
+    bool progressive = false;
+    LaunchParams launchParams;
+
+protected:
+    const Scene* scene;
+    uint32_t sceneId = 0;
+    float buildMs = 0.0f;
+    Camera lastSetCamera;
+    bool cameraSet = false;
+};
