@@ -14,6 +14,8 @@ LIB_PATH = Path(__file__).resolve().parent / "liboxr.so"
 
 OXR_FLAG_KEEP_PATH_RADIANCE = 1
 OXR_FLAG_COUNT_TRAVERSAL = 2
+OXR_FLAG_EAGER = 4
+OXR_FLAG_STAGE_TIMING = 8
 
 # MaterialType (core/include/Scene.h:59-68)
 DIFFUSE, CONDUCTOR, DIELECTRIC, PLASTIC, METAL_WORKFLOW, THIN_DIELECTRIC, CLEARCOAT_CONDUCTOR, DIFFUSE_TRANSMITTER = range(8)

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -172,6 +173,15 @@ struct oxr_ctx {
     size_t path_radiance_valid = 0;
     oxr_render_stats stats;
     std::vector<hipEvent_t> events;
+    int* batch = nullptr;
+    // captured batch graphs, keyed by the launch parameters they bake in
+    struct Graph {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec = nullptr;
+        std::vector<hipEvent_t> events;  // recorded between stages inside the graph
+        std::vector<int> stages;
+    };
+    std::vector<Graph> graphs;
 };
 
 namespace {
@@ -208,7 +218,16 @@ void free_scene(SceneDev* s) {
     delete s;
 }
 
+void drop_graphs(oxr_ctx* c) {
+    for (auto& g : c->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        for (auto e : g.events) (void)hipEventDestroy(e);
+    }
+    c->graphs.clear();
+}
+
 void free_wave(oxr_ctx* c) {
+    drop_graphs(c);
     dfree(c->ray_o); dfree(c->ray_d); dfree(c->thr); dfree(c->rad); dfree(c->aov_alb); dfree(c->aov_nrm);
     dfree(c->hits); dfree(c->sh_env); dfree(c->sh_light); dfree(c->q0); dfree(c->q1);
     c->cap = 0;
@@ -269,6 +288,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->trace_grid = c->num_cu * 4;
     c->shade_grid = c->num_cu * 8;
     if (dalloc(&c->ctr, kCtrCount) != hipSuccess || dalloc(&c->trav, 8) != hipSuccess ||
+        dalloc(&c->batch, 4) != hipSuccess ||
         dalloc(&c->overflow, 4) != hipSuccess ||
         dalloc(&c->gstack, (size_t)kStackGlobal * c->trace_grid * kTraceBlock) != hipSuccess)
         return bail(OXR_ERR_OOM);
@@ -290,7 +310,8 @@ int oxr_destroy(oxr_ctx* c) {
     c->scenes.clear();
     free_wave(c);
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
-    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance);
+    drop_graphs(c);
+    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance); dfree(c->batch);
     for (auto& e : c->events)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -584,6 +605,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     if (keep) {
         size_t need = (size_t)3 * prm->spp * npix;
         if (need > c->path_radiance_floats) {
+            drop_graphs(c);
             dfree(c->path_radiance);
             OXR_HIP(c, dalloc(&c->path_radiance, need));
             c->path_radiance_floats = need;
@@ -624,55 +646,117 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.color = c->color; p.albedo = c->albedo; p.normal = c->normal;
     p.path_radiance = keep ? c->path_radiance : nullptr;
 
+    p.batch = c->batch;
     OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
-    // stage timing: one event between consecutive launches
-    std::vector<int> stage;   // 0 generate 1 extend 2 shade 3 connect 4 accumulate 5 tonemap
+    const bool eager = (prm->flags & OXR_FLAG_EAGER) != 0;
+    const bool stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
+    oxr_render_stats& st = c->stats;
+    std::memset(&st, 0, sizeof(st));
+    double* stage_ms[] = {&st.generate_ms, &st.extend_ms, &st.shade_ms, &st.connect_ms, &st.accumulate_ms,
+                          &st.tonemap_ms};
+    // One batch: generate -> depth x (extend, shade, connect env, connect light) -> accumulate.
+    // mark(stage) records an event after the stage's launches.
+    auto enqueue_batch = [&](int nf, const std::function<hipError_t(int)>& mark) -> hipError_t {
+        hipError_t e;
+        if ((e = wave_generate(p, nf, c->stream)) != hipSuccess || (e = mark(0)) != hipSuccess) return e;
+        for (int b = 0; b < p.max_depth; b++) {
+            if ((e = wave_extend(p, b, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(1)) != hipSuccess)
+                return e;
+            if ((e = wave_shade(p, b, c->shade_grid, c->stream)) != hipSuccess || (e = mark(2)) != hipSuccess)
+                return e;
+            if ((e = wave_connect(p, 0, c->trace_grid, count, c->stream)) != hipSuccess ||
+                (e = wave_connect(p, 1, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(3)) != hipSuccess)
+                return e;
+        }
+        if ((e = wave_accumulate(p, nf, c->stream)) != hipSuccess) return e;
+        return mark(4);
+    };
+    auto graph_for = [&](int nf, oxr_ctx::Graph*& out) -> hipError_t {
+        std::vector<uint8_t> key(sizeof(WaveParams) + 3 * sizeof(int));
+        std::memcpy(key.data(), &p, sizeof(WaveParams));
+        int extra[3] = {nf, count ? 1 : 0, stage_timing ? 1 : 0};
+        std::memcpy(key.data() + sizeof(WaveParams), extra, sizeof(extra));
+        for (auto& g : c->graphs)
+            if (g.key == key) { out = &g; return hipSuccess; }
+        if (c->graphs.size() >= 4) drop_graphs(c);
+        oxr_ctx::Graph g;
+        g.key = key;
+        if (stage_timing) {  // events are created before the capture opens
+            g.events.resize(3 + 3 * (size_t)p.max_depth, nullptr);
+            for (auto& ev : g.events)
+                if (hipEventCreate(&ev) != hipSuccess) {
+                    for (auto x : g.events) if (x) (void)hipEventDestroy(x);
+                    return hipErrorOutOfMemory;
+                }
+        }
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        size_t used = 0;
+        if (stage_timing) (void)hipEventRecord(g.events[used++], c->stream);
+        auto mark = [&](int stage) -> hipError_t {
+            if (!stage_timing) return hipSuccess;
+            g.stages.push_back(stage);
+            return hipEventRecord(g.events[used++], c->stream);
+        };
+        hipError_t le = enqueue_batch(nf, mark);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(c->stream, &graph);
+        if (le != hipSuccess) e = le;
+        if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            for (auto ev : g.events) (void)hipEventDestroy(ev);
+            return e;
+        }
+        c->graphs.push_back(std::move(g));
+        out = &c->graphs.back();
+        return hipSuccess;
+    };
+
     size_t ev = 0;
-    auto mark = [&](int st) -> hipError_t {
-        if (ev + 1 >= c->events.size()) return hipSuccess;
-        stage.push_back(st);
+    std::vector<int> stage;
+    auto mark_eager = [&](int stg) -> hipError_t {
+        if (!stage_timing || ev + 1 >= c->events.size()) return hipSuccess;
+        stage.push_back(stg);
         return hipEventRecord(c->events[++ev], c->stream);
     };
-    OXR_HIP(c, hipEventRecord(c->events[0], c->stream));
+    hipEvent_t t0 = c->events[0], t1 = c->events[c->events.size() - 1];
+    OXR_HIP(c, hipEventRecord(t0, c->stream));
     int launches = 0;
     for (int f0 = 0; f0 < prm->spp; f0 += F) {
         int nf = std::min(F, prm->spp - f0);
-        int frame0 = c->frame_id + f0;
-        OXR_HIP(c, wave_generate(p, frame0, nf, c->stream));
-        OXR_HIP(c, mark(0));
-        launches++;
-        for (int b = 0; b < p.max_depth; b++) {
-            OXR_HIP(c, wave_extend(p, b, c->trace_grid, count, c->stream));
-            OXR_HIP(c, mark(1));
-            OXR_HIP(c, wave_shade(p, b, c->shade_grid, c->stream));
-            OXR_HIP(c, mark(2));
-            OXR_HIP(c, wave_connect(p, 0, c->trace_grid, count, c->stream));
-            OXR_HIP(c, wave_connect(p, 1, c->trace_grid, count, c->stream));
-            OXR_HIP(c, mark(3));
-            launches += 4;
+        OXR_HIP(c, wave_set_batch(p, c->frame_id + f0, f0, c->stream));
+        launches += 3 + 4 * p.max_depth;
+        if (eager) {
+            OXR_HIP(c, enqueue_batch(nf, mark_eager));
+            continue;
         }
-        OXR_HIP(c, wave_accumulate(p, frame0, nf, f0, c->stream));
-        OXR_HIP(c, mark(4));
-        launches++;
+        oxr_ctx::Graph* g = nullptr;
+        OXR_HIP(c, graph_for(nf, g));
+        OXR_HIP(c, hipGraphLaunch(g->exec, c->stream));
+        if (stage_timing) {
+            OXR_HIP(c, hipStreamSynchronize(c->stream));
+            for (size_t i = 0; i < g->stages.size(); i++) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, g->events[i], g->events[i + 1]) == hipSuccess) *stage_ms[g->stages[i]] += ms;
+            }
+        }
     }
     OXR_HIP(c, tonemap(c->color, c->rgba, npix, c->stream));
-    OXR_HIP(c, mark(5));
     launches++;
+    OXR_HIP(c, hipEventRecord(t1, c->stream));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long tv[8];
     OXR_HIP(c, hipMemcpy(tv, c->trav, sizeof(tv), hipMemcpyDeviceToHost));
     int ov = 0;
     OXR_HIP(c, hipMemcpy(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost));
-    oxr_render_stats& st = c->stats;
-    std::memset(&st, 0, sizeof(st));
     float total = 0.0f;
-    (void)hipEventElapsedTime(&total, c->events[0], c->events[ev]);
+    (void)hipEventElapsedTime(&total, t0, t1);
     st.render_ms = total;
     for (size_t i = 0; i < stage.size(); i++) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, c->events[i], c->events[i + 1]);
-        double* dst[] = {&st.generate_ms, &st.extend_ms, &st.shade_ms, &st.connect_ms, &st.accumulate_ms, &st.tonemap_ms};
-        *dst[stage[i]] += ms;
+        *stage_ms[stage[i]] += ms;
     }
     st.ext_node_fetches = tv[0];
     st.ext_tri_tests = tv[1];

@@ -94,6 +94,7 @@ struct WaveParams {
     float4* sh_env;    // 3 float4 per shadow ray
     float4* sh_light;
     int* ctr;          // device counters
+    int* batch;        // device batch block: [0] first frameID, [1] first frame index in path_radiance
     unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
     uint32_t* gstack;               // traversal stack spill, kStackGlobal * grid * kTraceBlock
     int* overflow;
@@ -116,10 +117,13 @@ enum CtrSlot {
     kCtrCount = 16
 };
 
-hipError_t wave_generate(const WaveParams& p, int frame0, int frames, hipStream_t s);
+// Writes the batch block; the stage kernels read it, so one captured graph
+// serves every batch of the same size.
+hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, hipStream_t s);
+hipError_t wave_generate(const WaveParams& p, int frames, hipStream_t s);
 hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s);
 hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s);
 hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
-hipError_t wave_accumulate(const WaveParams& p, int frame0, int frames, int path_frame_offset, hipStream_t s);
+hipError_t wave_accumulate(const WaveParams& p, int frames, hipStream_t s);
 
 }  // namespace oxr

@@ -180,8 +180,14 @@ __global__ void k_gather(const float* __restrict__ nrm, const uint32_t* __restri
 }
 
 // ------------------------------------------------------------ wavefront
-__global__ __launch_bounds__(256) void k_generate(WaveParams p, int frame0, int64_t npaths) {
+__global__ void k_set_batch(int* batch, int frame0, int offset) {
+    batch[0] = frame0;
+    batch[1] = offset;
+}
+
+__global__ __launch_bounds__(256) void k_generate(WaveParams p, int64_t npaths) {
     int64_t pid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int frame0 = p.batch[0];
     if (pid == 0) {
         p.ctr[kCtrQueue0] = (int)npaths;
         p.ctr[kCtrQueue1] = 0;
@@ -555,8 +561,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int bounce) {
 
 // devicePrograms.cu:499-531 with numPixelSamples = 1: a NaN radiance drops
 // the frame's sample (all three outputs become 0 for that frame).
-__global__ void k_accumulate(WaveParams p, int frame0, int frames, int path_frame_offset) {
+__global__ void k_accumulate(WaveParams p, int frames) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int frame0 = p.batch[0], path_frame_offset = p.batch[1];
     const int npix = p.width * p.height;
     if (pix >= npix) return;
     float4 c = p.color[pix], a = p.albedo[pix], nr = p.normal[pix];
@@ -640,9 +647,14 @@ hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const
     return hipGetLastError();
 }
 
-hipError_t wave_generate(const WaveParams& p, int frame0, int frames, hipStream_t s) {
+hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, hipStream_t s) {
+    k_set_batch<<<1, 1, 0, s>>>(p.batch, frame0, path_frame_offset);
+    return hipGetLastError();
+}
+
+hipError_t wave_generate(const WaveParams& p, int frames, hipStream_t s) {
     int64_t np = (int64_t)frames * p.width * p.height;
-    k_generate<<<blocks(np, 256), 256, 0, s>>>(p, frame0, np);
+    k_generate<<<blocks(np, 256), 256, 0, s>>>(p, np);
     return hipGetLastError();
 }
 
@@ -663,9 +675,9 @@ hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hi
     return hipGetLastError();
 }
 
-hipError_t wave_accumulate(const WaveParams& p, int frame0, int frames, int path_frame_offset, hipStream_t s) {
+hipError_t wave_accumulate(const WaveParams& p, int frames, hipStream_t s) {
     int npix = p.width * p.height;
-    k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, frame0, frames, path_frame_offset);
+    k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, frames);
     return hipGetLastError();
 }
 
