@@ -173,7 +173,7 @@ struct oxr_ctx {
     size_t path_radiance_valid = 0;
     oxr_render_stats stats;
     std::vector<hipEvent_t> events;
-    int* batch = nullptr;
+    ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
     // captured batch graphs, keyed by the launch parameters they bake in
     struct Graph {
         std::vector<uint8_t> key;
@@ -288,7 +288,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->trace_grid = c->num_cu * 4;
     c->shade_grid = c->num_cu * 8;
     if (dalloc(&c->ctr, kCtrCount) != hipSuccess || dalloc(&c->trav, 8) != hipSuccess ||
-        dalloc(&c->batch, 4) != hipSuccess ||
+        dalloc(&c->reqs, 256) != hipSuccess ||
         dalloc(&c->overflow, 4) != hipSuccess ||
         dalloc(&c->gstack, (size_t)kStackGlobal * c->trace_grid * kTraceBlock) != hipSuccess)
         return bail(OXR_ERR_OOM);
@@ -311,7 +311,7 @@ int oxr_destroy(oxr_ctx* c) {
     free_wave(c);
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
     drop_graphs(c);
-    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance); dfree(c->batch);
+    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance); dfree(c->reqs);
     for (auto& e : c->events)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -612,6 +612,24 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
         }
     }
     CamBasis cb = camera_basis(c->cam, W, H);
+    ReqDesc rq;
+    std::memset(&rq, 0, sizeof(rq));
+    rq.path_begin = 0;
+    rq.pix_begin = 0;
+    rq.width = W;
+    rq.height = H;
+    rq.frame0 = c->frame_id;
+    rq.frames = F;
+    rq.filter = prm->filter;
+    std::memcpy(rq.cam_pos, cb.pos, 12);
+    std::memcpy(rq.cam_dir, cb.dir, 12);
+    std::memcpy(rq.cam_h, cb.h, 12);
+    std::memcpy(rq.cam_v, cb.v, 12);
+    rq.color = c->color;
+    rq.albedo = c->albedo;
+    rq.normal = c->normal;
+    rq.path_radiance = keep ? c->path_radiance : nullptr;
+    OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
     std::memset(&p, 0, sizeof(p));
     p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
@@ -628,14 +646,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.env_h = s->env_h;
     p.lut_bsdf = c->lut_bsdf;
     p.lut_albedo = c->lut_albedo;
-    std::memcpy(p.cam_pos, cb.pos, 12);
-    std::memcpy(p.cam_dir, cb.dir, 12);
-    std::memcpy(p.cam_h, cb.h, 12);
-    std::memcpy(p.cam_v, cb.v, 12);
     p.cam_medium = c->cam.medium;
-    p.width = W;
-    p.height = H;
-    p.filter = prm->filter;
     p.max_depth = prm->max_depth > 0 ? prm->max_depth : 25;
     p.rr_start = prm->rr_start;
     p.ray_o = c->ray_o; p.ray_d = c->ray_d; p.thr = c->thr; p.rad = c->rad;
@@ -643,10 +654,9 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.queue[0] = c->q0; p.queue[1] = c->q1;
     p.hits = c->hits; p.sh_env = c->sh_env; p.sh_light = c->sh_light;
     p.ctr = c->ctr; p.traversal = c->trav; p.gstack = c->gstack; p.overflow = c->overflow;
-    p.color = c->color; p.albedo = c->albedo; p.normal = c->normal;
-    p.path_radiance = keep ? c->path_radiance : nullptr;
+    p.reqs = c->reqs;
+    p.nreq = 1;
 
-    p.batch = c->batch;
     OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
     const bool eager = (prm->flags & OXR_FLAG_EAGER) != 0;
     const bool stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
@@ -658,7 +668,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     // mark(stage) records an event after the stage's launches.
     auto enqueue_batch = [&](int nf, const std::function<hipError_t(int)>& mark) -> hipError_t {
         hipError_t e;
-        if ((e = wave_generate(p, nf, c->stream)) != hipSuccess || (e = mark(0)) != hipSuccess) return e;
+        if ((e = wave_generate(p, (int64_t)nf * npix, c->stream)) != hipSuccess || (e = mark(0)) != hipSuccess) return e;
         for (int b = 0; b < p.max_depth; b++) {
             if ((e = wave_extend(p, b, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(1)) != hipSuccess)
                 return e;
@@ -668,13 +678,13 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
                 (e = wave_connect(p, 1, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(3)) != hipSuccess)
                 return e;
         }
-        if ((e = wave_accumulate(p, nf, c->stream)) != hipSuccess) return e;
+        if ((e = wave_accumulate(p, npix, c->stream)) != hipSuccess) return e;
         return mark(4);
     };
     auto graph_for = [&](int nf, oxr_ctx::Graph*& out) -> hipError_t {
-        std::vector<uint8_t> key(sizeof(WaveParams) + 3 * sizeof(int));
+        std::vector<uint8_t> key(sizeof(WaveParams) + 4 * sizeof(int64_t));
         std::memcpy(key.data(), &p, sizeof(WaveParams));
-        int extra[3] = {nf, count ? 1 : 0, stage_timing ? 1 : 0};
+        int64_t extra[4] = {nf, npix, count ? 1 : 0, stage_timing ? 1 : 0};
         std::memcpy(key.data() + sizeof(WaveParams), extra, sizeof(extra));
         for (auto& g : c->graphs)
             if (g.key == key) { out = &g; return hipSuccess; }
@@ -725,7 +735,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     int launches = 0;
     for (int f0 = 0; f0 < prm->spp; f0 += F) {
         int nf = std::min(F, prm->spp - f0);
-        OXR_HIP(c, wave_set_batch(p, c->frame_id + f0, f0, c->stream));
+        OXR_HIP(c, wave_set_batch(p, c->frame_id + f0, f0, nf, c->stream));
         launches += 3 + 4 * p.max_depth;
         if (eager) {
             OXR_HIP(c, enqueue_batch(nf, mark_eager));

@@ -61,6 +61,23 @@ hipError_t material_batch(const oxr_material* d_mat, const float* d_lut_bsdf, co
 hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_mat_ids, const uint32_t* d_order,
                                     int64_t n, float* d_normals_sorted, uint32_t* d_mat_sorted, hipStream_t s);
 
+// One render request's share of a wavefront batch.  A batch holds `frames`
+// frames of every pixel of each request, laid out request after request:
+// path id = path_begin + s * width * height + pixel.
+struct ReqDesc {
+    int64_t path_begin;    // first path id of this request in the batch
+    int64_t pix_begin;     // first pixel of this request among the batch's pixels
+    int width, height;
+    int frame0, frames;    // frameIDs frame0 .. frame0 + frames - 1
+    int filter;
+    int path_frame_offset; // frame index of frame0 in path_radiance (debug output)
+    float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
+    float4* color;
+    float4* albedo;
+    float4* normal;
+    float* path_radiance;  // optional, frame-major 3 floats per path
+};
+
 // Wavefront integrator state for one context.
 struct WaveParams {
     // scene
@@ -77,11 +94,8 @@ struct WaveParams {
     int env_w, env_h;
     const float* lut_bsdf;
     const float* lut_albedo;
-    // camera (Renderer::SetCamera, Renderer.cpp:796-813)
-    float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
     int cam_medium;
-    int width, height;
-    int filter, max_depth, rr_start;
+    int max_depth, rr_start;
     // path state, capacity paths
     float4* ray_o;     // origin, pre_pdf
     float4* ray_d;     // direction, rng state bits
@@ -94,15 +108,11 @@ struct WaveParams {
     float4* sh_env;    // 3 float4 per shadow ray
     float4* sh_light;
     int* ctr;          // device counters
-    int* batch;        // device batch block: [0] first frameID, [1] first frame index in path_radiance
+    const ReqDesc* reqs;  // device table of the batch's requests
+    int nreq;
     unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
     uint32_t* gstack;               // traversal stack spill, kStackGlobal * grid * kTraceBlock
     int* overflow;
-    // accumulation targets
-    float4* color;
-    float4* albedo;
-    float4* normal;
-    float* path_radiance;  // optional, frame-major 3 floats per path
 };
 
 enum CtrSlot {
@@ -117,13 +127,13 @@ enum CtrSlot {
     kCtrCount = 16
 };
 
-// Writes the batch block; the stage kernels read it, so one captured graph
-// serves every batch of the same size.
-hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, hipStream_t s);
-hipError_t wave_generate(const WaveParams& p, int frames, hipStream_t s);
+// Rewrites frame0 / path_frame_offset of request 0 on the device, so one
+// captured graph serves every batch of the same size.
+hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, int frames, hipStream_t s);
+hipError_t wave_generate(const WaveParams& p, int64_t npaths, hipStream_t s);
 hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s);
 hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s);
 hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
-hipError_t wave_accumulate(const WaveParams& p, int frames, hipStream_t s);
+hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
 
 }  // namespace oxr

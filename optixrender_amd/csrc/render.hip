@@ -180,30 +180,44 @@ __global__ void k_gather(const float* __restrict__ nrm, const uint32_t* __restri
 }
 
 // ------------------------------------------------------------ wavefront
-__global__ void k_set_batch(int* batch, int frame0, int offset) {
-    batch[0] = frame0;
-    batch[1] = offset;
+__global__ void k_set_batch(ReqDesc* r, int frame0, int offset, int frames) {
+    r->frame0 = frame0;
+    r->path_frame_offset = offset;
+    r->frames = frames;
+}
+
+// Request owning a path (or pixel) index: the last descriptor whose begin <= i.
+template <bool PIX>
+OXR_D int find_req(const ReqDesc* r, int n, int64_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        int64_t b = PIX ? r[mid].pix_begin : r[mid].path_begin;
+        if (b <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
 __global__ __launch_bounds__(256) void k_generate(WaveParams p, int64_t npaths) {
     int64_t pid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int frame0 = p.batch[0];
     if (pid == 0) {
         p.ctr[kCtrQueue0] = (int)npaths;
         p.ctr[kCtrQueue1] = 0;
         p.ctr[kCtrFetchExtend] = 0;
     }
     if (pid >= npaths) return;
-    const int npix = p.width * p.height;
-    int s = (int)(pid / npix), pix = (int)(pid % npix);
-    int ix = pix % p.width, iy = pix / p.width;
-    Rng rng{lcg_seed((uint32_t)(ix + p.width * iy), (uint32_t)(frame0 + s))};
+    const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, pid)];
+    const int64_t local = pid - q.path_begin;
+    const int npix = q.width * q.height;
+    int s = (int)(local / npix), pix = (int)(local % npix);
+    int ix = pix % q.width, iy = pix / q.width;
+    Rng rng{lcg_seed((uint32_t)(ix + q.width * iy), (uint32_t)(q.frame0 + s))};
     float j0 = rng(), j1 = rng();
-    f2 jit = filter_jitter(f2{j0, j1}, p.filter);                         // devicePrograms.cu:220-227
-    float sx = ((float)ix + 0.5f + jit.x) / (float)p.width;
-    float sy = ((float)iy + 0.5f + jit.y) / (float)p.height;
-    f3 dir = normalize(make3(p.cam_dir) + (sx - 0.5f) * make3(p.cam_h) + (sy - 0.5f) * make3(p.cam_v));
-    p.ray_o[pid] = make_float4(p.cam_pos[0], p.cam_pos[1], p.cam_pos[2], 0.0f);
+    f2 jit = filter_jitter(f2{j0, j1}, q.filter);                         // devicePrograms.cu:220-227
+    float sx = ((float)ix + 0.5f + jit.x) / (float)q.width;
+    float sy = ((float)iy + 0.5f + jit.y) / (float)q.height;
+    f3 dir = normalize(make3(q.cam_dir) + (sx - 0.5f) * make3(q.cam_h) + (sy - 0.5f) * make3(q.cam_v));
+    p.ray_o[pid] = make_float4(q.cam_pos[0], q.cam_pos[1], q.cam_pos[2], 0.0f);
     p.ray_d[pid] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
     p.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     p.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -561,15 +575,16 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int bounce) {
 
 // devicePrograms.cu:499-531 with numPixelSamples = 1: a NaN radiance drops
 // the frame's sample (all three outputs become 0 for that frame).
-__global__ void k_accumulate(WaveParams p, int frames) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int frame0 = p.batch[0], path_frame_offset = p.batch[1];
-    const int npix = p.width * p.height;
-    if (pix >= npix) return;
-    float4 c = p.color[pix], a = p.albedo[pix], nr = p.normal[pix];
-    for (int s = 0; s < frames; s++) {
-        size_t pid = (size_t)s * npix + pix;
-        int frame = frame0 + s;
+__global__ void k_accumulate(WaveParams p, int64_t npix_total) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= npix_total) return;
+    const ReqDesc& q = p.reqs[find_req<true>(p.reqs, p.nreq, t)];
+    const int64_t pix = t - q.pix_begin;
+    const int64_t npix = (int64_t)q.width * q.height;
+    float4 c = q.color[pix], a = q.albedo[pix], nr = q.normal[pix];
+    for (int s = 0; s < q.frames; s++) {
+        size_t pid = (size_t)(q.path_begin + s * npix + pix);
+        int frame = q.frame0 + s;
         float4 r = p.rad[pid];
         float4 al = p.aov_alb[pid], no = p.aov_nrm[pid];
         float4 x = make_float4(r.x, r.y, r.z, 1.0f);
@@ -578,8 +593,8 @@ __global__ void k_accumulate(WaveParams p, int frames) {
             al = x;
             no = x;
         }
-        if (p.path_radiance) {
-            float* pr = p.path_radiance + 3 * ((size_t)(path_frame_offset + s) * npix + pix);
+        if (q.path_radiance) {
+            float* pr = q.path_radiance + 3 * ((size_t)(q.path_frame_offset + s) * npix + pix);
             pr[0] = r.x; pr[1] = r.y; pr[2] = r.z;
         }
         if (frame > 0) {
@@ -592,9 +607,9 @@ __global__ void k_accumulate(WaveParams p, int frames) {
         a = al;
         nr = no;
     }
-    p.color[pix] = c;
-    p.albedo[pix] = a;
-    p.normal[pix] = nr;
+    q.color[pix] = c;
+    q.albedo[pix] = a;
+    q.normal[pix] = nr;
 }
 
 inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -647,13 +662,12 @@ hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const
     return hipGetLastError();
 }
 
-hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, hipStream_t s) {
-    k_set_batch<<<1, 1, 0, s>>>(p.batch, frame0, path_frame_offset);
+hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, int frames, hipStream_t s) {
+    k_set_batch<<<1, 1, 0, s>>>(const_cast<ReqDesc*>(p.reqs), frame0, path_frame_offset, frames);
     return hipGetLastError();
 }
 
-hipError_t wave_generate(const WaveParams& p, int frames, hipStream_t s) {
-    int64_t np = (int64_t)frames * p.width * p.height;
+hipError_t wave_generate(const WaveParams& p, int64_t np, hipStream_t s) {
     k_generate<<<blocks(np, 256), 256, 0, s>>>(p, np);
     return hipGetLastError();
 }
@@ -675,9 +689,8 @@ hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hi
     return hipGetLastError();
 }
 
-hipError_t wave_accumulate(const WaveParams& p, int frames, hipStream_t s) {
-    int npix = p.width * p.height;
-    k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, frames);
+hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s) {
+    k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, npix);
     return hipGetLastError();
 }
 
