@@ -176,6 +176,55 @@ int oxr_download_aov(oxr_ctx* ctx, int which, float* out_rgba, size_t count);
    frame-major.  Needs OXR_FLAG_KEEP_PATH_RADIANCE. */
 int oxr_download_path_radiance(oxr_ctx* ctx, float* out, size_t count);
 
+/* ---- render server: continuous batching over resident scenes ----
+   No counterpart in the reference (its viewer renders one frame per Render()
+   call, main.cpp:66-128).  A scheduler thread admits requests FIFO while a
+   slot is free and their buffers fit in the HBM budget, and runs the active
+   requests of one (scene, max_depth, rr_start, camera medium) group through
+   shared wavefronts, handing out frames round-robin up to tick_paths paths
+   per tick.  A request's image equals a single oxr_render of the same
+   request (non-progressive, frames 0..spp-1). */
+typedef struct oxr_server oxr_server;
+typedef struct oxr_server_config {
+    size_t hbm_budget_bytes;   /* wavefront buffers + all admitted requests' buffers */
+    int32_t max_active;        /* admitted requests at a time (<= 256) */
+    int32_t reserved;
+    int64_t tick_paths;        /* paths per shared wavefront tick */
+    int64_t max_frame_pixels;  /* largest request frame accepted (width*height) */
+} oxr_server_config;
+typedef struct oxr_request {
+    uint32_t scene_id;
+    int32_t width, height, spp, max_depth, rr_start, filter;
+    int32_t reserved;
+    oxr_camera camera;
+} oxr_request;
+/* request states reported by oxr_poll */
+#define OXR_REQ_PENDING 0
+#define OXR_REQ_ACTIVE 1
+#define OXR_REQ_DONE 2
+#define OXR_REQ_CANCELLED 3
+#define OXR_REQ_FAILED 4
+typedef struct oxr_server_stats {
+    uint64_t submitted, completed, cancelled, rejected, ticks, paths;
+    double busy_ms, wall_ms;
+    double p50_ms, p99_ms, mean_ms;  /* submit -> done latency of completed requests */
+    int32_t active, pending;
+    size_t bytes_in_use, budget_bytes;
+} oxr_server_stats;
+int oxr_server_create(oxr_ctx* ctx, const oxr_server_config* cfg, oxr_server** out);
+int oxr_server_destroy(oxr_server* s);
+const char* oxr_server_last_error(const oxr_server* s);
+int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id);
+int oxr_poll(oxr_server* s, uint64_t id, int* state);
+/* blocks until done (0), failed/cancelled or timed out (non-zero); timeout_ms < 0 waits forever */
+int oxr_wait(oxr_server* s, uint64_t id, int timeout_ms);
+int oxr_cancel(oxr_server* s, uint64_t id);
+int oxr_result_rgba8(oxr_server* s, uint64_t id, uint32_t* out, size_t count);
+int oxr_result_color(oxr_server* s, uint64_t id, float* out_rgba, size_t count);
+/* frees a finished request's buffers and forgets it */
+int oxr_release_request(oxr_server* s, uint64_t id);
+int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out);
+
 /* ---- building blocks exposed for parity tests ---- */
 /* ComputeFinalPixelColorsKernel (PostProcessing.cu:26-48) over n host float4. */
 int oxr_tonemap_host(oxr_ctx* ctx, const float* rgba, int64_t n, uint32_t* out);

@@ -211,6 +211,46 @@ class RenderStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_active", C.c_int32),
+        ("reserved", C.c_int32),
+        ("tick_paths", C.c_int64),
+        ("max_frame_pixels", C.c_int64),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("scene_id", C.c_uint32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("spp", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("rr_start", C.c_int32),
+        ("filter", C.c_int32),
+        ("reserved", C.c_int32),
+        ("camera", Camera),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64), ("rejected", C.c_uint64),
+        ("ticks", C.c_uint64), ("paths", C.c_uint64),
+        ("busy_ms", C.c_double), ("wall_ms", C.c_double),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+        ("active", C.c_int32), ("pending", C.c_int32),
+        ("bytes_in_use", C.c_size_t), ("budget_bytes", C.c_size_t),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
+
 _P = C.c_void_p
 _I64 = C.c_int64
 # name -> (restype, argtypes); every symbol include/oxr.h declares.
@@ -238,6 +278,17 @@ SIGNATURES = {
     "oxr_download_rgba8": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_download_aov": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "oxr_download_path_radiance": (C.c_int, [_P, _P, C.c_size_t]),
+    "oxr_server_create": (C.c_int, [_P, _P, C.POINTER(C.c_void_p)]),
+    "oxr_server_destroy": (C.c_int, [_P]),
+    "oxr_server_last_error": (C.c_char_p, [_P]),
+    "oxr_submit": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
+    "oxr_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    "oxr_wait": (C.c_int, [_P, C.c_uint64, C.c_int]),
+    "oxr_cancel": (C.c_int, [_P, C.c_uint64]),
+    "oxr_result_rgba8": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
+    "oxr_result_color": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
+    "oxr_release_request": (C.c_int, [_P, C.c_uint64]),
+    "oxr_server_get_stats": (C.c_int, [_P, _P]),
     "oxr_tonemap_host": (C.c_int, [_P, _P, _I64, _P]),
     "oxr_kulla_conty": (C.c_int, [_P, _P]),
     "oxr_material_batch": (C.c_int, [_P, _P, C.c_int, _P, _I64, _P]),

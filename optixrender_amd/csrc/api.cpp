@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "common.h"
+#include "internal.h"
 
 using namespace oxr;
 
@@ -99,18 +99,19 @@ const KcTable& kc_table() {
     return t;
 }
 
+}  // namespace
+
 // ------------------------------------------------------------ camera
 // Renderer::SetCamera (Renderer.cpp:796-813).
-struct CamBasis { float pos[3], dir[3], h[3], v[3]; };
 inline HV hcross(HV a, HV b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
-CamBasis camera_basis(const oxr_camera& c, int w, int h) {
+oxr::CamBasis oxr::camera_basis(const oxr_camera& c, int w, int h) {
     HV from{c.from[0], c.from[1], c.from[2]}, at{c.at[0], c.at[1], c.at[2]}, up{c.up[0], c.up[1], c.up[2]};
     HV dir = hnorm(hsub(at, from));
     float cf = c.cos_fovy > 0.0f ? c.cos_fovy : 0.66f;
     float aspect = (float)w / (float)h;
     HV hor = hmul(cf * aspect, hnorm(hcross(dir, up)));
     HV ver = hmul(cf, hnorm(hcross(hor, dir)));
-    CamBasis b;
+    oxr::CamBasis b;
     b.pos[0] = from.x; b.pos[1] = from.y; b.pos[2] = from.z;
     b.dir[0] = dir.x; b.dir[1] = dir.y; b.dir[2] = dir.z;
     b.h[0] = hor.x; b.h[1] = hor.y; b.h[2] = hor.z;
@@ -118,71 +119,8 @@ CamBasis camera_basis(const oxr_camera& c, int w, int h) {
     return b;
 }
 
-struct SceneDev {
-    int64_t n = 0;
-    float* positions = nullptr;
-    float* normals = nullptr;
-    uint32_t* mat_ids = nullptr;
-    float* normals_sorted = nullptr;
-    uint32_t* mat_sorted = nullptr;
-    oxr_material* mats = nullptr;
-    int n_mats = 0;
-    oxr_light* lights = nullptr;
-    int n_lights = 0;
-    oxr_medium* media = nullptr;
-    int n_media = 0;
-    float4* env = nullptr;
-    float4* env_cache = nullptr;
-    int env_w = 0, env_h = 0;
-    BvhBuffers bvh;
-    bool built = false;
-    float build_ms = 0.0f;
-};
 
-}  // namespace
 
-struct oxr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int num_cu = 256;
-    std::string err;
-    std::map<uint32_t, SceneDev*> scenes;
-    uint32_t next_id = 1;
-    uint32_t cur = 0;
-    float* lut_bsdf = nullptr;
-    float* lut_albedo = nullptr;
-    int width = 0, height = 0;
-    float4* color = nullptr;
-    float4* albedo = nullptr;
-    float4* normal = nullptr;
-    uint32_t* rgba = nullptr;
-    oxr_camera cam;
-    int frame_id = 0;
-    // wavefront buffers
-    int64_t cap = 0;
-    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
-    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr;
-    int* ctr = nullptr;
-    unsigned long long* trav = nullptr;
-    uint32_t* gstack = nullptr;
-    int* overflow = nullptr;
-    int trace_grid = 0, shade_grid = 0;
-    float* path_radiance = nullptr;
-    size_t path_radiance_floats = 0;
-    size_t path_radiance_valid = 0;
-    oxr_render_stats stats;
-    std::vector<hipEvent_t> events;
-    ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
-    // captured batch graphs, keyed by the launch parameters they bake in
-    struct Graph {
-        std::vector<uint8_t> key;
-        hipGraphExec_t exec = nullptr;
-        std::vector<hipEvent_t> events;  // recorded between stages inside the graph
-        std::vector<int> stages;
-    };
-    std::vector<Graph> graphs;
-};
 
 namespace {
 
@@ -257,6 +195,24 @@ SceneDev* selected(oxr_ctx* c) {
 }
 
 }  // namespace
+
+void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, const float* lut_albedo) {
+    std::memset(&p, 0, sizeof(p));
+    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
+    p.normals = s->normals_sorted;
+    p.mat_ids = s->mat_sorted;
+    p.mats = s->mats;
+    p.lights = s->lights;
+    p.n_lights = s->n_lights;
+    p.media = s->media;
+    p.n_media = s->n_media;
+    p.env = s->env;
+    p.env_cache = s->env_cache;
+    p.env_w = s->env_w;
+    p.env_h = s->env_h;
+    p.lut_bsdf = lut_bsdf;
+    p.lut_albedo = lut_albedo;
+}
 
 extern "C" {
 
@@ -631,21 +587,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     rq.path_radiance = keep ? c->path_radiance : nullptr;
     OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
-    std::memset(&p, 0, sizeof(p));
-    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
-    p.normals = s->normals_sorted;
-    p.mat_ids = s->mat_sorted;
-    p.mats = s->mats;
-    p.lights = s->lights;
-    p.n_lights = s->n_lights;
-    p.media = s->media;
-    p.n_media = s->n_media;
-    p.env = s->env;
-    p.env_cache = s->env_cache;
-    p.env_w = s->env_w;
-    p.env_h = s->env_h;
-    p.lut_bsdf = c->lut_bsdf;
-    p.lut_albedo = c->lut_albedo;
+    fill_scene_params(p, s, c->lut_bsdf, c->lut_albedo);
     p.cam_medium = c->cam.medium;
     p.max_depth = prm->max_depth > 0 ? prm->max_depth : 25;
     p.rr_start = prm->rr_start;

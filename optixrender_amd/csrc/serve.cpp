@@ -1,0 +1,489 @@
+// Render server: continuous batching of render requests over resident scenes.
+//
+// The reference renders one frame per Render() call from the viewer loop
+// (main.cpp:66-128); this server is new.  Scenes and their BVHs stay resident
+// in the context; each request gets its own accumulation buffers (colour,
+// albedo, normal, RGBA8).  A scheduler thread repeatedly
+//   1. admits pending requests (FIFO) while a slot is free and their buffers
+//      fit in the HBM budget left over by the shared wavefront buffers,
+//   2. takes the active requests that share (scene, max depth, RR start,
+//      camera medium) with the oldest one and hands out frames round-robin up
+//      to the tick's path budget (every request gets at least one frame),
+//   3. runs one shared wavefront over all of them -- generate, depth x
+//      (extend, shade, connect), accumulate -- and tone-maps the requests that
+//      finished, recording their latency.
+// A path is (pixel, frameID) of its own request, so a request's image does
+// not depend on what it shared the device with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "internal.h"
+
+using namespace oxr;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+enum State { kPending = 0, kActive = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
+
+struct Request {
+    uint64_t id;
+    oxr_request r;
+    SceneDev* scene;
+    State state = kPending;
+    int frames_done = 0;
+    size_t bytes = 0;
+    float4 *color = nullptr, *albedo = nullptr, *normal = nullptr;
+    uint32_t* rgba = nullptr;
+    Clock::time_point submitted, finished;
+    std::string error;
+};
+
+size_t request_bytes(const oxr_request& r) { return (size_t)r.width * r.height * (3 * 16 + 4); }
+
+}  // namespace
+
+struct oxr_server {
+    oxr_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, wave_bytes = 0, in_use = 0;
+    int max_active = 0;
+    int64_t tick_paths = 0, cap = 0;
+    int trace_grid = 0, shade_grid = 0;
+    // shared wavefront buffers
+    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
+    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr, *gstack = nullptr;
+    int *ctr = nullptr, *overflow = nullptr;
+    unsigned long long* trav = nullptr;
+    ReqDesc* reqs_dev = nullptr;
+    ReqDesc* reqs_host = nullptr;  // pinned staging
+    int max_reqs = 0;
+    // requests
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<Request*> pending;
+    std::vector<Request*> active;
+    std::unordered_map<uint64_t, Request*> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    uint64_t submitted = 0, completed = 0, cancelled = 0, rejected = 0, ticks = 0, paths = 0;
+    double busy_ms = 0.0;
+    std::vector<double> latencies;
+    Clock::time_point started;
+    std::string err;
+};
+
+namespace {
+
+template <typename T>
+hipError_t salloc(T** p, size_t n) {
+    *p = nullptr;
+    return n ? hipMalloc((void**)p, n * sizeof(T)) : hipSuccess;
+}
+template <typename T>
+void sfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_request_buffers(oxr_server* s, Request* q) {
+    sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba);
+    if (q->bytes) s->in_use -= q->bytes;
+    q->bytes = 0;
+}
+
+bool same_group(const Request* a, const Request* b) {
+    return a->scene == b->scene && a->r.max_depth == b->r.max_depth && a->r.rr_start == b->r.rr_start &&
+           a->r.camera.medium == b->r.camera.medium;
+}
+
+// Admission, under the lock: FIFO, stop at the first request that does not fit.
+void admit(oxr_server* s) {
+    while (!s->pending.empty() && (int)s->active.size() < s->max_active) {
+        Request* q = s->pending.front();
+        if (q->state == kCancelled) { s->pending.pop_front(); continue; }
+        size_t need = request_bytes(q->r);
+        if (s->in_use + need + s->wave_bytes > s->budget) break;
+        hipError_t e;
+        size_t np = (size_t)q->r.width * q->r.height;
+        if ((e = salloc(&q->color, np)) != hipSuccess || (e = salloc(&q->albedo, np)) != hipSuccess ||
+            (e = salloc(&q->normal, np)) != hipSuccess || (e = salloc(&q->rgba, np)) != hipSuccess ||
+            (e = hipMemsetAsync(q->color, 0, 16 * np, s->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(q->albedo, 0, 16 * np, s->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(q->normal, 0, 16 * np, s->stream)) != hipSuccess) {
+            free_request_buffers(s, q);
+            q->state = kFailed;
+            q->error = std::string("allocation: ") + hipGetErrorString(e);
+            s->pending.pop_front();
+            s->done_cv.notify_all();
+            continue;
+        }
+        q->bytes = need;
+        s->in_use += need;
+        q->state = kActive;
+        s->pending.pop_front();
+        s->active.push_back(q);
+    }
+}
+
+hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int>& frames) {
+    oxr_ctx* c = s->ctx;
+    SceneDev* scene = group[0]->scene;
+    const oxr_request& r0 = group[0]->r;
+    int64_t path = 0, pix = 0;
+    for (size_t i = 0; i < group.size(); i++) {
+        Request* q = group[i];
+        ReqDesc& d = s->reqs_host[i];
+        std::memset(&d, 0, sizeof(d));
+        d.path_begin = path;
+        d.pix_begin = pix;
+        d.width = q->r.width;
+        d.height = q->r.height;
+        d.frame0 = q->frames_done;
+        d.frames = frames[i];
+        d.filter = q->r.filter;
+        CamBasis cb = camera_basis(q->r.camera, q->r.width, q->r.height);
+        std::memcpy(d.cam_pos, cb.pos, 12);
+        std::memcpy(d.cam_dir, cb.dir, 12);
+        std::memcpy(d.cam_h, cb.h, 12);
+        std::memcpy(d.cam_v, cb.v, 12);
+        d.color = q->color;
+        d.albedo = q->albedo;
+        d.normal = q->normal;
+        const int64_t np = (int64_t)q->r.width * q->r.height;
+        path += np * frames[i];
+        pix += np;
+    }
+    hipError_t e = hipMemcpyAsync(s->reqs_dev, s->reqs_host, sizeof(ReqDesc) * group.size(), hipMemcpyHostToDevice,
+                                  s->stream);
+    if (e != hipSuccess) return e;
+    WaveParams p;
+    fill_scene_params(p, scene, c->lut_bsdf, c->lut_albedo);
+    p.cam_medium = r0.camera.medium;
+    p.max_depth = r0.max_depth > 0 ? r0.max_depth : 25;
+    p.rr_start = r0.rr_start;
+    p.ray_o = s->ray_o; p.ray_d = s->ray_d; p.thr = s->thr; p.rad = s->rad;
+    p.aov_alb = s->aov_alb; p.aov_nrm = s->aov_nrm;
+    p.queue[0] = s->q0; p.queue[1] = s->q1;
+    p.hits = s->hits; p.sh_env = s->sh_env; p.sh_light = s->sh_light;
+    p.ctr = s->ctr; p.traversal = s->trav; p.gstack = s->gstack; p.overflow = s->overflow;
+    p.reqs = s->reqs_dev;
+    p.nreq = (int)group.size();
+    if ((e = wave_generate(p, path, s->stream)) != hipSuccess) return e;
+    for (int b = 0; b < p.max_depth; b++) {
+        if ((e = wave_extend(p, b, s->trace_grid, false, s->stream)) != hipSuccess) return e;
+        if ((e = wave_shade(p, b, s->shade_grid, s->stream)) != hipSuccess) return e;
+        if ((e = wave_connect(p, 0, s->trace_grid, false, s->stream)) != hipSuccess) return e;
+        if ((e = wave_connect(p, 1, s->trace_grid, false, s->stream)) != hipSuccess) return e;
+    }
+    if ((e = wave_accumulate(p, pix, s->stream)) != hipSuccess) return e;
+    for (size_t i = 0; i < group.size(); i++) {
+        Request* q = group[i];
+        if (q->frames_done + frames[i] >= q->r.spp)
+            if ((e = tonemap(q->color, q->rgba, (int64_t)q->r.width * q->r.height, s->stream)) != hipSuccess) return e;
+    }
+    s->paths += (uint64_t)path;
+    return hipStreamSynchronize(s->stream);
+}
+
+void worker_loop(oxr_server* s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        admit(s);
+        // drop cancelled active requests
+        for (auto it = s->active.begin(); it != s->active.end();) {
+            if ((*it)->state == kCancelled) {
+                free_request_buffers(s, *it);
+                it = s->active.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (s->active.empty()) {
+            s->cv.wait_for(lk, std::chrono::milliseconds(50));
+            continue;
+        }
+        // the group of the oldest active request, frames handed out round-robin
+        std::vector<Request*> group;
+        for (Request* q : s->active)
+            if (same_group(q, s->active[0]) && (int)group.size() < s->max_reqs) group.push_back(q);
+        // Frames round-robin within the tick's path budget; only the oldest
+        // request may overshoot it (by at most one of its frames, which the
+        // wavefront capacity covers).  Requests left without frames sit out.
+        std::vector<int> frames(group.size(), 0);
+        int64_t budget = s->tick_paths, total = 0;
+        bool progress = true;
+        while (progress) {
+            progress = false;
+            for (size_t i = 0; i < group.size(); i++) {
+                const int64_t np = (int64_t)group[i]->r.width * group[i]->r.height;
+                int left = group[i]->r.spp - group[i]->frames_done - frames[i];
+                if (left <= 0) continue;
+                bool first_of_oldest = (i == 0 && frames[0] == 0);
+                if (np > budget && !first_of_oldest) continue;
+                frames[i]++;
+                budget -= np;
+                total += np;
+                progress = true;
+            }
+        }
+        {
+            std::vector<Request*> g2;
+            std::vector<int> f2;
+            for (size_t i = 0; i < group.size(); i++)
+                if (frames[i] > 0) { g2.push_back(group[i]); f2.push_back(frames[i]); }
+            group.swap(g2);
+            frames.swap(f2);
+        }
+        if (total > s->cap || group.empty()) {  // cannot happen: submit bounds a frame by cap - tick_paths
+            for (Request* q : group) { q->state = kFailed; q->error = "tick exceeds wavefront capacity"; }
+            for (Request* q : group) s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+            s->done_cv.notify_all();
+            continue;
+        }
+        lk.unlock();
+        auto t0 = Clock::now();
+        hipError_t e = run_tick(s, group, frames);
+        auto t1 = Clock::now();
+        lk.lock();
+        s->ticks++;
+        s->busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        for (size_t i = 0; i < group.size(); i++) {
+            Request* q = group[i];
+            if (e != hipSuccess) {
+                q->state = kFailed;
+                q->error = std::string("render tick: ") + hipGetErrorString(e);
+            } else {
+                q->frames_done += frames[i];
+                if (q->frames_done < q->r.spp || q->state == kCancelled) continue;
+                q->state = kDone;
+                q->finished = t1;
+                s->completed++;
+                s->latencies.push_back(std::chrono::duration<double, std::milli>(q->finished - q->submitted).count());
+            }
+            s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+        }
+        s->done_cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out) {
+    if (!c || !cfg || !out) return OXR_ERR_INVALID;
+    *out = nullptr;
+    if (cfg->max_active < 1 || cfg->tick_paths < 1 || cfg->max_active > 256) {
+        c->err = "server: max_active in [1, 256] and tick_paths >= 1 required";
+        return OXR_ERR_INVALID;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return OXR_ERR_HIP;
+    oxr_server* s = new oxr_server();
+    s->ctx = c;
+    s->budget = cfg->hbm_budget_bytes;
+    s->max_active = cfg->max_active;
+    s->max_reqs = cfg->max_active;
+    s->tick_paths = cfg->tick_paths;
+    s->cap = cfg->tick_paths + (int64_t)cfg->max_frame_pixels;  // one frame of the largest request may overshoot
+    s->trace_grid = c->trace_grid;
+    s->shade_grid = c->shade_grid;
+    const int64_t n = s->cap;
+    s->wave_bytes = (size_t)n * (16 * 7 + 16 * 6 + 8) + (size_t)kStackGlobal * s->trace_grid * kTraceBlock * 4;
+    auto bail = [&](int code, const char* m) {
+        c->err = m;
+        oxr_server_destroy(s);
+        return code;
+    };
+    if (s->wave_bytes >= s->budget) return bail(OXR_ERR_INVALID, "server: budget smaller than the wavefront buffers");
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP, "stream");
+    if (salloc(&s->ray_o, n) || salloc(&s->ray_d, n) || salloc(&s->thr, n) || salloc(&s->rad, n) ||
+        salloc(&s->aov_alb, n) || salloc(&s->aov_nrm, n) || salloc(&s->hits, n) || salloc(&s->sh_env, 3 * n) ||
+        salloc(&s->sh_light, 3 * n) || salloc(&s->q0, n) || salloc(&s->q1, n) ||
+        salloc(&s->gstack, (size_t)kStackGlobal * s->trace_grid * kTraceBlock) || salloc(&s->ctr, kCtrCount) ||
+        salloc(&s->overflow, 4) || salloc(&s->trav, 8) || salloc(&s->reqs_dev, s->max_reqs))
+        return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
+    if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
+        return bail(OXR_ERR_OOM, "server: pinned staging");
+    if (hipMemset(s->ctr, 0, kCtrCount * sizeof(int)) != hipSuccess ||
+        hipMemset(s->overflow, 0, 4 * sizeof(int)) != hipSuccess ||
+        hipMemset(s->trav, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+        return bail(OXR_ERR_HIP, "server: memset");
+    s->started = Clock::now();
+    s->worker = std::thread(worker_loop, s);
+    *out = s;
+    return OXR_OK;
+}
+
+int oxr_server_destroy(oxr_server* s) {
+    if (!s) return OXR_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& kv : s->all) {
+        free_request_buffers(s, kv.second);
+        delete kv.second;
+    }
+    sfree(s->ray_o); sfree(s->ray_d); sfree(s->thr); sfree(s->rad); sfree(s->aov_alb); sfree(s->aov_nrm);
+    sfree(s->hits); sfree(s->sh_env); sfree(s->sh_light); sfree(s->q0); sfree(s->q1); sfree(s->gstack);
+    sfree(s->ctr); sfree(s->overflow); sfree(s->trav); sfree(s->reqs_dev);
+    if (s->reqs_host) (void)hipHostFree(s->reqs_host);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return OXR_OK;
+}
+
+const char* oxr_server_last_error(const oxr_server* s) { return s ? s->err.c_str() : "null server"; }
+
+int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id) {
+    if (!s || !r || !id) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->ctx->scenes.find(r->scene_id);
+    if (it == s->ctx->scenes.end() || !it->second->built) {
+        s->err = "submit: unknown or unbuilt scene";
+        s->rejected++;
+        return OXR_ERR_INVALID;
+    }
+    if (r->width <= 0 || r->height <= 0 || r->spp < 1 || r->max_depth < 0 || r->filter < 0 || r->filter > 3 ||
+        (int64_t)r->width * r->height > s->cap - s->tick_paths || r->camera.medium < -1 || r->camera.medium >= it->second->n_media) {
+        s->err = "submit: invalid request (size, spp, depth, filter or camera medium)";
+        s->rejected++;
+        return OXR_ERR_INVALID;
+    }
+    if (request_bytes(*r) + s->wave_bytes > s->budget) {
+        s->err = "submit: request can never fit in the HBM budget";
+        s->rejected++;
+        return OXR_ERR_OOM;
+    }
+    Request* q = new Request();
+    q->id = s->next_id++;
+    q->r = *r;
+    q->scene = it->second;
+    q->submitted = Clock::now();
+    s->pending.push_back(q);
+    s->all[q->id] = q;
+    s->submitted++;
+    *id = q->id;
+    s->cv.notify_all();
+    return OXR_OK;
+}
+
+int oxr_poll(oxr_server* s, uint64_t id, int* state) {
+    if (!s || !state) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return OXR_ERR_INVALID;
+    *state = it->second->state;
+    return OXR_OK;
+}
+
+int oxr_wait(oxr_server* s, uint64_t id, int timeout_ms) {
+    if (!s) return OXR_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return OXR_ERR_INVALID;
+    Request* q = it->second;
+    auto pred = [&] { return q->state == kDone || q->state == kCancelled || q->state == kFailed; };
+    if (timeout_ms < 0) s->done_cv.wait(lk, pred);
+    else if (!s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) return OXR_ERR_STATE;
+    if (q->state == kFailed) {
+        s->err = q->error;
+        return OXR_ERR_HIP;
+    }
+    return q->state == kDone ? OXR_OK : OXR_ERR_STATE;
+}
+
+int oxr_cancel(oxr_server* s, uint64_t id) {
+    if (!s) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return OXR_ERR_INVALID;
+    Request* q = it->second;
+    if (q->state == kPending || q->state == kActive) {
+        q->state = kCancelled;
+        s->cancelled++;
+        s->done_cv.notify_all();
+    }
+    return OXR_OK;
+}
+
+static int copy_result(oxr_server* s, uint64_t id, void* out, size_t bytes_per_px, size_t count, int which) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end() || !out) return OXR_ERR_INVALID;
+    Request* q = it->second;
+    if (q->state != kDone) return OXR_ERR_STATE;
+    if (count != (size_t)q->r.width * q->r.height) return OXR_ERR_INVALID;
+    const void* src = which == 0 ? (const void*)q->rgba : (const void*)q->color;
+    if (hipMemcpy(out, src, bytes_per_px * count, hipMemcpyDeviceToHost) != hipSuccess) return OXR_ERR_HIP;
+    return OXR_OK;
+}
+
+int oxr_result_rgba8(oxr_server* s, uint64_t id, uint32_t* out, size_t count) {
+    return s ? copy_result(s, id, out, 4, count, 0) : OXR_ERR_INVALID;
+}
+int oxr_result_color(oxr_server* s, uint64_t id, float* out, size_t count) {
+    return s ? copy_result(s, id, out, 16, count, 1) : OXR_ERR_INVALID;
+}
+
+int oxr_release_request(oxr_server* s, uint64_t id) {
+    if (!s) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return OXR_ERR_INVALID;
+    Request* q = it->second;
+    if (q->state == kPending || q->state == kActive) return OXR_ERR_STATE;
+    free_request_buffers(s, q);
+    delete q;
+    s->all.erase(it);
+    s->cv.notify_all();
+    return OXR_OK;
+}
+
+int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out) {
+    if (!s || !out) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->cancelled = s->cancelled;
+    out->rejected = s->rejected;
+    out->ticks = s->ticks;
+    out->paths = s->paths;
+    out->busy_ms = s->busy_ms;
+    out->wall_ms = std::chrono::duration<double, std::milli>(Clock::now() - s->started).count();
+    out->active = (int32_t)s->active.size();
+    out->pending = (int32_t)s->pending.size();
+    out->bytes_in_use = s->in_use + s->wave_bytes;
+    out->budget_bytes = s->budget;
+    if (!s->latencies.empty()) {
+        std::vector<double> l = s->latencies;
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
+        out->p50_ms = pct(0.50);
+        out->p99_ms = pct(0.99);
+        double sum = 0.0;
+        for (double v : l) sum += v;
+        out->mean_ms = sum / l.size();
+    }
+    return OXR_OK;
+}
+
+}  // extern "C"

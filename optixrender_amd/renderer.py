@@ -243,3 +243,72 @@ class Renderer:
             h_pixels[:] = px
             return h_pixels
         return px
+
+
+class Server:
+    """Continuous-batching render server over a context's resident scenes (oxr_server_*)."""
+
+    def __init__(self, ctx: Context, budget_bytes: int, max_active: int = 16, tick_paths: int = 1 << 24,
+                 max_frame_pixels: int = 1920 * 1080):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        cfg = abi.ServerConfig(budget_bytes, max_active, 0, tick_paths, max_frame_pixels)
+        h = C.c_void_p()
+        ctx.check(self._lib.oxr_server_create(ctx.h, C.byref(cfg), C.byref(h)), "oxr_server_create")
+        self.h = h
+        self.sizes = {}
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.oxr_server_last_error(self.h)
+            raise abi.OxrError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self._lib.oxr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, scene_id, width, height, spp, camera, max_depth=16, rr_start=0, filter=abi.FILTER_GAUSSIAN):
+        r = abi.Request(scene_id, width, height, spp, max_depth, rr_start, filter, 0, camera)
+        rid = C.c_uint64()
+        self.check(self._lib.oxr_submit(self.h, C.byref(r), C.byref(rid)), "oxr_submit")
+        self.sizes[rid.value] = (width, height)
+        return rid.value
+
+    def poll(self, rid):
+        st = C.c_int()
+        self.check(self._lib.oxr_poll(self.h, rid, C.byref(st)), "oxr_poll")
+        return st.value
+
+    def wait(self, rid, timeout_ms=-1):
+        return self._lib.oxr_wait(self.h, rid, timeout_ms)
+
+    def cancel(self, rid):
+        self.check(self._lib.oxr_cancel(self.h, rid), "oxr_cancel")
+
+    def result_color(self, rid):
+        w, h = self.sizes[rid]
+        out = np.zeros((h, w, 4), np.float32)
+        self.check(self._lib.oxr_result_color(self.h, rid, abi.ptr(out), w * h), "oxr_result_color")
+        return out
+
+    def result_rgba8(self, rid):
+        w, h = self.sizes[rid]
+        out = np.zeros((h, w), np.uint32)
+        self.check(self._lib.oxr_result_rgba8(self.h, rid, abi.ptr(out), w * h), "oxr_result_rgba8")
+        return out
+
+    def release(self, rid):
+        self.check(self._lib.oxr_release_request(self.h, rid), "oxr_release_request")
+        self.sizes.pop(rid, None)
+
+    def stats(self) -> abi.ServerStats:
+        s = abi.ServerStats()
+        self.check(self._lib.oxr_server_get_stats(self.h, C.byref(s)), "oxr_server_get_stats")
+        return s
