@@ -109,6 +109,23 @@ def cornell_spheres(subdiv: int = 3) -> SceneData:
     return SceneData(pos, nrm, mid, mats, lights, [], None, cam)
 
 
+def foggy_cornell(phase_type: int = 1, g: float = 0.3) -> SceneData:
+    """The Cornell scene filled with a homogeneous medium (camera, walls and
+    light all inside medium 0): exercises free-flight sampling, medium NEE
+    and the phase functions (core/cuda/Medium.h, devicePrograms.cu:276-338)."""
+    sc = cornell_spheres()
+    med = abi.medium(phase_type=phase_type, g=(g, g, g), sigma_s=(0.35, 0.30, 0.25),
+                     sigma_a=(0.03, 0.05, 0.08), scale=1.0)
+    for m in sc.materials:
+        m.in_medium = 0
+        m.out_medium = 0
+    for lt in sc.lights:
+        lt.medium = 0
+    sc.media = [med]
+    sc.camera = abi.camera((0.0, 1.0, 3.0), (0.0, 1.0, 0.0), medium=0)
+    return sc
+
+
 def sky_env(w: int = 2048, h: int = 1024, sun_peak: float = 5e4, sun_sigma_deg: float = 0.5,
             sun_dir=(0.35, 0.65, -0.45)) -> np.ndarray:
     """Procedural equirect sky: gradient plus a Gaussian sun, (h, w, 4) float32.

@@ -240,3 +240,11 @@ def test_render_sphere_field_per_path(ctx):
     print("[render] sphere_field 96x54x2", flush=True)
     paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 96, 54, 2, 16, 0)
     assert per_path_check(paths, rp, "sphere_field") >= PASS_FRACTION
+
+
+@pytest.mark.parametrize("phase,g", [(0, 0.0), (1, 0.3), (1, -0.6)])
+def test_render_fog_per_path(ctx, phase, g):
+    sc = scenes.foggy_cornell(phase_type=phase, g=g)
+    print(f"[render] fog phase={phase} g={g} 48x48x3", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 48, 48, 3, 8, 2)
+    assert per_path_check(paths, rp, f"fog{phase}") >= PASS_FRACTION
