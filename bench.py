@@ -95,7 +95,8 @@ def main():
     print(f"[bench] scene {scene.n_tris} tris generated in {gen_s:.1f}s, BVH build {build_ms:.2f} ms", file=sys.stderr,
           flush=True)
 
-    def step(flags=0):
+    def step(flags=abi.OXR_FLAG_STAGE_TIMING):
+        # stage timing: kernels launched one by one with HIP events between stages
         return ctx.render(spp=spp, max_depth=depth, rr_start=0, progressive=False, flags=flags)
 
     counted = None
@@ -118,6 +119,14 @@ def main():
               f"{st.shade_ms:.1f}, connect {st.connect_ms:.1f})", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
+    graph_ms = None
+    if not args.profile_steps:   # the same steps replayed as captured hipGraphs, for comparison
+        step(0)
+        tg = time.perf_counter()
+        for i in range(args.steps):
+            step(0)
+        graph_ms = (time.perf_counter() - tg) * 1e3 / args.steps
+        print(f"[bench] graph replay: {graph_ms:.1f} ms/step", file=sys.stderr, flush=True)
     paths = W * H * spp
     value = paths / (wall / args.steps) / 1e6
     out = {
@@ -138,8 +147,12 @@ def main():
                    "bvh_build_ms": round(build_ms, 3), "device_ms_per_step": round(float(np.mean(ctx_ms)), 3),
                    "extend_ms_per_step": round(float(np.mean(ext_ms)), 3),
                    "shade_ms_per_step": round(float(np.mean(shade_ms)), 3),
-                   "connect_ms_per_step": round(float(np.mean(conn_ms)), 3)},
+                   "connect_ms_per_step": round(float(np.mean(conn_ms)), 3),
+                   "launch": "eager launches with HIP events between stages"},
     }
+    if graph_ms is not None:
+        out["config"]["graph_replay_ms_per_step"] = round(graph_ms, 3)
+        out["config"]["graph_replay_Mpaths_s"] = round(paths / (graph_ms / 1e3) / 1e6, 3)
     if counted is not None:
         ext_bytes = NODE_BYTES * counted.ext_node_fetches + TRI_BYTES * counted.ext_tri_tests + \
             RAY_BYTES * counted.extension_rays

@@ -105,7 +105,7 @@ typedef struct oxr_scene_desc {
 #define OXR_FLAG_KEEP_PATH_RADIANCE 1u  /* keep per-(frame, pixel) radiance for oxr_download_path_radiance */
 #define OXR_FLAG_COUNT_TRAVERSAL 2u     /* count node fetches / triangle tests (slower; measurement only) */
 #define OXR_FLAG_EAGER 4u               /* launch kernels one by one instead of replaying a captured hipGraph */
-#define OXR_FLAG_STAGE_TIMING 8u        /* per-stage device times (events between stages; one sync per batch) */
+#define OXR_FLAG_STAGE_TIMING 8u        /* per-stage device times (HIP events between stages; implies eager) */
 
 /* One render call = spp progressive frames of one sample each (reference
    Render() with numPixelSamples = 1, Renderer.cpp:635-793). */

@@ -407,12 +407,15 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
                                          c->stream);
     }
     OXR_HIP(c, hipEventRecord(e1, c->stream));
-    if (e == hipSuccess && s->env_w > 0)
-        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_cache, (float*)scratch, c->stream);
     hipError_t es = hipStreamSynchronize(c->stream);
     uint32_t bad = 0;
     if (e == hipSuccess && es == hipSuccess && n > 0)
         es = hipMemcpy(&bad, bvh_error_word(scratch, n), sizeof(bad), hipMemcpyDeviceToHost);
+    // the environment cache reuses the scratch only after the BVH error word was read
+    if (e == hipSuccess && es == hipSuccess && s->env_w > 0) {
+        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_cache, (float*)scratch, c->stream);
+        if (e == hipSuccess) es = hipStreamSynchronize(c->stream);
+    }
     (void)hipFree(scratch);
     if (bad) return fail(c, OXR_ERR_STATE, "bvh build produced a malformed hierarchy");
     if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("bvh build: ") + hipGetErrorString(e));
@@ -600,8 +603,10 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.nreq = 1;
 
     OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
-    const bool eager = (prm->flags & OXR_FLAG_EAGER) != 0;
     const bool stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
+    // Per-stage events need the eager launch path (events are not captured
+    // into the batch graph).
+    const bool eager = (prm->flags & OXR_FLAG_EAGER) != 0 || stage_timing;
     oxr_render_stats& st = c->stats;
     std::memset(&st, 0, sizeof(st));
     double* stage_ms[] = {&st.generate_ms, &st.extend_ms, &st.shade_ms, &st.connect_ms, &st.accumulate_ms,

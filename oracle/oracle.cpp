@@ -1408,7 +1408,7 @@ Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
             hit[k] = box_test(r, &b.lo[3 * idx], &b.hi[3 * idx], tmin, h.t, tn[k]);
         }
         bool done = false;
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < 2 && !done; k++) {   // any hit stops at the first occluder
             if (hit[k] && (c[k] & kLeaf)) {
                 hit[k] = false;
                 if (try_leaf(c[k] & ~kLeaf) && ANY) done = true;

@@ -65,18 +65,15 @@ def test_budget_admission_serialises(ctx, cornell):
     srv = Server(ctx, budget_bytes=wave + one + 100, max_active=8, tick_paths=tick, max_frame_pixels=fp)
     try:
         ids = [srv.submit(sid, 64, 48, 2, abi.camera((0, 1, 3), (0, 1, 0)), max_depth=8) for _ in range(3)]
-        seen_two_active = False
-        import time
-        for _ in range(5000):
-            st = srv.stats()
-            assert st.bytes_in_use <= st.budget_bytes
-            seen_two_active |= st.active > 1
-            if st.completed == 3:
-                break
-            time.sleep(0.002)
+        # a finished request keeps its buffers until released, so each one is
+        # released once waited for; the next one is admitted only then
         for rid in ids:
+            st = srv.stats()
+            assert st.bytes_in_use <= st.budget_bytes and st.active <= 1
             assert srv.wait(rid, 60000) == 0
-        assert not seen_two_active
+            assert srv.stats().active == 0
+            srv.release(rid)
+        assert srv.stats().completed == 3
         with pytest.raises(abi.OxrError):
             srv.submit(sid, 200, 200, 1, abi.camera((0, 1, 3), (0, 1, 0)))   # larger than max_frame_pixels
         assert srv.stats().rejected == 1
