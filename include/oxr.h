@@ -225,6 +225,21 @@ int oxr_result_color(oxr_server* s, uint64_t id, float* out_rgba, size_t count);
 int oxr_release_request(oxr_server* s, uint64_t id);
 int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out);
 
+/* ---- launch tuning (no reference counterpart: OptiX chose these internally) ---- */
+typedef struct oxr_tuning {
+    int32_t trace_blocks_per_cu;  /* persistent grid of the extend/connect kernels (1..16) */
+    int32_t shade_blocks_per_cu;  /* persistent grid of the shade kernel (1..16) */
+    int32_t stack_lds;            /* traversal stack entries kept in LDS: 8, 16 or 32 */
+    int32_t reserved;
+    int64_t batch_paths;          /* target paths per wavefront batch (queue capacity) */
+} oxr_tuning;
+int oxr_get_tuning(oxr_ctx* ctx, oxr_tuning* out);
+int oxr_set_tuning(oxr_ctx* ctx, const oxr_tuning* t);
+/* Coordinate-descent search over oxr_tuning: renders `probe` on the selected
+   scene for each candidate value of one knob at a time, keeps the fastest,
+   applies it and returns it with its render time.  Resets frameID. */
+int oxr_autotune(oxr_ctx* ctx, const oxr_render_params* probe, oxr_tuning* best, double* best_ms);
+
 /* ---- building blocks exposed for parity tests ---- */
 /* ComputeFinalPixelColorsKernel (PostProcessing.cu:26-48) over n host float4. */
 int oxr_tonemap_host(oxr_ctx* ctx, const float* rgba, int64_t n, uint32_t* out);

@@ -249,6 +249,19 @@ class ServerStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("trace_blocks_per_cu", C.c_int32),
+        ("shade_blocks_per_cu", C.c_int32),
+        ("stack_lds", C.c_int32),
+        ("reserved", C.c_int32),
+        ("batch_paths", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 
 _P = C.c_void_p
@@ -289,6 +302,9 @@ SIGNATURES = {
     "oxr_result_color": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
     "oxr_release_request": (C.c_int, [_P, C.c_uint64]),
     "oxr_server_get_stats": (C.c_int, [_P, _P]),
+    "oxr_get_tuning": (C.c_int, [_P, _P]),
+    "oxr_set_tuning": (C.c_int, [_P, _P]),
+    "oxr_autotune": (C.c_int, [_P, _P, _P, C.POINTER(C.c_double)]),
     "oxr_tonemap_host": (C.c_int, [_P, _P, _I64, _P]),
     "oxr_kulla_conty": (C.c_int, [_P, _P]),
     "oxr_material_batch": (C.c_int, [_P, _P, C.c_int, _P, _I64, _P]),

@@ -241,12 +241,17 @@ int oxr_create(int device, oxr_ctx** out) {
     if (hipMemcpy(c->lut_bsdf, t.bsdf, sizeof(t.bsdf), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->lut_albedo, t.albedo, sizeof(t.albedo), hipMemcpyHostToDevice) != hipSuccess)
         return bail(OXR_ERR_HIP);
-    c->trace_grid = c->num_cu * 4;
-    c->shade_grid = c->num_cu * 8;
+    c->tuning.trace_blocks_per_cu = 8;
+    c->tuning.shade_blocks_per_cu = 8;
+    c->tuning.stack_lds = 32;
+    c->tuning.reserved = 0;
+    c->tuning.batch_paths = 1 << 24;
+    c->trace_grid = c->num_cu * c->tuning.trace_blocks_per_cu;
+    c->shade_grid = c->num_cu * c->tuning.shade_blocks_per_cu;
     if (dalloc(&c->ctr, kCtrCount) != hipSuccess || dalloc(&c->trav, 8) != hipSuccess ||
         dalloc(&c->reqs, 256) != hipSuccess ||
         dalloc(&c->overflow, 4) != hipSuccess ||
-        dalloc(&c->gstack, (size_t)kStackGlobal * c->trace_grid * kTraceBlock) != hipSuccess)
+        dalloc(&c->gstack, gstack_words(c->num_cu * 16)) != hipSuccess)
         return bail(OXR_ERR_OOM);
     if (hipMemset(c->ctr, 0, kCtrCount * sizeof(int)) != hipSuccess ||
         hipMemset(c->overflow, 0, 4 * sizeof(int)) != hipSuccess)
@@ -555,7 +560,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     if (!prm->progressive) c->frame_id = 0;  // Renderer.cpp:642-644
     const int W = c->width, H = c->height;
     const int64_t npix = (int64_t)W * H;
-    int F = prm->batch_frames > 0 ? prm->batch_frames : (int)std::max<int64_t>(1, (int64_t)(1 << 24) / npix);
+    int F = prm->batch_frames > 0 ? prm->batch_frames : (int)std::max<int64_t>(1, c->tuning.batch_paths / npix);
     F = std::min(F, prm->spp);
     int rc = ensure_capacity(c, (int64_t)F * npix);
     if (rc != OXR_OK) return rc;
@@ -601,6 +606,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.ctr = c->ctr; p.traversal = c->trav; p.gstack = c->gstack; p.overflow = c->overflow;
     p.reqs = c->reqs;
     p.nreq = 1;
+    p.stack_lds = c->tuning.stack_lds;
 
     OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
     const bool stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
@@ -727,6 +733,72 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.frame_id = c->frame_id;
     c->path_radiance_valid = keep ? (size_t)3 * prm->spp * npix : 0;
     if (ov) return fail(c, OXR_ERR_STATE, "traversal stack overflow");
+    return OXR_OK;
+}
+
+int oxr_get_tuning(oxr_ctx* c, oxr_tuning* out) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    *out = c->tuning;
+    return OXR_OK;
+}
+
+int oxr_set_tuning(oxr_ctx* c, const oxr_tuning* t) {
+    if (!c || !t) return OXR_ERR_INVALID;
+    if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > 16 || t->shade_blocks_per_cu < 1 ||
+        t->shade_blocks_per_cu > 16 || !(t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32) ||
+        t->batch_paths < 1)
+        return fail(c, OXR_ERR_INVALID, "tuning out of range");
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->tuning = *t;
+    c->trace_grid = c->num_cu * t->trace_blocks_per_cu;
+    c->shade_grid = c->num_cu * t->shade_blocks_per_cu;
+    return OXR_OK;
+}
+
+int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, double* best_ms) {
+    if (!c || !probe) return OXR_ERR_INVALID;
+    oxr_tuning cur = c->tuning;
+    oxr_render_params pr = *probe;
+    pr.progressive = 0;
+    pr.flags &= ~(uint32_t)(OXR_FLAG_COUNT_TRAVERSAL | OXR_FLAG_KEEP_PATH_RADIANCE | OXR_FLAG_STAGE_TIMING);
+    auto time_it = [&](const oxr_tuning& t, double& ms) -> int {
+        int rc = oxr_set_tuning(c, &t);
+        if (rc != OXR_OK) return rc;
+        double tot = 0.0;
+        for (int rep = 0; rep < 3; rep++) {  // first run warms up (graph capture, allocation)
+            rc = oxr_render(c, &pr);
+            if (rc != OXR_OK) return rc;
+            if (rep > 0) tot += c->stats.render_ms;
+        }
+        ms = tot / 2.0;
+        return OXR_OK;
+    };
+    double cur_ms = 0.0;
+    int rc = time_it(cur, cur_ms);
+    if (rc != OXR_OK) return rc;
+    const std::vector<int> stacks = {8, 16, 32}, tracebpc = {4, 8, 12, 16}, shadebpc = {2, 4, 8, 16};
+    const std::vector<int64_t> batches = {1 << 22, 1 << 23, 1 << 24, 1 << 25};
+    for (int knob = 0; knob < 4; knob++) {
+        size_t nc = knob == 0 ? stacks.size() : knob == 1 ? tracebpc.size() : knob == 2 ? shadebpc.size() : batches.size();
+        for (size_t i = 0; i < nc; i++) {
+            oxr_tuning t = cur;
+            if (knob == 0) t.stack_lds = stacks[i];
+            if (knob == 1) t.trace_blocks_per_cu = tracebpc[i];
+            if (knob == 2) t.shade_blocks_per_cu = shadebpc[i];
+            if (knob == 3) t.batch_paths = batches[i];
+            if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
+            double ms = 0.0;
+            rc = time_it(t, ms);
+            if (rc != OXR_OK) return rc;
+            if (ms < cur_ms) { cur_ms = ms; cur = t; }
+        }
+    }
+    rc = oxr_set_tuning(c, &cur);
+    if (rc != OXR_OK) return rc;
+    c->frame_id = 0;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
     return OXR_OK;
 }
 

@@ -9,8 +9,8 @@ namespace oxr {
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kNoHit = 0xffffffffu;
-constexpr int kStackLds = 32;     // traversal stack entries held in LDS per thread
-constexpr int kStackGlobal = 32;  // further entries spilled to HBM (LBVH depth <= 30 + log2 n < 64)
+constexpr int kStackTotal = 64;   // traversal stack entries per lane (LBVH depth <= 30 + log2 n < 64)
+constexpr int kStackLdsMin = 8;   // fewest entries an instance keeps in LDS; the rest spill to HBM
 constexpr int kTraceBlock = 256;
 
 // BVH in traversal layout.  pairs: for internal node i, the two children
@@ -49,10 +49,11 @@ uint32_t* bvh_error_word(void* d_scratch, int64_t n);
 hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void* d_scratch, hipStream_t s);
 
 // ---- launchers (render.hip) ----
-// gstack: kStackGlobal * grid * kTraceBlock words; overflow: one int set to
-// 1 if a traversal ever needed more than kStackLds + kStackGlobal entries.
+// gstack: (kStackTotal - kStackLdsMin) * grid * kTraceBlock words; overflow:
+// one int set non-zero if a traversal ever needed more than kStackTotal entries.
 hipError_t trace_rays(const BvhView& b, const float* d_rays, int64_t n, int any, float4* d_hits, uint8_t* d_occ,
                       uint32_t* d_counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s);
+inline size_t gstack_words(int grid) { return (size_t)(kStackTotal - kStackLdsMin) * grid * kTraceBlock; }
 hipError_t env_cache_build(const float4* d_env, int w, int h, float4* d_cache, float* d_scratch, hipStream_t s);
 size_t env_scratch_floats(int w, int h);
 hipError_t tonemap(const float4* d_in, uint32_t* d_out, int64_t n, hipStream_t s);
@@ -111,7 +112,8 @@ struct WaveParams {
     const ReqDesc* reqs;  // device table of the batch's requests
     int nreq;
     unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
-    uint32_t* gstack;               // traversal stack spill, kStackGlobal * grid * kTraceBlock
+    uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
+    int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
     int* overflow;
 };
 

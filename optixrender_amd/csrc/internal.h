@@ -68,6 +68,7 @@ struct oxr_ctx {
     uint32_t* gstack = nullptr;
     int* overflow = nullptr;
     int trace_grid = 0, shade_grid = 0;
+    oxr_tuning tuning;
     float* path_radiance = nullptr;
     size_t path_radiance_floats = 0;
     size_t path_radiance_valid = 0;

@@ -50,10 +50,10 @@ template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(BvhView b, const float* __restrict__ rays, int64_t n,
                                                              float4* hits, uint8_t* occ, uint32_t* counts,
                                                              uint32_t* gstack, int* overflow) {
-    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    __shared__ uint32_t lds[32 * kTraceBlock];
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-    Stack st{lds + threadIdx.x, gstack + gt, gstride, 0};
+    Stack<32> st{lds + threadIdx.x, gstack + gt, gstride, 0};
     for (int64_t i = gt; i < n; i += gstride) {
         const float* r = rays + 8 * i;
         uint32_t nn = 0, nt = 0;
@@ -224,11 +224,11 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams p, int64_t npaths) 
     p.queue[0][pid] = (uint32_t)pid;
 }
 
-template <bool COUNT>
+template <bool COUNT, int S>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int bounce) {
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
-    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    __shared__ uint32_t lds[S * kTraceBlock];
     const int qi = bounce & 1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.ctr[qi ^ 1] = 0;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int bounce
     const int count = p.ctr[qi];
     const uint32_t* q = p.queue[qi];
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
+    Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
     uint32_t nn = 0, nt = 0, nr = 0;
     while (true) {
         int base = wave_fetch(&p.ctr[kCtrFetchExtend]);
@@ -262,16 +262,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int bounce
     wave_count_add(&p.traversal[2], nr);
 }
 
-template <bool COUNT>
+template <bool COUNT, int S>
 __global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which) {
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
-    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    __shared__ uint32_t lds[S * kTraceBlock];
     const int count = p.ctr[which == 0 ? kCtrShEnv : kCtrShLight];
     const float4* rec = which == 0 ? p.sh_env : p.sh_light;
     int* fetch = &p.ctr[which == 0 ? kCtrFetchConnEnv : kCtrFetchConnLight];
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    Stack st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
+    Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
     uint32_t nn = 0, nt = 0, nr = 0;
     while (true) {
         int base = wave_fetch(fetch);
@@ -672,9 +672,22 @@ hipError_t wave_generate(const WaveParams& p, int64_t np, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <bool C>
+void launch_extend(const WaveParams& p, int bounce, int grid, hipStream_t s) {
+    if (p.stack_lds <= 8) k_extend<C, 8><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+    else if (p.stack_lds <= 16) k_extend<C, 16><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+    else k_extend<C, 32><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+}
+template <bool C>
+void launch_connect(const WaveParams& p, int which, int grid, hipStream_t s) {
+    if (p.stack_lds <= 8) k_connect<C, 8><<<grid, kTraceBlock, 0, s>>>(p, which);
+    else if (p.stack_lds <= 16) k_connect<C, 16><<<grid, kTraceBlock, 0, s>>>(p, which);
+    else k_connect<C, 32><<<grid, kTraceBlock, 0, s>>>(p, which);
+}
+
 hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s) {
-    if (count) k_extend<true><<<grid, kTraceBlock, 0, s>>>(p, bounce);
-    else k_extend<false><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+    if (count) launch_extend<true>(p, bounce, grid, s);
+    else launch_extend<false>(p, bounce, grid, s);
     return hipGetLastError();
 }
 
@@ -684,8 +697,8 @@ hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s) 
 }
 
 hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s) {
-    if (count) k_connect<true><<<grid, kTraceBlock, 0, s>>>(p, which);
-    else k_connect<false><<<grid, kTraceBlock, 0, s>>>(p, which);
+    if (count) launch_connect<true>(p, which, grid, s);
+    else launch_connect<false>(p, which, grid, s);
     return hipGetLastError();
 }
 

@@ -57,7 +57,7 @@ struct oxr_server {
     size_t budget = 0, wave_bytes = 0, in_use = 0;
     int max_active = 0;
     int64_t tick_paths = 0, cap = 0;
-    int trace_grid = 0, shade_grid = 0;
+    int trace_grid = 0, shade_grid = 0, stack_lds = 32;
     // shared wavefront buffers
     float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
     float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
@@ -180,6 +180,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     p.ctr = s->ctr; p.traversal = s->trav; p.gstack = s->gstack; p.overflow = s->overflow;
     p.reqs = s->reqs_dev;
     p.nreq = (int)group.size();
+    p.stack_lds = s->stack_lds;
     if ((e = wave_generate(p, path, s->stream)) != hipSuccess) return e;
     for (int b = 0; b < p.max_depth; b++) {
         if ((e = wave_extend(p, b, s->trace_grid, false, s->stream)) != hipSuccess) return e;
@@ -301,7 +302,8 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->trace_grid = c->trace_grid;
     s->shade_grid = c->shade_grid;
     const int64_t n = s->cap;
-    s->wave_bytes = (size_t)n * (16 * 7 + 16 * 6 + 8) + (size_t)kStackGlobal * s->trace_grid * kTraceBlock * 4;
+    s->stack_lds = c->tuning.stack_lds;
+    s->wave_bytes = (size_t)n * (16 * 7 + 16 * 6 + 8) + gstack_words(s->trace_grid) * 4;
     auto bail = [&](int code, const char* m) {
         c->err = m;
         oxr_server_destroy(s);
@@ -312,7 +314,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     if (salloc(&s->ray_o, n) || salloc(&s->ray_d, n) || salloc(&s->thr, n) || salloc(&s->rad, n) ||
         salloc(&s->aov_alb, n) || salloc(&s->aov_nrm, n) || salloc(&s->hits, n) || salloc(&s->sh_env, 3 * n) ||
         salloc(&s->sh_light, 3 * n) || salloc(&s->q0, n) || salloc(&s->q1, n) ||
-        salloc(&s->gstack, (size_t)kStackGlobal * s->trace_grid * kTraceBlock) || salloc(&s->ctr, kCtrCount) ||
+        salloc(&s->gstack, gstack_words(s->trace_grid)) || salloc(&s->ctr, kCtrCount) ||
         salloc(&s->overflow, 4) || salloc(&s->trav, 8) || salloc(&s->reqs_dev, s->max_reqs))
         return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
     if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)

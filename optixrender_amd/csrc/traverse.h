@@ -7,8 +7,8 @@
 // (left, right) order with a watertight ray/triangle test (Woop, Benthin,
 // Wald 2013) on a 48-byte triangle record, then the nearer hit internal
 // child is descended and the farther pushed.  The stack lives in LDS
-// (kStackLds entries per lane, [entry][lane] so a wave touches consecutive
-// banks) and spills the deeper entries to a per-lane HBM region.
+// (S entries per lane, [entry][lane] so a wave touches consecutive banks) and
+// spills the deeper entries to a per-lane HBM region.
 // Closest-hit ties in t go to the lower triangle id, so the result does not
 // depend on visit order.
 #pragma once
@@ -80,26 +80,28 @@ struct Hit {
     float u, v;
 };
 
+// S entries per lane in LDS, the remaining kStackTotal - S in HBM.
+template <int S>
 struct Stack {
     uint32_t* lds;   // this lane's first entry; entries kTraceBlock apart
     uint32_t* gbl;   // this lane's first spill entry; entries gstride apart
     int64_t gstride;
     int sp;
     OXR_D bool push(uint32_t x) {
-        if (sp < kStackLds) lds[sp * kTraceBlock] = x;
-        else if (sp < kStackLds + kStackGlobal) gbl[(int64_t)(sp - kStackLds) * gstride] = x;
+        if (sp < S) lds[sp * kTraceBlock] = x;
+        else if (sp < kStackTotal) gbl[(int64_t)(sp - S) * gstride] = x;
         else return false;
         sp++;
         return true;
     }
     OXR_D uint32_t pop() {
         sp--;
-        return sp < kStackLds ? lds[sp * kTraceBlock] : gbl[(int64_t)(sp - kStackLds) * gstride];
+        return sp < S ? lds[sp * kTraceBlock] : gbl[(int64_t)(sp - S) * gstride];
     }
 };
 
-template <bool ANY, bool COUNT>
-OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, Stack& st, uint32_t& nn, uint32_t& nt,
+template <bool ANY, bool COUNT, class StackT>
+OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT& st, uint32_t& nn, uint32_t& nt,
                    int* overflow) {
     Hit h;
     h.t = tmax;

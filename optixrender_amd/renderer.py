@@ -155,6 +155,21 @@ class Context:
         self.check(self._lib.oxr_get_stats(self.h, C.byref(s)), "oxr_get_stats")
         return s
 
+    def get_tuning(self) -> abi.Tuning:
+        t = abi.Tuning()
+        self.check(self._lib.oxr_get_tuning(self.h, C.byref(t)), "oxr_get_tuning")
+        return t
+
+    def set_tuning(self, t: abi.Tuning):
+        self.check(self._lib.oxr_set_tuning(self.h, C.byref(t)), "oxr_set_tuning")
+
+    def autotune(self, spp=4, max_depth=16, rr_start=0):
+        p = abi.RenderParams(spp, max_depth, rr_start, 0, abi.FILTER_GAUSSIAN, 0, 0, 0)
+        t = abi.Tuning()
+        ms = C.c_double()
+        self.check(self._lib.oxr_autotune(self.h, C.byref(p), C.byref(t), C.byref(ms)), "oxr_autotune")
+        return t, ms.value
+
     def download_rgba8(self, w: int, h: int) -> np.ndarray:
         out = np.zeros(w * h, np.uint32)
         self.check(self._lib.oxr_download_rgba8(self.h, abi.ptr(out), out.size), "oxr_download_rgba8")

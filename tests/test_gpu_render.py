@@ -88,3 +88,31 @@ def test_stage_timing_and_counts(ctx, cornell):
     assert st.node_fetches == ref["node_fetches"]
     assert st.tri_tests == ref["tri_tests"]
     assert st.extend_ms > 0 and st.shade_ms > 0
+
+
+@pytest.mark.parametrize("stack,tbpc,sbpc,batch", [(8, 4, 2, 1 << 12), (16, 16, 16, 1 << 14), (32, 8, 8, 1 << 24)])
+def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, batch):
+    sc, sid = cornell
+    base = ctx.get_tuning()
+    try:
+        ref, _, _ = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
+        t = abi.Tuning(tbpc, sbpc, stack, 0, batch)
+        ctx.set_tuning(t)
+        got, _, _ = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
+        assert np.array_equal(ref.view(np.uint32), got.view(np.uint32))
+    finally:
+        ctx.set_tuning(base)
+
+
+def test_autotune_picks_a_valid_tuning(ctx, cornell):
+    sc, sid = cornell
+    base = ctx.get_tuning()
+    try:
+        ctx.select(sid)
+        ctx.resize(W, H)
+        ctx.set_camera(sc.camera)
+        t, ms = ctx.autotune(spp=2, max_depth=8, rr_start=3)
+        assert t.stack_lds in (8, 16, 32) and 1 <= t.trace_blocks_per_cu <= 16 and ms > 0
+        assert ctx.get_tuning().as_dict() == t.as_dict()
+    finally:
+        ctx.set_tuning(base)
