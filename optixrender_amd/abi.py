@@ -319,6 +319,10 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m optixrender_amd.build` (no CPU fallback)")
+        from . import build as _build
+
+        if not _build.STAMP.exists() or _build.STAMP.read_text().strip() != _build.source_hash():
+            raise RuntimeError(f"{LIB_PATH} was built from other sources than the ones in the tree: rebuild")
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -34,6 +34,20 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["api.cpp", "serve.cpp", "bvh.hip", "render.hip"]
+STAMP = PKG / "liboxr.stamp"
+
+
+def source_hash() -> str:
+    """Hash of everything liboxr.so is built from; stored next to the library."""
+    import hashlib
+
+    h = hashlib.sha256()
+    files = sorted(CSRC.glob("*")) + [ROOT / "include" / "oxr.h"]
+    for f in files:
+        if f.is_file():
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()
 
 
 def _run(cmd, cwd=None):
@@ -62,8 +76,10 @@ def build_library(verbose: bool = False) -> Path:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([hipcc(), *flags, *lang, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))
+    digest = source_hash()
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     tmp.replace(LIB)
+    STAMP.write_text(digest + "\n")
     if verbose:
         print(f"built {LIB}")
     return LIB
