@@ -1593,7 +1593,9 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
         }
 
         if (!scattered) {
-            if (!hitLight && is.mat == nullptr && closest_distance == FLT_MAX) break;  // no surface yet: nothing to shade
+            // A ray inside a medium that leaves the scene unscattered would
+            // shade stale surface data (no surface was hit); it ends here.
+            if (!hitLight && is.mat == nullptr && closest_distance == FLT_MAX && medium_index != -1) break;
             if (hitLight) {
                 float mw = 1.0f;
                 if (bounce != 0) {
