@@ -2,9 +2,11 @@
 
 One step = one render call of the headline workload: the 983,040-triangle
 sphere field (12 icospheres at subdivision 6 under a 2048x1024 procedural HDR
-sky) at 1920x1080, 64 spp, max depth 16, through generate -> extend -> shade
--> connect -> accumulate -> tone map.  Scene upload and the BVH build happen
-before the timed region (the build time is reported separately).
+sky) at 1920x1080, 64 spp, max depth 16: 132.7 M paths through the
+regenerating wavefront loop (extend -> shade -> connect -> regenerate on a
+resident path pool, then accumulate and tone map).  Scene upload and the BVH
+build happen before the timed region (the build time is reported
+separately).
 
 Prints one JSON line with Mpaths/s, the traversal roofline of the extend
 kernel (algorithmic bytes from an untimed counting pass of the same frames,
@@ -41,6 +43,8 @@ def parse():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (the known-size reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -73,6 +77,43 @@ def spp_total(args):
     return args.spp
 
 
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """A plain device-to-device copy of COPY_BYTES (reads and writes exactly
+    that many bytes): under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE give
+    the factor that turns those counters into HBM bytes on this GPU."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: 3 copies of {COPY_BYTES} bytes", file=sys.stderr, flush=True)
+    del src, dst
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_k_extend_sphere_field_1080p_64spp.md"
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per k_extend launch from the committed counter summary, if
+    it was measured on this workload (a line `traffic_bytes_per_launch: N`
+    under a line `workload: ...`), else None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    fields = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if ":" in line and not line.startswith(("|", "#", " ")):
+            k, v = line.split(":", 1)
+            fields[k.strip().strip("`")] = v.strip().strip("`")
+    if fields.get("workload") != workload or "traffic_bytes_per_launch" not in fields:
+        return None
+    return int(float(fields["traffic_bytes_per_launch"]))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -81,6 +122,9 @@ def main():
         sys.exit(2)
     from optixrender_amd import abi, scenes
     from optixrender_amd.renderer import Context
+
+    if args.copy_calibration:
+        copy_calibration()
 
     t0 = time.perf_counter()
     scene = scenes.sphere_field(seed=1, subdiv=args.subdiv)
@@ -107,7 +151,7 @@ def main():
     for i in range(args.warmup):
         step()
         print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    ctx_ms, ext_ms, conn_ms, shade_ms = [], [], [], []
+    ctx_ms, ext_ms, conn_ms, shade_ms, regen_ms = [], [], [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = step()
@@ -115,8 +159,11 @@ def main():
         ext_ms.append(st.extend_ms)
         conn_ms.append(st.connect_ms)
         shade_ms.append(st.shade_ms)
+        regen_ms.append(st.generate_ms)
         print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms (extend {st.extend_ms:.1f}, shade "
-              f"{st.shade_ms:.1f}, connect {st.connect_ms:.1f})", file=sys.stderr, flush=True)
+              f"{st.shade_ms:.1f}, connect {st.connect_ms:.1f}, regenerate {st.generate_ms:.1f}, accumulate "
+              f"{st.accumulate_ms:.1f}; {st.iterations} iterations, {st.kernel_launches} launches)",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     graph_ms = None
@@ -148,6 +195,9 @@ def main():
                    "extend_ms_per_step": round(float(np.mean(ext_ms)), 3),
                    "shade_ms_per_step": round(float(np.mean(shade_ms)), 3),
                    "connect_ms_per_step": round(float(np.mean(conn_ms)), 3),
+                   "regenerate_ms_per_step": round(float(np.mean(regen_ms)), 3),
+                   "iterations_per_step": int(st.iterations), "launches_per_step": int(st.kernel_launches),
+                   "tuning": ctx.get_tuning().as_dict(),
                    "launch": "eager launches with HIP events between stages"},
     }
     if graph_ms is not None:
@@ -164,8 +214,9 @@ def main():
         both = (ext_bytes + conn_bytes) / ((float(np.mean(ext_ms)) + float(np.mean(conn_ms))) / 1e3) / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_extend (all bounces of one step)",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(out["config"]["workload"]),
+            "kernel": "k_extend (all launches of one step)",
+            "launches_per_step": int(st.iterations),
             "bytes_per_step": int(ext_bytes),
             "per_ray_bytes": round(ext_bytes / max(counted.extension_rays, 1), 1),
             "extend_plus_connect_GBs": round(both, 1),

@@ -117,18 +117,22 @@ typedef struct oxr_render_params {
     int32_t filter;       /* 0 box 1 tent 2 triangle 3 gaussian (reference default 3) */
     int32_t denoise;      /* 0 plain accumulation */
     uint32_t flags;       /* OXR_FLAG_* */
-    int32_t batch_frames; /* frames per wavefront batch, 0 = automatic */
+    int32_t batch_frames; /* frames per sample window, 0 = automatic (oxr_tuning.window_samples) */
 } oxr_render_params;
 
 typedef struct oxr_render_stats {
     double render_ms;        /* whole oxr_render wall time on the device stream */
-    double extend_ms, shade_ms, connect_ms, generate_ms, accumulate_ms, tonemap_ms;
+    double extend_ms, shade_ms, connect_ms;
+    double generate_ms;      /* window start and path regeneration */
+    double accumulate_ms, tonemap_ms;
     uint64_t extension_rays, shadow_rays;
     uint64_t node_fetches, tri_tests;   /* all traversals; only with OXR_FLAG_COUNT_TRAVERSAL */
     uint64_t ext_node_fetches, ext_tri_tests;  /* extension rays only; same condition */
     uint64_t paths;
     int32_t frame_id;                   /* next frameID */
     int32_t kernel_launches;
+    int32_t iterations;                 /* wavefront iterations (extend .. regenerate) over all windows */
+    int32_t windows;                    /* sample windows rendered */
 } oxr_render_stats;
 
 /* ---- context and memory (CUDABuffer-style API, common/CUDABuffer.h:26-81) ---- */
@@ -227,11 +231,12 @@ int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out);
 
 /* ---- launch tuning (no reference counterpart: OptiX chose these internally) ---- */
 typedef struct oxr_tuning {
-    int32_t trace_blocks_per_cu;  /* persistent grid of the extend/connect kernels (1..16) */
-    int32_t shade_blocks_per_cu;  /* persistent grid of the shade kernel (1..16) */
+    int32_t trace_blocks_per_cu;  /* persistent grid of extend/connect per CU (1..16), capped at what fits at once */
+    int32_t shade_blocks_per_cu;  /* persistent grid of shade/regenerate per CU (1..16), capped likewise */
     int32_t stack_lds;            /* traversal stack entries kept in LDS: 8, 16 or 32 */
-    int32_t reserved;
-    int64_t batch_paths;          /* target paths per wavefront batch (queue capacity) */
+    int32_t iters_per_poll;       /* wavefront iterations per graph replay between completion polls (even, 2..64) */
+    int64_t pool_paths;           /* resident path slots (rounded up to a multiple of 2048) */
+    int64_t window_samples;       /* samples whose radiance and AOVs are kept at once (48 B each) */
 } oxr_tuning;
 int oxr_get_tuning(oxr_ctx* ctx, oxr_tuning* out);
 int oxr_set_tuning(oxr_ctx* ctx, const oxr_tuning* t);

@@ -205,6 +205,8 @@ class RenderStats(C.Structure):
         ("paths", C.c_uint64),
         ("frame_id", C.c_int32),
         ("kernel_launches", C.c_int32),
+        ("iterations", C.c_int32),
+        ("windows", C.c_int32),
     ]
 
     def as_dict(self):
@@ -254,12 +256,13 @@ class Tuning(C.Structure):
         ("trace_blocks_per_cu", C.c_int32),
         ("shade_blocks_per_cu", C.c_int32),
         ("stack_lds", C.c_int32),
-        ("reserved", C.c_int32),
-        ("batch_paths", C.c_int64),
+        ("iters_per_poll", C.c_int32),
+        ("pool_paths", C.c_int64),
+        ("window_samples", C.c_int64),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)

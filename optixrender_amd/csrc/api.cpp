@@ -156,37 +156,14 @@ void free_scene(SceneDev* s) {
     delete s;
 }
 
-void drop_graphs(oxr_ctx* c) {
-    for (auto& g : c->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        for (auto e : g.events) (void)hipEventDestroy(e);
-    }
-    c->graphs.clear();
-}
-
-void free_wave(oxr_ctx* c) {
-    drop_graphs(c);
-    dfree(c->ray_o); dfree(c->ray_d); dfree(c->thr); dfree(c->rad); dfree(c->aov_alb); dfree(c->aov_nrm);
-    dfree(c->hits); dfree(c->sh_env); dfree(c->sh_light); dfree(c->q0); dfree(c->q1);
-    c->cap = 0;
-}
-
-int ensure_capacity(oxr_ctx* c, int64_t paths) {
-    if (paths <= c->cap) return OXR_OK;
-    free_wave(c);
-    OXR_HIP(c, dalloc(&c->ray_o, paths));
-    OXR_HIP(c, dalloc(&c->ray_d, paths));
-    OXR_HIP(c, dalloc(&c->thr, paths));
-    OXR_HIP(c, dalloc(&c->rad, paths));
-    OXR_HIP(c, dalloc(&c->aov_alb, paths));
-    OXR_HIP(c, dalloc(&c->aov_nrm, paths));
-    OXR_HIP(c, dalloc(&c->hits, paths));
-    OXR_HIP(c, dalloc(&c->sh_env, 3 * paths));
-    OXR_HIP(c, dalloc(&c->sh_light, 3 * paths));
-    OXR_HIP(c, dalloc(&c->q0, paths));
-    OXR_HIP(c, dalloc(&c->q1, paths));
-    c->cap = paths;
-    return OXR_OK;
+// Grids of the stage kernels: the tuned blocks per CU, capped at what fits
+// on a CU at once (a persistent grid must be resident in full), rounded so
+// that every shard gets a wave.
+void apply_tuning(oxr_ctx* c) {
+    const int tb = std::min(c->tuning.trace_blocks_per_cu, wave_trace_occupancy(c->tuning.stack_lds));
+    const int sb = std::min(c->tuning.shade_blocks_per_cu, wave_shade_occupancy());
+    c->trace_grid = wave_grid(c->num_cu * tb);
+    c->shade_grid = wave_grid(c->num_cu * sb);
 }
 
 SceneDev* selected(oxr_ctx* c) {
@@ -244,21 +221,12 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.trace_blocks_per_cu = 8;
     c->tuning.shade_blocks_per_cu = 8;
     c->tuning.stack_lds = 32;
-    c->tuning.reserved = 0;
-    c->tuning.batch_paths = 1 << 24;
-    c->trace_grid = c->num_cu * c->tuning.trace_blocks_per_cu;
-    c->shade_grid = c->num_cu * c->tuning.shade_blocks_per_cu;
-    if (dalloc(&c->ctr, kCtrCount) != hipSuccess || dalloc(&c->trav, 8) != hipSuccess ||
-        dalloc(&c->reqs, 256) != hipSuccess ||
-        dalloc(&c->overflow, 4) != hipSuccess ||
-        dalloc(&c->gstack, gstack_words(c->num_cu * 16)) != hipSuccess)
-        return bail(OXR_ERR_OOM);
-    if (hipMemset(c->ctr, 0, kCtrCount * sizeof(int)) != hipSuccess ||
-        hipMemset(c->overflow, 0, 4 * sizeof(int)) != hipSuccess)
-        return bail(OXR_ERR_HIP);
-    c->events.resize(1024);
-    for (auto& e : c->events)
-        if (hipEventCreate(&e) != hipSuccess) return bail(OXR_ERR_HIP);
+    c->tuning.iters_per_poll = 4;
+    c->tuning.pool_paths = 1 << 23;
+    c->tuning.window_samples = 1 << 27;
+    apply_tuning(c);
+    if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return bail(OXR_ERR_HIP);
     *out = c;
     return OXR_OK;
 }
@@ -269,12 +237,11 @@ int oxr_destroy(oxr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->scenes) free_scene(kv.second);
     c->scenes.clear();
-    free_wave(c);
+    wave_release(c->eng);
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
-    drop_graphs(c);
-    dfree(c->ctr); dfree(c->trav); dfree(c->overflow); dfree(c->gstack); dfree(c->path_radiance); dfree(c->reqs);
-    for (auto& e : c->events)
-        if (e) (void)hipEventDestroy(e);
+    dfree(c->path_radiance); dfree(c->reqs);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OXR_OK;
@@ -402,7 +369,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
     size_t sb = bvh_scratch_bytes(n);
     size_t eb = s->env_w > 0 ? env_scratch_floats(s->env_w, s->env_h) * sizeof(float) : 0;
     OXR_HIP(c, hipMalloc(&scratch, std::max<size_t>(std::max(sb, eb), 256)));
-    hipEvent_t e0 = c->events[0], e1 = c->events[1];
+    hipEvent_t e0 = c->t0, e1 = c->t1;
     OXR_HIP(c, hipEventRecord(e0, c->stream));
     hipError_t e = hipSuccess;
     if (n > 0) {
@@ -504,13 +471,15 @@ int oxr_trace_rays(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float*
         return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
     }
     BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
-    e = trace_rays(bv, dr, n, any_hit, dh, dOcc, dc, c->gstack, c->trace_grid, c->overflow, c->stream);
+    if ((e = wave_reserve(c->eng, 0, 0, c->trace_grid)) == hipSuccess &&
+        (e = hipMemsetAsync(c->eng.overflow, 0, sizeof(int), c->stream)) == hipSuccess)
+        e = trace_rays(bv, dr, n, any_hit, dh, dOcc, dc, c->eng.gstack, c->trace_grid, c->eng.overflow, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && !any_hit) e = hipMemcpy(hits, dh, 16 * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess && any_hit) e = hipMemcpy(occ, dOcc, n, hipMemcpyDeviceToHost);
     if (e == hipSuccess && counts) e = hipMemcpy(counts, dc, 8 * n, hipMemcpyDeviceToHost);
     int ov = 0;
-    if (e == hipSuccess) e = hipMemcpy(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&ov, c->eng.overflow, sizeof(int), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
     if (ov) return fail(c, OXR_ERR_STATE, ov == 2 ? "traversal visited more nodes than the tree holds"
@@ -560,16 +529,19 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     if (!prm->progressive) c->frame_id = 0;  // Renderer.cpp:642-644
     const int W = c->width, H = c->height;
     const int64_t npix = (int64_t)W * H;
-    int F = prm->batch_frames > 0 ? prm->batch_frames : (int)std::max<int64_t>(1, c->tuning.batch_paths / npix);
-    F = std::min(F, prm->spp);
-    int rc = ensure_capacity(c, (int64_t)F * npix);
-    if (rc != OXR_OK) return rc;
+    if (npix > (int64_t)1 << 30) return fail(c, OXR_ERR_INVALID, "frame larger than 2^30 pixels");
+    // frames per window: each window keeps its samples' radiance and AOVs
+    int F = prm->batch_frames > 0 ? prm->batch_frames : (int)std::max<int64_t>(1, c->tuning.window_samples / npix);
+    F = (int)std::min<int64_t>({(int64_t)F, (int64_t)prm->spp, ((int64_t)1 << 31) / npix - 1});
+    F = std::max(F, 1);
+    const int64_t window = (int64_t)F * npix;
+    const int64_t pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, window));
+    OXR_HIP(c, wave_reserve(c->eng, pool, window, c->trace_grid));
     const bool keep = (prm->flags & OXR_FLAG_KEEP_PATH_RADIANCE) != 0;
     const bool count = (prm->flags & OXR_FLAG_COUNT_TRAVERSAL) != 0;
     if (keep) {
         size_t need = (size_t)3 * prm->spp * npix;
         if (need > c->path_radiance_floats) {
-            drop_graphs(c);
             dfree(c->path_radiance);
             OXR_HIP(c, dalloc(&c->path_radiance, need));
             c->path_radiance_floats = need;
@@ -596,131 +568,63 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
     fill_scene_params(p, s, c->lut_bsdf, c->lut_albedo);
+    wave_bind(c->eng, pool, p);
     p.cam_medium = c->cam.medium;
     p.max_depth = prm->max_depth > 0 ? prm->max_depth : 25;
     p.rr_start = prm->rr_start;
-    p.ray_o = c->ray_o; p.ray_d = c->ray_d; p.thr = c->thr; p.rad = c->rad;
-    p.aov_alb = c->aov_alb; p.aov_nrm = c->aov_nrm;
-    p.queue[0] = c->q0; p.queue[1] = c->q1;
-    p.hits = c->hits; p.sh_env = c->sh_env; p.sh_light = c->sh_light;
-    p.ctr = c->ctr; p.traversal = c->trav; p.gstack = c->gstack; p.overflow = c->overflow;
     p.reqs = c->reqs;
     p.nreq = 1;
     p.stack_lds = c->tuning.stack_lds;
 
-    OXR_HIP(c, hipMemsetAsync(c->trav, 0, 8 * sizeof(unsigned long long), c->stream));
-    const bool stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
-    // Per-stage events need the eager launch path (events are not captured
-    // into the batch graph).
-    const bool eager = (prm->flags & OXR_FLAG_EAGER) != 0 || stage_timing;
     oxr_render_stats& st = c->stats;
     std::memset(&st, 0, sizeof(st));
-    double* stage_ms[] = {&st.generate_ms, &st.extend_ms, &st.shade_ms, &st.connect_ms, &st.accumulate_ms,
-                          &st.tonemap_ms};
-    // One batch: generate -> depth x (extend, shade, connect env, connect light) -> accumulate.
-    // mark(stage) records an event after the stage's launches.
-    auto enqueue_batch = [&](int nf, const std::function<hipError_t(int)>& mark) -> hipError_t {
-        hipError_t e;
-        if ((e = wave_generate(p, (int64_t)nf * npix, c->stream)) != hipSuccess || (e = mark(0)) != hipSuccess) return e;
-        for (int b = 0; b < p.max_depth; b++) {
-            if ((e = wave_extend(p, b, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(1)) != hipSuccess)
-                return e;
-            if ((e = wave_shade(p, b, c->shade_grid, c->stream)) != hipSuccess || (e = mark(2)) != hipSuccess)
-                return e;
-            if ((e = wave_connect(p, 0, c->trace_grid, count, c->stream)) != hipSuccess ||
-                (e = wave_connect(p, 1, c->trace_grid, count, c->stream)) != hipSuccess || (e = mark(3)) != hipSuccess)
-                return e;
-        }
-        if ((e = wave_accumulate(p, npix, c->stream)) != hipSuccess) return e;
-        return mark(4);
-    };
-    auto graph_for = [&](int nf, oxr_ctx::Graph*& out) -> hipError_t {
-        std::vector<uint8_t> key(sizeof(WaveParams) + 4 * sizeof(int64_t));
-        std::memcpy(key.data(), &p, sizeof(WaveParams));
-        int64_t extra[4] = {nf, npix, count ? 1 : 0, stage_timing ? 1 : 0};
-        std::memcpy(key.data() + sizeof(WaveParams), extra, sizeof(extra));
-        for (auto& g : c->graphs)
-            if (g.key == key) { out = &g; return hipSuccess; }
-        if (c->graphs.size() >= 4) drop_graphs(c);
-        oxr_ctx::Graph g;
-        g.key = key;
-        if (stage_timing) {  // events are created before the capture opens
-            g.events.resize(3 + 3 * (size_t)p.max_depth, nullptr);
-            for (auto& ev : g.events)
-                if (hipEventCreate(&ev) != hipSuccess) {
-                    for (auto x : g.events) if (x) (void)hipEventDestroy(x);
-                    return hipErrorOutOfMemory;
-                }
-        }
-        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return e;
-        size_t used = 0;
-        if (stage_timing) (void)hipEventRecord(g.events[used++], c->stream);
-        auto mark = [&](int stage) -> hipError_t {
-            if (!stage_timing) return hipSuccess;
-            g.stages.push_back(stage);
-            return hipEventRecord(g.events[used++], c->stream);
-        };
-        hipError_t le = enqueue_batch(nf, mark);
-        hipGraph_t graph = nullptr;
-        e = hipStreamEndCapture(c->stream, &graph);
-        if (le != hipSuccess) e = le;
-        if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            for (auto ev : g.events) (void)hipEventDestroy(ev);
-            return e;
-        }
-        c->graphs.push_back(std::move(g));
-        out = &c->graphs.back();
-        return hipSuccess;
-    };
-
-    size_t ev = 0;
-    std::vector<int> stage;
-    auto mark_eager = [&](int stg) -> hipError_t {
-        if (!stage_timing || ev + 1 >= c->events.size()) return hipSuccess;
-        stage.push_back(stg);
-        return hipEventRecord(c->events[++ev], c->stream);
-    };
-    hipEvent_t t0 = c->events[0], t1 = c->events[c->events.size() - 1];
-    OXR_HIP(c, hipEventRecord(t0, c->stream));
-    int launches = 0;
+    OXR_HIP(c, hipMemsetAsync(c->eng.trav, 0, 8 * sizeof(unsigned long long), c->stream));
+    OXR_HIP(c, hipMemsetAsync(c->eng.overflow, 0, sizeof(int), c->stream));
+    OXR_HIP(c, hipEventRecord(c->t0, c->stream));
+    WindowRun r;
+    r.trace_grid = c->trace_grid;
+    r.shade_grid = c->shade_grid;
+    r.iters_per_poll = c->tuning.iters_per_poll;
+    r.count = count;
+    r.stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;
+    r.eager = (prm->flags & OXR_FLAG_EAGER) != 0 || r.stage_timing;
+    r.set_req0 = true;
+    r.npix_total = npix;
+    int launches = 0, iterations = 0, windows = 0;
+    double stage[5] = {0, 0, 0, 0, 0};
     for (int f0 = 0; f0 < prm->spp; f0 += F) {
-        int nf = std::min(F, prm->spp - f0);
-        OXR_HIP(c, wave_set_batch(p, c->frame_id + f0, f0, nf, c->stream));
-        launches += 3 + 4 * p.max_depth;
-        if (eager) {
-            OXR_HIP(c, enqueue_batch(nf, mark_eager));
-            continue;
+        const int nf = std::min(F, prm->spp - f0);
+        r.frame0 = c->frame_id + f0;
+        r.path_frame_offset = f0;
+        r.frames = nf;
+        r.total_samples = (int64_t)nf * npix;
+        for (double& x : r.stage_ms) x = 0.0;
+        hipError_t e = wave_run_window(c->eng, p, r, c->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            return fail(c, OXR_ERR_HIP, std::string("render window: ") + hipGetErrorString(e));
         }
-        oxr_ctx::Graph* g = nullptr;
-        OXR_HIP(c, graph_for(nf, g));
-        OXR_HIP(c, hipGraphLaunch(g->exec, c->stream));
-        if (stage_timing) {
-            OXR_HIP(c, hipStreamSynchronize(c->stream));
-            for (size_t i = 0; i < g->stages.size(); i++) {
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, g->events[i], g->events[i + 1]) == hipSuccess) *stage_ms[g->stages[i]] += ms;
-            }
-        }
+        for (int i = 0; i < 5; i++) stage[i] += r.stage_ms[i];
+        launches += r.launches;
+        iterations += r.iterations;
+        windows++;
     }
     OXR_HIP(c, tonemap(c->color, c->rgba, npix, c->stream));
     launches++;
-    OXR_HIP(c, hipEventRecord(t1, c->stream));
+    OXR_HIP(c, hipEventRecord(c->t1, c->stream));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long tv[8];
-    OXR_HIP(c, hipMemcpy(tv, c->trav, sizeof(tv), hipMemcpyDeviceToHost));
+    OXR_HIP(c, hipMemcpy(tv, c->eng.trav, sizeof(tv), hipMemcpyDeviceToHost));
     int ov = 0;
-    OXR_HIP(c, hipMemcpy(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost));
+    OXR_HIP(c, hipMemcpy(&ov, c->eng.overflow, sizeof(int), hipMemcpyDeviceToHost));
     float total = 0.0f;
-    (void)hipEventElapsedTime(&total, t0, t1);
+    (void)hipEventElapsedTime(&total, c->t0, c->t1);
     st.render_ms = total;
-    for (size_t i = 0; i < stage.size(); i++) {
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, c->events[i], c->events[i + 1]);
-        *stage_ms[stage[i]] += ms;
-    }
+    st.generate_ms = stage[0];
+    st.extend_ms = stage[1];
+    st.shade_ms = stage[2];
+    st.connect_ms = stage[3];
+    st.accumulate_ms = stage[4];
     st.ext_node_fetches = tv[0];
     st.ext_tri_tests = tv[1];
     st.node_fetches = tv[0] + tv[4];
@@ -729,6 +633,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.shadow_rays = tv[3];
     st.paths = (uint64_t)prm->spp * npix;
     st.kernel_launches = launches;
+    st.iterations = iterations;
+    st.windows = windows;
     c->frame_id += prm->spp;
     st.frame_id = c->frame_id;
     c->path_radiance_valid = keep ? (size_t)3 * prm->spp * npix : 0;
@@ -746,13 +652,13 @@ int oxr_set_tuning(oxr_ctx* c, const oxr_tuning* t) {
     if (!c || !t) return OXR_ERR_INVALID;
     if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > 16 || t->shade_blocks_per_cu < 1 ||
         t->shade_blocks_per_cu > 16 || !(t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32) ||
-        t->batch_paths < 1)
+        t->iters_per_poll < 2 || t->iters_per_poll > 64 || (t->iters_per_poll & 1) || t->pool_paths < 1 ||
+        t->pool_paths > ((int64_t)1 << 28) || t->window_samples < 1 || t->window_samples > ((int64_t)1 << 30))
         return fail(c, OXR_ERR_INVALID, "tuning out of range");
     OXR_HIP(c, hipStreamSynchronize(c->stream));
-    drop_graphs(c);
+    wave_drop_graphs(c->eng);
     c->tuning = *t;
-    c->trace_grid = c->num_cu * t->trace_blocks_per_cu;
-    c->shade_grid = c->num_cu * t->shade_blocks_per_cu;
+    apply_tuning(c);
     return OXR_OK;
 }
 
@@ -778,15 +684,15 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
     int rc = time_it(cur, cur_ms);
     if (rc != OXR_OK) return rc;
     const std::vector<int> stacks = {8, 16, 32}, tracebpc = {4, 8, 12, 16}, shadebpc = {2, 4, 8, 16};
-    const std::vector<int64_t> batches = {1 << 22, 1 << 23, 1 << 24, 1 << 25};
+    const std::vector<int64_t> pools = {1 << 21, 1 << 22, 1 << 23, 1 << 24};
     for (int knob = 0; knob < 4; knob++) {
-        size_t nc = knob == 0 ? stacks.size() : knob == 1 ? tracebpc.size() : knob == 2 ? shadebpc.size() : batches.size();
+        size_t nc = knob == 0 ? stacks.size() : knob == 1 ? tracebpc.size() : knob == 2 ? shadebpc.size() : pools.size();
         for (size_t i = 0; i < nc; i++) {
             oxr_tuning t = cur;
             if (knob == 0) t.stack_lds = stacks[i];
             if (knob == 1) t.trace_blocks_per_cu = tracebpc[i];
             if (knob == 2) t.shade_blocks_per_cu = shadebpc[i];
-            if (knob == 3) t.batch_paths = batches[i];
+            if (knob == 3) t.pool_paths = pools[i];
             if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0.0;
             rc = time_it(t, ms);

@@ -62,12 +62,12 @@ hipError_t material_batch(const oxr_material* d_mat, const float* d_lut_bsdf, co
 hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_mat_ids, const uint32_t* d_order,
                                     int64_t n, float* d_normals_sorted, uint32_t* d_mat_sorted, hipStream_t s);
 
-// One render request's share of a wavefront batch.  A batch holds `frames`
-// frames of every pixel of each request, laid out request after request:
-// path id = path_begin + s * width * height + pixel.
+// One render request's share of a wavefront window.  A window holds
+// `frames` frames of every pixel of each request, laid out request after
+// request: sample id = path_begin + s * width * height + pixel.
 struct ReqDesc {
-    int64_t path_begin;    // first path id of this request in the batch
-    int64_t pix_begin;     // first pixel of this request among the batch's pixels
+    int64_t path_begin;    // first sample id of this request in the window
+    int64_t pix_begin;     // first pixel of this request among the window's pixels
     int width, height;
     int frame0, frames;    // frameIDs frame0 .. frame0 + frames - 1
     int filter;
@@ -76,8 +76,34 @@ struct ReqDesc {
     float4* color;
     float4* albedo;
     float4* normal;
-    float* path_radiance;  // optional, frame-major 3 floats per path
+    float* path_radiance;  // optional, frame-major 3 floats per sample
 };
+
+// The path pool is split into kShards independent shards of pool/kShards
+// slots.  Wave w of any stage kernel serves shard w % kShards, so with the
+// round-robin dispatch of workgroups over the 8 XCDs a shard's paths stay on
+// one XCD, and every queue counter, fetch counter and sample cursor exists
+// once per shard: same-address atomics serialise (about 11 ns each on
+// MI355X), different addresses proceed in parallel.
+constexpr int kShards = 32;
+constexpr int kCtrStride = 64;  // ints between counters: one 256-byte line each
+
+// Counter ids (int index = id * kCtrStride).
+enum CtrId {
+    kCtrQueue = 0,                       // 2 x 64 region counts: [parity][region]; region k survivors, 32+k new paths
+    kCtrShEnv = kCtrQueue + 2 * 2 * kShards,   // per shard: env shadow records
+    kCtrShLight = kCtrShEnv + kShards,   // per shard: light shadow records
+    kCtrTerm = kCtrShLight + kShards,    // per shard: slots whose path ended
+    kCtrFetchExtend = kCtrTerm + kShards,
+    kCtrFetchShade = kCtrFetchExtend + kShards,
+    kCtrFetchConnEnv = kCtrFetchShade + kShards,
+    kCtrFetchConnLight = kCtrFetchConnEnv + kShards,
+    kCtrFetchRegen = kCtrFetchConnLight + kShards,
+    kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
+    kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
+    kCtrIds = kCtrWindowTotal + 1
+};
+constexpr size_t ctr_ints() { return (size_t)kCtrIds * kCtrStride; }
 
 // Wavefront integrator state for one context.
 struct WaveParams {
@@ -97,45 +123,50 @@ struct WaveParams {
     const float* lut_albedo;
     int cam_medium;
     int max_depth, rr_start;
-    // path state, capacity paths
+    // path pool: `pool` slots, shard k owns slots [k*region, (k+1)*region)
+    int64_t pool;
+    int region;        // pool / kShards
     float4* ray_o;     // origin, pre_pdf
     float4* ray_d;     // direction, rng state bits
-    float4* thr;       // history (throughput)
-    float4* rad;       // radiance
-    float4* aov_alb;
-    float4* aov_nrm;
-    uint32_t* queue[2];
-    float4* hits;
-    float4* sh_env;    // 3 float4 per shadow ray
+    float4* thr;       // history (throughput), bounce (int bits)
+    float4* rad;       // radiance, sample id (uint bits; kNoSample when the slot holds none)
+    uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
+    float4* hits;      // per queue position (2*kShards regions)
+    float4* sh_env;    // kShards regions of `region` records, 3 float4 each
     float4* sh_light;
-    int* ctr;          // device counters
-    const ReqDesc* reqs;  // device table of the batch's requests
+    uint32_t* term;    // kShards regions of `region` slot ids
+    // per-sample outputs of the window
+    float4* s_rad;
+    float4* s_alb;
+    float4* s_nrm;
+    int* ctr;          // device counters, ctr_ints() ints
+    const ReqDesc* reqs;  // device table of the window's requests
     int nreq;
     unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
     uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
     int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
     int* overflow;
 };
+constexpr uint32_t kNoSample = 0xffffffffu;
 
-enum CtrSlot {
-    kCtrQueue0 = 0,
-    kCtrQueue1 = 1,
-    kCtrShEnv = 2,
-    kCtrShLight = 3,
-    kCtrFetchExtend = 4,
-    kCtrFetchShade = 5,
-    kCtrFetchConnEnv = 6,
-    kCtrFetchConnLight = 7,
-    kCtrCount = 16
-};
-
-// Rewrites frame0 / path_frame_offset of request 0 on the device, so one
-// captured graph serves every batch of the same size.
-hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, int frames, hipStream_t s);
-hipError_t wave_generate(const WaveParams& p, int64_t npaths, hipStream_t s);
-hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s);
-hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s);
+// Window set-up: rewrites frame0 / path_frame_offset / frames of request 0
+// (single renders; the server writes its table from the host), zeroes every
+// counter, stores the window's sample count and lists every pool slot as
+// free.  Followed by one wave_regen to fill the pool.
+hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
+                            int64_t total_samples, hipStream_t s);
+// One iteration at queue parity x: extend -> shade -> connect(env) ->
+// connect(lights) -> regen.  Each stage is its own launch.
+hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStream_t s);
+hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s);
 hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
+// Finishes the paths that ended in the last shade (writes their radiance to
+// s_rad) and starts new samples in their slots, appended to queue x.
+hipError_t wave_regen(const WaveParams& p, int x, int grid, hipStream_t s);
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
+// Largest resident workgroups per CU of the extend / connect instance for
+// stack_lds, and of shade / regen.
+int wave_trace_occupancy(int stack_lds);
+int wave_shade_occupancy();
 
 }  // namespace oxr

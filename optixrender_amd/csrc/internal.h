@@ -39,6 +39,49 @@ CamBasis camera_basis(const oxr_camera& c, int w, int h);
 // Scene part of the wavefront parameters.
 void fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, const float* lut_albedo);
 
+// Path pool, shard queues, per-sample window outputs and captured iteration
+// graphs of one stream (wavefront.cpp).
+struct WaveEngine {
+    int64_t pool = 0, window = 0;  // allocated slots / samples
+    int grid = 0;                  // trace grid the stack spill is sized for
+    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr;
+    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
+    float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr;
+    int* ctr = nullptr;
+    int* overflow = nullptr;
+    unsigned long long* trav = nullptr;
+    int* host_counts = nullptr;  // pinned: two copies of one queue parity's region counts
+    hipEvent_t poll[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> stage_events;
+    struct Graph {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+};
+
+// One window: launch shape, options, and what it measured.
+struct WindowRun {
+    int trace_grid = 0, shade_grid = 0;
+    int iters_per_poll = 4;
+    bool count = false, eager = false, stage_timing = false;
+    bool set_req0 = false;  // write frame0/offset/frames into request 0 on the device
+    int frame0 = 0, path_frame_offset = 0, frames = 0;
+    int64_t total_samples = 0, npix_total = 0;
+    double stage_ms[5] = {0, 0, 0, 0, 0};  // regen (with window start), extend, shade, connect, accumulate
+    int launches = 0, iterations = 0;
+};
+
+int64_t wave_round_pool(int64_t paths);   // to a positive multiple of kShards * 64
+int wave_grid(int blocks);                // to a multiple of the blocks that give every shard a wave
+size_t wave_bytes(int64_t pool, int64_t window, int trace_grid);
+hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_grid);
+void wave_release(WaveEngine& e);
+void wave_drop_graphs(WaveEngine& e);
+void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p);
+hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hipStream_t s);
+
 }  // namespace oxr
 
 struct oxr_ctx {
@@ -58,29 +101,13 @@ struct oxr_ctx {
     uint32_t* rgba = nullptr;
     oxr_camera cam;
     int frame_id = 0;
-    // wavefront buffers
-    int64_t cap = 0;
-    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
-    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr;
-    int* ctr = nullptr;
-    unsigned long long* trav = nullptr;
-    uint32_t* gstack = nullptr;
-    int* overflow = nullptr;
+    oxr::WaveEngine eng;
     int trace_grid = 0, shade_grid = 0;
     oxr_tuning tuning;
     float* path_radiance = nullptr;
     size_t path_radiance_floats = 0;
     size_t path_radiance_valid = 0;
     oxr_render_stats stats;
-    std::vector<hipEvent_t> events;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     oxr::ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
-    // captured batch graphs, keyed by the launch parameters they bake in
-    struct Graph {
-        std::vector<uint8_t> key;
-        hipGraphExec_t exec = nullptr;
-        std::vector<hipEvent_t> events;  // recorded between stages inside the graph
-        std::vector<int> stages;
-    };
-    std::vector<Graph> graphs;
 };

@@ -1,15 +1,20 @@
-// Wavefront path tracer on gfx950: generate -> [extend -> shade -> connect]
-// x depth -> accumulate, plus the environment importance cache, the tone map
-// and the test entry points.  Replaces __raygen__renderFrame and its
-// closest-hit/miss programs (core/cuda/devicePrograms.cu:68-532) and
-// ComputeFinalPixelColorsKernel (core/cuda/PostProcessing.cu:26-48).
+// Wavefront path tracer on gfx950 with path regeneration, plus the
+// environment importance cache, the tone map and the test entry points.
+// Replaces __raygen__renderFrame and its closest-hit/miss programs
+// (core/cuda/devicePrograms.cu:68-532) and ComputeFinalPixelColorsKernel
+// (core/cuda/PostProcessing.cu:26-48).
 //
-// A path is (pixel, frame): RNG init(ix + W*iy, frameID), one sample per
-// frame -- the reference with numPixelSamples = 1 in progressive mode.  Path
-// state is SoA in HBM (64 B/path: origin+pre_pdf, direction+rng, throughput,
-// radiance); queues hold path ids and shrink as paths end.  The stage
-// kernels are persistent: each wave grabs 64 queue slots at a time from an
-// atomic counter, so one fixed launch shape serves every bounce.
+// A sample is (pixel, frame): RNG init(ix + W*iy, frameID), one sample per
+// frame -- the reference with numPixelSamples = 1 in progressive mode.  A
+// resident pool of path slots (SoA in HBM, 64 B/slot: origin+pre_pdf,
+// direction+rng, throughput+bounce, radiance+sample id) runs the loop
+//   extend -> shade -> connect(env) -> connect(lights) -> regen
+// until the window's samples are done: regen writes each ended path's
+// radiance to its sample's slot in s_rad and starts the next sample in the
+// freed slot, so every launch works on a full pool instead of a shrinking
+// batch.  The pool is split into kShards shards with their own queues and
+// counters (common.h); stage kernels are persistent and a wave takes 64 queue
+// entries of its shard per atomic.
 #include <algorithm>
 #include "common.h"
 #include "shading.h"
@@ -20,13 +25,6 @@ namespace {
 
 OXR_D int lane_id() { return (int)(threadIdx.x & 63); }
 OXR_D uint64_t lanes_below() { int l = lane_id(); return l == 0 ? 0ull : (~0ull >> (64 - l)); }
-
-// A wave takes the next 64 slots of a queue.
-OXR_D int wave_fetch(int* ctr) {
-    int base = 0;
-    if (lane_id() == 0) base = atomicAdd(ctr, 64);
-    return __shfl(base, 0);
-}
 
 // Wave-aggregated append: returns this lane's slot or -1.
 OXR_D int wave_append(int* ctr, bool want) {
@@ -180,13 +178,36 @@ __global__ void k_gather(const float* __restrict__ nrm, const uint32_t* __restri
 }
 
 // ------------------------------------------------------------ wavefront
-__global__ void k_set_batch(ReqDesc* r, int frame0, int offset, int frames) {
-    r->frame0 = frame0;
-    r->path_frame_offset = offset;
-    r->frames = frames;
+OXR_D int* ctr_at(int* ctr, int id) { return ctr + (size_t)id * kCtrStride; }
+// Shard served by the calling wave (wave-uniform).
+OXR_D int wave_shard() { return (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kShards - 1)); }
+// A wave takes the next chunk index of its shard.
+OXR_D int wave_next(int* ctr) {
+    int j = 0;
+    if (lane_id() == 0) j = atomicAdd(ctr, 1);
+    return __shfl(j, 0);
 }
 
-// Request owning a path (or pixel) index: the last descriptor whose begin <= i.
+// Queue `x` of shard k: region k (paths that survived the last shade) then
+// region kShards + k (paths started by the last regen), in 64-slot chunks.
+struct ShardQueue {
+    int n_surv, n_new, c_surv, chunks;
+    OXR_D ShardQueue(int* ctr, int x, int k) {
+        n_surv = *ctr_at(ctr, kCtrQueue + x * 2 * kShards + k);
+        n_new = *ctr_at(ctr, kCtrQueue + x * 2 * kShards + kShards + k);
+        c_surv = (n_surv + 63) >> 6;
+        chunks = c_surv + ((n_new + 63) >> 6);
+    }
+    // Queue position of this lane in chunk j, or -1 past the region's end.
+    OXR_D int64_t position(int j, int k, int region) const {
+        const bool surv = j < c_surv;
+        const int slot = ((surv ? j : j - c_surv) << 6) + lane_id();
+        if (slot >= (surv ? n_surv : n_new)) return -1;
+        return (int64_t)(surv ? k : kShards + k) * region + slot;
+    }
+};
+
+// Request owning a sample (or pixel) index: the last descriptor whose begin <= i.
 template <bool PIX>
 OXR_D int find_req(const ReqDesc* r, int n, int64_t i) {
     int lo = 0, hi = n - 1;
@@ -198,16 +219,15 @@ OXR_D int find_req(const ReqDesc* r, int n, int64_t i) {
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_generate(WaveParams p, int64_t npaths) {
-    int64_t pid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pid == 0) {
-        p.ctr[kCtrQueue0] = (int)npaths;
-        p.ctr[kCtrQueue1] = 0;
-        p.ctr[kCtrFetchExtend] = 0;
-    }
-    if (pid >= npaths) return;
-    const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, pid)];
-    const int64_t local = pid - q.path_begin;
+// Sample id of position `pos` of shard k's draw sequence: shard k owns the
+// 64-sample blocks k, k + kShards, k + 2 kShards, ... so a wave that draws
+// consecutive positions starts 64 neighbouring pixels of one frame.
+OXR_D int64_t sample_of(int k, int pos) { return (((int64_t)(pos >> 6) * kShards + k) << 6) + (pos & 63); }
+
+// Camera ray of sample g (devicePrograms.cu:214-236) into slot pid.
+OXR_D void start_path(const WaveParams& p, uint32_t pid, int64_t g) {
+    const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, g)];
+    const int64_t local = g - q.path_begin;
     const int npix = q.width * q.height;
     int s = (int)(local / npix), pix = (int)(local % npix);
     int ix = pix % q.width, iy = pix / q.width;
@@ -219,47 +239,118 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams p, int64_t npaths) 
     f3 dir = normalize(make3(q.cam_dir) + (sx - 0.5f) * make3(q.cam_h) + (sy - 0.5f) * make3(q.cam_v));
     p.ray_o[pid] = make_float4(q.cam_pos[0], q.cam_pos[1], q.cam_pos[2], 0.0f);
     p.ray_d[pid] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
-    p.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    p.queue[0][pid] = (uint32_t)pid;
+    p.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
+    p.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)g));
+}
+
+__global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame0, int offset, int frames,
+                              int total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < (int64_t)ctr_ints()) {
+        const int id = (int)(t / kCtrStride);
+        int v = 0;
+        if (t % kCtrStride == 0) {
+            if (id >= kCtrTerm && id < kCtrTerm + kShards) v = p.region;  // every slot is free
+            if (id == kCtrWindowTotal) v = total;
+        }
+        p.ctr[t] = v;
+    }
+    if (t == 0 && set_req0) {
+        r0->frame0 = frame0;
+        r0->path_frame_offset = offset;
+        r0->frames = frames;
+    }
+    if (t < p.pool) {
+        p.term[t] = (uint32_t)t;
+        p.rad[t] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoSample));
+    }
+}
+
+// Finish the paths that ended in the last shade, then start new samples in
+// their slots (queue y, new-path region of the shard).
+__global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
+    if (blockIdx.x == 0 && threadIdx.x < kShards) *ctr_at(p.ctr, kCtrFetchExtend + threadIdx.x) = 0;
+    const int k = wave_shard();
+    const int n = *ctr_at(p.ctr, kCtrTerm + k);
+    const int chunks = (n + 63) >> 6;
+    if (chunks == 0) return;
+    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
+    int* fetch = ctr_at(p.ctr, kCtrFetchRegen + k);
+    int* drawn = ctr_at(p.ctr, kCtrSamplePos + k);
+    int* qn_count = ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + kShards + k);
+    const uint32_t* term = p.term + (size_t)k * p.region;
+    uint32_t* qn = p.queue[y] + (size_t)(kShards + k) * p.region;
+    while (true) {
+        const int j = wave_next(fetch);
+        if (j >= chunks) break;
+        const int slot = (j << 6) + lane_id();
+        const bool have = slot < n;
+        uint32_t pid = 0;
+        if (have) {
+            pid = term[slot];
+            float4 r = p.rad[pid];
+            uint32_t g = __float_as_uint(r.w);
+            if (g != kNoSample) p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
+        }
+        const uint64_t m = __ballot(have);
+        int base = 0;
+        if (lane_id() == 0) base = atomicAdd(drawn, (int)__popcll(m));
+        base = __shfl(base, 0);
+        const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
+        const bool start = have && g < total;
+        if (start) start_path(p, pid, g);
+        const int o = wave_append(qn_count, start);
+        if (o >= 0) qn[o] = pid;
+    }
 }
 
 template <bool COUNT, int S>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int bounce) {
+__global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int x) {
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
     __shared__ uint32_t lds[S * kTraceBlock];
-    const int qi = bounce & 1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.ctr[qi ^ 1] = 0;
-        p.ctr[kCtrShEnv] = 0;
-        p.ctr[kCtrShLight] = 0;
-        p.ctr[kCtrFetchShade] = 0;
-        p.ctr[kCtrFetchConnEnv] = 0;
-        p.ctr[kCtrFetchConnLight] = 0;
+    if (blockIdx.x == 0) {
+        // counters of the stages after this one, all idle now
+        const int t = threadIdx.x;
+        if (t < 2 * kShards) *ctr_at(p.ctr, kCtrQueue + (x ^ 1) * 2 * kShards + t) = 0;
+        if (t < kShards) {
+            *ctr_at(p.ctr, kCtrShEnv + t) = 0;
+            *ctr_at(p.ctr, kCtrShLight + t) = 0;
+            *ctr_at(p.ctr, kCtrTerm + t) = 0;
+            *ctr_at(p.ctr, kCtrFetchShade + t) = 0;
+            *ctr_at(p.ctr, kCtrFetchConnEnv + t) = 0;
+            *ctr_at(p.ctr, kCtrFetchConnLight + t) = 0;
+            *ctr_at(p.ctr, kCtrFetchRegen + t) = 0;
+        }
+        if (t == 0) {
+            unsigned long long rays = 0;
+            for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, kCtrQueue + x * 2 * kShards + r);
+            if (rays) atomicAdd(&p.traversal[2], rays);
+        }
     }
-    const int count = p.ctr[qi];
-    const uint32_t* q = p.queue[qi];
+    const int k = wave_shard();
+    const ShardQueue sq(p.ctr, x, k);
+    if (sq.chunks == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchExtend + k);
+    const uint32_t* q = p.queue[x];
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
-    uint32_t nn = 0, nt = 0, nr = 0;
+    uint32_t nn = 0, nt = 0;
     while (true) {
-        int base = wave_fetch(&p.ctr[kCtrFetchExtend]);
-        if (base >= count) break;
-        int slot = base + lane_id();
-        if (slot < count) {
-            uint32_t pid = q[slot];
+        const int j = wave_next(fetch);
+        if (j >= sq.chunks) break;
+        const int64_t qp = sq.position(j, k, p.region);
+        if (qp >= 0) {
+            uint32_t pid = q[qp];
             float4 o = p.ray_o[pid], d = p.ray_d[pid];
             Hit h = traverse<false, COUNT>(p.bvh, xyz(o), xyz(d), kEps, kFltMax, st, nn, nt, overflow);
-            p.hits[slot] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
-            nr++;
+            p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
         }
     }
     if (COUNT) {
         wave_count_add(&p.traversal[0], nn);
         wave_count_add(&p.traversal[1], nt);
     }
-    wave_count_add(&p.traversal[2], nr);
 }
 
 template <bool COUNT, int S>
@@ -267,20 +358,28 @@ __global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
     __shared__ uint32_t lds[S * kTraceBlock];
-    const int count = p.ctr[which == 0 ? kCtrShEnv : kCtrShLight];
-    const float4* rec = which == 0 ? p.sh_env : p.sh_light;
-    int* fetch = &p.ctr[which == 0 ? kCtrFetchConnEnv : kCtrFetchConnLight];
+    const int cnt_id = which == 0 ? kCtrShEnv : kCtrShLight;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        unsigned long long rays = 0;
+        for (int r = 0; r < kShards; r++) rays += (unsigned)*ctr_at(p.ctr, cnt_id + r);
+        if (rays) atomicAdd(&p.traversal[3], rays);
+    }
+    const int k = wave_shard();
+    const int count = *ctr_at(p.ctr, cnt_id + k);
+    const int chunks = (count + 63) >> 6;
+    if (chunks == 0) return;
+    const float4* rec = (which == 0 ? p.sh_env : p.sh_light) + 3 * (size_t)k * p.region;
+    int* fetch = ctr_at(p.ctr, (which == 0 ? kCtrFetchConnEnv : kCtrFetchConnLight) + k);
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
-    uint32_t nn = 0, nt = 0, nr = 0;
+    uint32_t nn = 0, nt = 0;
     while (true) {
-        int base = wave_fetch(fetch);
-        if (base >= count) break;
-        int slot = base + lane_id();
+        const int j = wave_next(fetch);
+        if (j >= chunks) break;
+        const int slot = (j << 6) + lane_id();
         if (slot < count) {
             float4 a = rec[3 * (size_t)slot], b = rec[3 * (size_t)slot + 1];
             Hit h = traverse<true, COUNT>(p.bvh, xyz(a), xyz(b), kEps, a.w, st, nn, nt, overflow);
-            nr++;
             if (h.leaf == kNoHit) {
                 float4 c = rec[3 * (size_t)slot + 2];
                 uint32_t pid = __float_as_uint(b.w);
@@ -296,7 +395,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which
         wave_count_add(&p.traversal[4], nn);
         wave_count_add(&p.traversal[5], nt);
     }
-    wave_count_add(&p.traversal[3], nr);
 }
 
 struct ShadowOut {
@@ -326,7 +424,7 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 // for one path, after its radiance ray was traced.  Returns whether the path
 // continues; NEE shadow rays are returned for the connect stage with their
 // contribution precomputed (it is added only if the ray is unoccluded).
-OXR_D bool shade_path(const WaveParams& p, const Lut& lut, int bounce, uint32_t pid, float4 hit, ShadowOut& sh_env,
+OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, ShadowOut& sh_env,
                       ShadowOut& sh_light) {
     sh_env.emit = false;
     sh_light.emit = false;
@@ -335,8 +433,10 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, int bounce, uint32_t 
     float pre_pdf = ro.w;
     Rng rng{__float_as_uint(rd.w)};
     f3 history = xyz(th);
+    const int bounce = __float_as_int(th.w);
     float4 rad4 = p.rad[pid];
     f3 radiance = xyz(rad4);
+    const uint32_t g = __float_as_uint(rad4.w);
     f3 V = -dir, L = dir;
     float mult_trans_pdf = 1.0f;
 
@@ -363,15 +463,15 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, int bounce, uint32_t 
         mi_in = s.m->in_medium;
         mi_out = s.m->out_medium;
         if (bounce == 0) {
-            p.aov_nrm[pid] = make_float4(Ns.x, Ns.y, Ns.z, 1.0f);
-            p.aov_alb[pid] = make_float4(s.m->albedo[0], s.m->albedo[1], s.m->albedo[2], 1.0f);
+            p.s_nrm[g] = make_float4(Ns.x, Ns.y, Ns.z, 1.0f);
+            p.s_alb[g] = make_float4(s.m->albedo[0], s.m->albedo[1], s.m->albedo[2], 1.0f);
         }
     } else {
         distance = kFltMax;
         mi_in = mi_out = p.cam_medium;
         if (bounce == 0) {
-            p.aov_nrm[pid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-            p.aov_alb[pid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         }
     }
     float closest = distance;
@@ -528,13 +628,14 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, int bounce, uint32_t 
         }
         if (alive && any_nan(history)) alive = false;                       // devicePrograms.cu:487-489
     }
-    p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-    if (alive) {
+    p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, rad4.w);
+    const bool cont = alive && bounce + 1 < p.max_depth;
+    if (cont) {
         p.ray_o[pid] = make_float4(pos.x, pos.y, pos.z, pre_pdf);
         p.ray_d[pid] = make_float4(L.x, L.y, L.z, __uint_as_float(rng.s));
-        p.thr[pid] = make_float4(history.x, history.y, history.z, 0.0f);
+        p.thr[pid] = make_float4(history.x, history.y, history.z, __int_as_float(bounce + 1));
     }
-    return alive && bounce + 1 < p.max_depth;
+    return cont;
 }
 
 OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) {
@@ -546,30 +647,41 @@ OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) 
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(WaveParams p, int bounce) {
-    const int qi = bounce & 1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.ctr[kCtrFetchExtend] = 0;
-    const int count = p.ctr[qi];
-    const uint32_t* q = p.queue[qi];
-    uint32_t* qo = p.queue[qi ^ 1];
+__global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
+    const int k = wave_shard();
+    const ShardQueue sq(p.ctr, x, k);
+    if (sq.chunks == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchShade + k);
+    const uint32_t* q = p.queue[x];
+    const size_t rbase = (size_t)k * p.region;
+    uint32_t* qo = p.queue[x ^ 1] + rbase;  // survivors region of shard k
+    int* qo_count = ctr_at(p.ctr, kCtrQueue + (x ^ 1) * 2 * kShards + k);
+    uint32_t* term = p.term + rbase;
+    int* term_count = ctr_at(p.ctr, kCtrTerm + k);
+    float4* se = p.sh_env + 3 * rbase;
+    float4* sl = p.sh_light + 3 * rbase;
+    int* se_count = ctr_at(p.ctr, kCtrShEnv + k);
+    int* sl_count = ctr_at(p.ctr, kCtrShLight + k);
     Lut lut{p.lut_bsdf, p.lut_albedo};
     while (true) {
-        int base = wave_fetch(&p.ctr[kCtrFetchShade]);
-        if (base >= count) break;
-        int slot = base + lane_id();
-        bool active = slot < count;
+        const int j = wave_next(fetch);
+        if (j >= sq.chunks) break;
+        const int64_t qp = sq.position(j, k, p.region);
+        const bool active = qp >= 0;
         uint32_t pid = 0;
         bool cont = false;
-        ShadowOut se, sl;
-        se.emit = sl.emit = false;
+        ShadowOut sev, slv;
+        sev.emit = slv.emit = false;
         if (active) {
-            pid = q[slot];
-            cont = shade_path(p, lut, bounce, pid, p.hits[slot], se, sl);
+            pid = q[qp];
+            cont = shade_path(p, lut, pid, p.hits[qp], sev, slv);
         }
-        int o = wave_append(&p.ctr[qi ^ 1], cont);
+        int o = wave_append(qo_count, cont);
         if (o >= 0) qo[o] = pid;
-        emit_shadow(p.sh_env, &p.ctr[kCtrShEnv], se, pid);
-        emit_shadow(p.sh_light, &p.ctr[kCtrShLight], sl, pid);
+        o = wave_append(term_count, active && !cont);
+        if (o >= 0) term[o] = pid;
+        emit_shadow(se, se_count, sev, pid);
+        emit_shadow(sl, sl_count, slv, pid);
     }
 }
 
@@ -585,8 +697,8 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     for (int s = 0; s < q.frames; s++) {
         size_t pid = (size_t)(q.path_begin + s * npix + pix);
         int frame = q.frame0 + s;
-        float4 r = p.rad[pid];
-        float4 al = p.aov_alb[pid], no = p.aov_nrm[pid];
+        float4 r = p.s_rad[pid];
+        float4 al = p.s_alb[pid], no = p.s_nrm[pid];
         float4 x = make_float4(r.x, r.y, r.z, 1.0f);
         if (isnan(r.x) || isnan(r.y) || isnan(r.z)) {
             x = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
@@ -662,21 +774,24 @@ hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const
     return hipGetLastError();
 }
 
-hipError_t wave_set_batch(const WaveParams& p, int frame0, int path_frame_offset, int frames, hipStream_t s) {
-    k_set_batch<<<1, 1, 0, s>>>(const_cast<ReqDesc*>(p.reqs), frame0, path_frame_offset, frames);
+hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
+                            int64_t total_samples, hipStream_t s) {
+    const int64_t n = std::max<int64_t>(p.pool, (int64_t)ctr_ints());
+    k_window_init<<<blocks(n, 256), 256, 0, s>>>(p, const_cast<ReqDesc*>(p.reqs), set_req0 ? 1 : 0, frame0,
+                                                 path_frame_offset, frames, (int)total_samples);
     return hipGetLastError();
 }
 
-hipError_t wave_generate(const WaveParams& p, int64_t np, hipStream_t s) {
-    k_generate<<<blocks(np, 256), 256, 0, s>>>(p, np);
+hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s) {
+    k_regen<<<grid, 256, 0, s>>>(p, y);
     return hipGetLastError();
 }
 
 template <bool C>
-void launch_extend(const WaveParams& p, int bounce, int grid, hipStream_t s) {
-    if (p.stack_lds <= 8) k_extend<C, 8><<<grid, kTraceBlock, 0, s>>>(p, bounce);
-    else if (p.stack_lds <= 16) k_extend<C, 16><<<grid, kTraceBlock, 0, s>>>(p, bounce);
-    else k_extend<C, 32><<<grid, kTraceBlock, 0, s>>>(p, bounce);
+void launch_extend(const WaveParams& p, int x, int grid, hipStream_t s) {
+    if (p.stack_lds <= 8) k_extend<C, 8><<<grid, kTraceBlock, 0, s>>>(p, x);
+    else if (p.stack_lds <= 16) k_extend<C, 16><<<grid, kTraceBlock, 0, s>>>(p, x);
+    else k_extend<C, 32><<<grid, kTraceBlock, 0, s>>>(p, x);
 }
 template <bool C>
 void launch_connect(const WaveParams& p, int which, int grid, hipStream_t s) {
@@ -685,14 +800,14 @@ void launch_connect(const WaveParams& p, int which, int grid, hipStream_t s) {
     else k_connect<C, 32><<<grid, kTraceBlock, 0, s>>>(p, which);
 }
 
-hipError_t wave_extend(const WaveParams& p, int bounce, int grid, bool count, hipStream_t s) {
-    if (count) launch_extend<true>(p, bounce, grid, s);
-    else launch_extend<false>(p, bounce, grid, s);
+hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStream_t s) {
+    if (count) launch_extend<true>(p, x, grid, s);
+    else launch_extend<false>(p, x, grid, s);
     return hipGetLastError();
 }
 
-hipError_t wave_shade(const WaveParams& p, int bounce, int grid, hipStream_t s) {
-    k_shade<<<grid, 256, 0, s>>>(p, bounce);
+hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
+    k_shade<<<grid, 256, 0, s>>>(p, x);
     return hipGetLastError();
 }
 
@@ -701,6 +816,23 @@ hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hi
     else launch_connect<false>(p, which, grid, s);
     return hipGetLastError();
 }
+
+template <class K>
+int occupancy(K kernel, int block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0) != hipSuccess) return 1;
+    return std::max(n, 1);
+}
+
+int wave_trace_occupancy(int stack_lds) {
+    if (stack_lds <= 8)
+        return std::min(occupancy(k_extend<false, 8>, kTraceBlock), occupancy(k_connect<false, 8>, kTraceBlock));
+    if (stack_lds <= 16)
+        return std::min(occupancy(k_extend<false, 16>, kTraceBlock), occupancy(k_connect<false, 16>, kTraceBlock));
+    return std::min(occupancy(k_extend<false, 32>, kTraceBlock), occupancy(k_connect<false, 32>, kTraceBlock));
+}
+
+int wave_shade_occupancy() { return std::min(occupancy(k_shade, 256), occupancy(k_regen, 256)); }
 
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s) {
     k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, npix);

@@ -9,9 +9,10 @@
 //   2. takes the active requests that share (scene, max depth, RR start,
 //      camera medium) with the oldest one and hands out frames round-robin up
 //      to the tick's path budget (every request gets at least one frame),
-//   3. runs one shared wavefront over all of them -- generate, depth x
-//      (extend, shade, connect), accumulate -- and tone-maps the requests that
-//      finished, recording their latency.
+//   3. runs one shared sample window over all of them on the path pool
+//      (extend, shade, connect, regenerate until every sample is done, then
+//      accumulate) and tone-maps the requests that finished, recording their
+//      latency.
 // A path is (pixel, frameID) of its own request, so a request's image does
 // not depend on what it shared the device with.
 #include <hip/hip_runtime.h>
@@ -56,14 +57,9 @@ struct oxr_server {
     hipStream_t stream = nullptr;
     size_t budget = 0, wave_bytes = 0, in_use = 0;
     int max_active = 0;
-    int64_t tick_paths = 0, cap = 0;
-    int trace_grid = 0, shade_grid = 0, stack_lds = 32;
-    // shared wavefront buffers
-    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr, *aov_alb = nullptr, *aov_nrm = nullptr;
-    float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr, *gstack = nullptr;
-    int *ctr = nullptr, *overflow = nullptr;
-    unsigned long long* trav = nullptr;
+    int64_t tick_paths = 0, cap = 0, pool = 0;
+    int trace_grid = 0, shade_grid = 0, stack_lds = 32, iters_per_poll = 4;
+    WaveEngine eng;  // shared path pool; a tick is one sample window
     ReqDesc* reqs_dev = nullptr;
     ReqDesc* reqs_host = nullptr;  // pinned staging
     int max_reqs = 0;
@@ -170,25 +166,20 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     if (e != hipSuccess) return e;
     WaveParams p;
     fill_scene_params(p, scene, c->lut_bsdf, c->lut_albedo);
+    wave_bind(s->eng, std::min(s->pool, wave_round_pool(path)), p);
     p.cam_medium = r0.camera.medium;
     p.max_depth = r0.max_depth > 0 ? r0.max_depth : 25;
     p.rr_start = r0.rr_start;
-    p.ray_o = s->ray_o; p.ray_d = s->ray_d; p.thr = s->thr; p.rad = s->rad;
-    p.aov_alb = s->aov_alb; p.aov_nrm = s->aov_nrm;
-    p.queue[0] = s->q0; p.queue[1] = s->q1;
-    p.hits = s->hits; p.sh_env = s->sh_env; p.sh_light = s->sh_light;
-    p.ctr = s->ctr; p.traversal = s->trav; p.gstack = s->gstack; p.overflow = s->overflow;
     p.reqs = s->reqs_dev;
     p.nreq = (int)group.size();
     p.stack_lds = s->stack_lds;
-    if ((e = wave_generate(p, path, s->stream)) != hipSuccess) return e;
-    for (int b = 0; b < p.max_depth; b++) {
-        if ((e = wave_extend(p, b, s->trace_grid, false, s->stream)) != hipSuccess) return e;
-        if ((e = wave_shade(p, b, s->shade_grid, s->stream)) != hipSuccess) return e;
-        if ((e = wave_connect(p, 0, s->trace_grid, false, s->stream)) != hipSuccess) return e;
-        if ((e = wave_connect(p, 1, s->trace_grid, false, s->stream)) != hipSuccess) return e;
-    }
-    if ((e = wave_accumulate(p, pix, s->stream)) != hipSuccess) return e;
+    WindowRun run;
+    run.trace_grid = s->trace_grid;
+    run.shade_grid = s->shade_grid;
+    run.iters_per_poll = s->iters_per_poll;
+    run.total_samples = path;
+    run.npix_total = pix;
+    if ((e = wave_run_window(s->eng, p, run, s->stream)) != hipSuccess) return e;
     for (size_t i = 0; i < group.size(); i++) {
         Request* q = group[i];
         if (q->frames_done + frames[i] >= q->r.spp)
@@ -301,9 +292,10 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->cap = cfg->tick_paths + (int64_t)cfg->max_frame_pixels;  // one frame of the largest request may overshoot
     s->trace_grid = c->trace_grid;
     s->shade_grid = c->shade_grid;
-    const int64_t n = s->cap;
     s->stack_lds = c->tuning.stack_lds;
-    s->wave_bytes = (size_t)n * (16 * 7 + 16 * 6 + 8) + gstack_words(s->trace_grid) * 4;
+    s->iters_per_poll = c->tuning.iters_per_poll;
+    s->pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, s->cap));
+    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid);
     auto bail = [&](int code, const char* m) {
         c->err = m;
         oxr_server_destroy(s);
@@ -311,18 +303,11 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     };
     if (s->wave_bytes >= s->budget) return bail(OXR_ERR_INVALID, "server: budget smaller than the wavefront buffers");
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP, "stream");
-    if (salloc(&s->ray_o, n) || salloc(&s->ray_d, n) || salloc(&s->thr, n) || salloc(&s->rad, n) ||
-        salloc(&s->aov_alb, n) || salloc(&s->aov_nrm, n) || salloc(&s->hits, n) || salloc(&s->sh_env, 3 * n) ||
-        salloc(&s->sh_light, 3 * n) || salloc(&s->q0, n) || salloc(&s->q1, n) ||
-        salloc(&s->gstack, gstack_words(s->trace_grid)) || salloc(&s->ctr, kCtrCount) ||
-        salloc(&s->overflow, 4) || salloc(&s->trav, 8) || salloc(&s->reqs_dev, s->max_reqs))
+    if (s->cap > ((int64_t)1 << 31) - 1) return bail(OXR_ERR_INVALID, "server: tick larger than 2^31 samples");
+    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs))
         return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
     if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
         return bail(OXR_ERR_OOM, "server: pinned staging");
-    if (hipMemset(s->ctr, 0, kCtrCount * sizeof(int)) != hipSuccess ||
-        hipMemset(s->overflow, 0, 4 * sizeof(int)) != hipSuccess ||
-        hipMemset(s->trav, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
-        return bail(OXR_ERR_HIP, "server: memset");
     s->started = Clock::now();
     s->worker = std::thread(worker_loop, s);
     *out = s;
@@ -343,9 +328,8 @@ int oxr_server_destroy(oxr_server* s) {
         free_request_buffers(s, kv.second);
         delete kv.second;
     }
-    sfree(s->ray_o); sfree(s->ray_d); sfree(s->thr); sfree(s->rad); sfree(s->aov_alb); sfree(s->aov_nrm);
-    sfree(s->hits); sfree(s->sh_env); sfree(s->sh_light); sfree(s->q0); sfree(s->q1); sfree(s->gstack);
-    sfree(s->ctr); sfree(s->overflow); sfree(s->trav); sfree(s->reqs_dev);
+    wave_release(s->eng);
+    sfree(s->reqs_dev);
     if (s->reqs_host) (void)hipHostFree(s->reqs_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

@@ -210,35 +210,46 @@ def per_path_check(gpu, ref, label):
     return frac
 
 
-def _render_pair(ctx, scene, w, h, spp, depth, rr):
+def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None):
+    """GPU render and oracle render of the same scene.  pool: path slots of
+    the GPU pool (None = default); a pool smaller than spp*w*h restarts
+    every slot many times."""
     sid = ctx.upload(scene)
+    base = ctx.get_tuning()
     try:
         ctx.build_bvh(sid)
         ctx.select(sid)
         ctx.resize(w, h)
         ctx.set_camera(scene.camera)
+        if pool is not None:
+            t = ctx.get_tuning()
+            t.pool_paths = pool
+            ctx.set_tuning(t)
         ctx.render(spp=spp, max_depth=depth, rr_start=rr, flags=abi.OXR_FLAG_KEEP_PATH_RADIANCE)
         paths = ctx.download_path_radiance(spp, w, h)
         color = ctx.download_aov(0, w, h)
         albedo = ctx.download_aov(1, w, h)
     finally:
+        ctx.set_tuning(base)
         ctx.release(sid)
     rc, ra, rn, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, keep_paths=True)
     return paths, rp, color, rc, albedo, ra
 
 
-def test_render_cornell_per_path(ctx):
+@pytest.mark.parametrize("pool", [None, 2048])
+def test_render_cornell_per_path(ctx, pool):
     sc = scenes.cornell_spheres()
-    print("[render] cornell 64x64x4", flush=True)
-    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3)
+    print(f"[render] cornell 64x64x4 pool={pool}", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3, pool)
     assert per_path_check(paths, rp, "cornell") >= PASS_FRACTION
     assert np.nanmax(np.abs(albedo - ra)) < 1e-5
 
 
-def test_render_sphere_field_per_path(ctx):
+@pytest.mark.parametrize("pool", [None, 4096])
+def test_render_sphere_field_per_path(ctx, pool):
     sc = scenes.sphere_field(subdiv=2, env_w=256, env_h=128)
-    print("[render] sphere_field 96x54x2", flush=True)
-    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 96, 54, 2, 16, 0)
+    print(f"[render] sphere_field 96x54x2 pool={pool}", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 96, 54, 2, 16, 0, pool)
     assert per_path_check(paths, rp, "sphere_field") >= PASS_FRACTION
 
 

@@ -1,6 +1,6 @@
-"""Render-loop invariants on the GPU: graph replay, batch size and progressive
-accumulation do not change a single bit; the RGBA8 output equals the tone
-map of the colour buffer."""
+"""Render-loop invariants on the GPU: graph replay, window size, path-pool
+size, launch tuning and progressive accumulation do not change a single bit;
+the RGBA8 output equals the tone map of the colour buffer."""
 import numpy as np
 import pytest
 
@@ -90,18 +90,38 @@ def test_stage_timing_and_counts(ctx, cornell):
     assert st.extend_ms > 0 and st.shade_ms > 0
 
 
-@pytest.mark.parametrize("stack,tbpc,sbpc,batch", [(8, 4, 2, 1 << 12), (16, 16, 16, 1 << 14), (32, 8, 8, 1 << 24)])
-def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, batch):
+# (stack_lds, trace blocks/CU, shade blocks/CU, iterations per poll, pool slots, window samples).
+# A pool of 2048 slots (64 per shard) for a 48x40x4 render restarts every
+# slot several times, and a window of one frame splits the render in four.
+@pytest.mark.parametrize("stack,tbpc,sbpc,iters,pool,window", [
+    (8, 4, 2, 2, 2048, 1 << 12),
+    (16, 16, 16, 6, 4096, W * H),
+    (32, 8, 8, 4, 1 << 23, 1 << 27),
+    (32, 1, 1, 64, 2048, 3 * W * H),
+])
+def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, iters, pool, window):
     sc, sid = cornell
     base = ctx.get_tuning()
     try:
         ref, _, _ = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
-        t = abi.Tuning(tbpc, sbpc, stack, 0, batch)
+        t = abi.Tuning(tbpc, sbpc, stack, iters, pool, window)
+        print(f"[render] tuning {t.as_dict()}", flush=True)
         ctx.set_tuning(t)
-        got, _, _ = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
+        got, _, st = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
         assert np.array_equal(ref.view(np.uint32), got.view(np.uint32))
+        assert st.paths == 4 * W * H and st.windows >= 1 and st.iterations >= 8
     finally:
         ctx.set_tuning(base)
+
+
+def test_invalid_tuning_is_rejected(ctx):
+    base = ctx.get_tuning()
+    for bad in [abi.Tuning(0, 8, 32, 4, 1 << 20, 1 << 20), abi.Tuning(8, 8, 24, 4, 1 << 20, 1 << 20),
+                abi.Tuning(8, 8, 32, 3, 1 << 20, 1 << 20), abi.Tuning(8, 8, 32, 4, 0, 1 << 20),
+                abi.Tuning(8, 8, 32, 4, 1 << 20, 0)]:
+        with pytest.raises(abi.OxrError):
+            ctx.set_tuning(bad)
+    assert ctx.get_tuning().as_dict() == base.as_dict()
 
 
 def test_autotune_picks_a_valid_tuning(ctx, cornell):
