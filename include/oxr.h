@@ -176,6 +176,9 @@ int oxr_get_stats(oxr_ctx* ctx, oxr_render_stats* out);
 int oxr_download_rgba8(oxr_ctx* ctx, uint32_t* out, size_t count);
 /* which: 0 colour, 1 albedo, 2 normal; count = width*height float4. */
 int oxr_download_aov(oxr_ctx* ctx, int which, float* out_rgba, size_t count);
+/* First-hit depth AOV, running mean of d / (1 + d) (1 for a miss): a
+   denoiser input with no reference counterpart.  count = width*height. */
+int oxr_download_depth(oxr_ctx* ctx, float* out, size_t count);
 /* radiance of every path of the last render call: spp*width*height*3 floats,
    frame-major.  Needs OXR_FLAG_KEEP_PATH_RADIANCE. */
 int oxr_download_path_radiance(oxr_ctx* ctx, float* out, size_t count);

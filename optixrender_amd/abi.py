@@ -293,6 +293,7 @@ SIGNATURES = {
     "oxr_get_stats": (C.c_int, [_P, _P]),
     "oxr_download_rgba8": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_download_aov": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "oxr_download_depth": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_download_path_radiance": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_server_create": (C.c_int, [_P, _P, C.POINTER(C.c_void_p)]),
     "oxr_server_destroy": (C.c_int, [_P]),

@@ -239,6 +239,7 @@ int oxr_destroy(oxr_ctx* c) {
     c->scenes.clear();
     wave_release(c->eng);
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
+    dfree(c->depth);
     dfree(c->path_radiance); dfree(c->reqs);
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
@@ -491,12 +492,14 @@ int oxr_resize(oxr_ctx* c, int w, int h) {
     if (!c || w <= 0 || h <= 0 || (int64_t)w * h > (1LL << 30)) return fail(c, OXR_ERR_INVALID, "invalid size");
     OXR_HIP(c, hipSetDevice(c->device));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
-    dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
+    dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba); dfree(c->depth);
     const size_t np = (size_t)w * h;
     OXR_HIP(c, dalloc(&c->color, np));
     OXR_HIP(c, dalloc(&c->albedo, np));
     OXR_HIP(c, dalloc(&c->normal, np));
     OXR_HIP(c, dalloc(&c->rgba, np));
+    OXR_HIP(c, dalloc(&c->depth, np));
+    OXR_HIP(c, hipMemset(c->depth, 0, 4 * np));
     OXR_HIP(c, hipMemset(c->color, 0, 16 * np));
     OXR_HIP(c, hipMemset(c->albedo, 0, 16 * np));
     OXR_HIP(c, hipMemset(c->normal, 0, 16 * np));
@@ -564,6 +567,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     rq.color = c->color;
     rq.albedo = c->albedo;
     rq.normal = c->normal;
+    rq.depth = c->depth;
     rq.path_radiance = keep ? c->path_radiance : nullptr;
     OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
@@ -728,6 +732,14 @@ int oxr_download_aov(oxr_ctx* c, int which, float* out, size_t count) {
     float4* src = which == 0 ? c->color : (which == 1 ? c->albedo : c->normal);
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     OXR_HIP(c, hipMemcpy(out, src, 16 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_download_depth(oxr_ctx* c, float* out, size_t count) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    if (count != (size_t)c->width * c->height) return fail(c, OXR_ERR_INVALID, "count must equal width*height");
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    OXR_HIP(c, hipMemcpy(out, c->depth, 4 * count, hipMemcpyDeviceToHost));
     return OXR_OK;
 }
 

@@ -76,6 +76,7 @@ struct ReqDesc {
     float4* color;
     float4* albedo;
     float4* normal;
+    float* depth;          // first-hit depth d / (1 + d), 1 for a miss
     float* path_radiance;  // optional, frame-major 3 floats per sample
 };
 
@@ -139,6 +140,7 @@ struct WaveParams {
     float4* s_rad;
     float4* s_alb;
     float4* s_nrm;
+    float* s_dep;
     int* ctr;          // device counters, ctr_ints() ints
     const ReqDesc* reqs;  // device table of the window's requests
     int nreq;

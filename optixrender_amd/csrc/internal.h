@@ -47,6 +47,7 @@ struct WaveEngine {
     float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr;
     float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
     float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
+    float* s_dep = nullptr;
     uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr;
     int* ctr = nullptr;
     int* overflow = nullptr;
@@ -98,6 +99,7 @@ struct oxr_ctx {
     float4* color = nullptr;
     float4* albedo = nullptr;
     float4* normal = nullptr;
+    float* depth = nullptr;
     uint32_t* rgba = nullptr;
     oxr_camera cam;
     int frame_id = 0;

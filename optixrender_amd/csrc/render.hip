@@ -480,6 +480,9 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     int mi_light = 0;
     bool hitLight = light_trace(p, org, dir, mi_light, light_radiance, light_pdf, closest);
     if (hitLight) mi_in = mi_out = mi_light;
+    // depth AOV (no reference counterpart; a denoiser input): first surface
+    // or light along the camera ray as d / (1 + d), 1 for a miss
+    if (bounce == 0) p.s_dep[g] = closest < kFltMax ? closest / (1.0f + closest) : 1.0f;
 
     bool scattered = false;
     int medium_index = -1;
@@ -694,16 +697,19 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     const int64_t pix = t - q.pix_begin;
     const int64_t npix = (int64_t)q.width * q.height;
     float4 c = q.color[pix], a = q.albedo[pix], nr = q.normal[pix];
+    float dz = q.depth[pix];
     for (int s = 0; s < q.frames; s++) {
         size_t pid = (size_t)(q.path_begin + s * npix + pix);
         int frame = q.frame0 + s;
         float4 r = p.s_rad[pid];
         float4 al = p.s_alb[pid], no = p.s_nrm[pid];
         float4 x = make_float4(r.x, r.y, r.z, 1.0f);
+        float z = p.s_dep[pid];
         if (isnan(r.x) || isnan(r.y) || isnan(r.z)) {
             x = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
             al = x;
             no = x;
+            z = 0.0f;
         }
         if (q.path_radiance) {
             float* pr = q.path_radiance + 3 * ((size_t)(q.path_frame_offset + s) * npix + pix);
@@ -714,14 +720,17 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
             x.x = (x.x + f * c.x) / g; x.y = (x.y + f * c.y) / g; x.z = (x.z + f * c.z) / g; x.w = (x.w + f * c.w) / g;
             al.x = (al.x + f * a.x) / g; al.y = (al.y + f * a.y) / g; al.z = (al.z + f * a.z) / g; al.w = (al.w + f * a.w) / g;
             no.x = (no.x + f * nr.x) / g; no.y = (no.y + f * nr.y) / g; no.z = (no.z + f * nr.z) / g; no.w = (no.w + f * nr.w) / g;
+            z = (z + f * dz) / g;
         }
         c = x;
         a = al;
         nr = no;
+        dz = z;
     }
     q.color[pix] = c;
     q.albedo[pix] = a;
     q.normal[pix] = nr;
+    q.depth[pix] = dz;
 }
 
 inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }

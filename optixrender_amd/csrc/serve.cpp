@@ -43,12 +43,13 @@ struct Request {
     int frames_done = 0;
     size_t bytes = 0;
     float4 *color = nullptr, *albedo = nullptr, *normal = nullptr;
+    float* depth = nullptr;
     uint32_t* rgba = nullptr;
     Clock::time_point submitted, finished;
     std::string error;
 };
 
-size_t request_bytes(const oxr_request& r) { return (size_t)r.width * r.height * (3 * 16 + 4); }
+size_t request_bytes(const oxr_request& r) { return (size_t)r.width * r.height * (3 * 16 + 4 + 4); }
 
 }  // namespace
 
@@ -94,7 +95,7 @@ void sfree(T*& p) {
 }
 
 void free_request_buffers(oxr_server* s, Request* q) {
-    sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba);
+    sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba); sfree(q->depth);
     if (q->bytes) s->in_use -= q->bytes;
     q->bytes = 0;
 }
@@ -115,6 +116,8 @@ void admit(oxr_server* s) {
         size_t np = (size_t)q->r.width * q->r.height;
         if ((e = salloc(&q->color, np)) != hipSuccess || (e = salloc(&q->albedo, np)) != hipSuccess ||
             (e = salloc(&q->normal, np)) != hipSuccess || (e = salloc(&q->rgba, np)) != hipSuccess ||
+            (e = salloc(&q->depth, np)) != hipSuccess ||
+            (e = hipMemsetAsync(q->depth, 0, 4 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->color, 0, 16 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->albedo, 0, 16 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->normal, 0, 16 * np, s->stream)) != hipSuccess) {
@@ -157,6 +160,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
         d.color = q->color;
         d.albedo = q->albedo;
         d.normal = q->normal;
+        d.depth = q->depth;
         const int64_t np = (int64_t)q->r.width * q->r.height;
         path += np * frames[i];
         pix += np;

@@ -41,7 +41,7 @@ int64_t wave_round_pool(int64_t paths) {
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
     // per slot: state 64, hits 2x16, queues 2x2x4, term 4, shadow records 2x48
-    return (size_t)pool * (64 + 32 + 16 + 4 + 96) + (size_t)window * 48 + gstack_words(trace_grid) * 4 +
+    return (size_t)pool * (64 + 32 + 16 + 4 + 96) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
            ctr_ints() * 4 + 4096;
 }
 
@@ -54,7 +54,7 @@ void wave_drop_graphs(WaveEngine& e) {
 void wave_release(WaveEngine& e) {
     wave_drop_graphs(e);
     wfree(e.ray_o); wfree(e.ray_d); wfree(e.thr); wfree(e.rad); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light);
-    wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.gstack);
+    wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.gstack);
     wfree(e.ctr); wfree(e.overflow); wfree(e.trav);
     if (e.host_counts) (void)hipHostFree(e.host_counts);
     e.host_counts = nullptr;
@@ -95,10 +95,10 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
     }
     if (window > e.window) {
         wave_drop_graphs(e);
-        wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm);
+        wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep);
         e.window = 0;
         if ((r = walloc(&e.s_rad, window)) != hipSuccess || (r = walloc(&e.s_alb, window)) != hipSuccess ||
-            (r = walloc(&e.s_nrm, window)) != hipSuccess)
+            (r = walloc(&e.s_nrm, window)) != hipSuccess || (r = walloc(&e.s_dep, window)) != hipSuccess)
             return r;
         e.window = window;
     }
@@ -118,7 +118,7 @@ void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p) {
     p.ray_o = e.ray_o; p.ray_d = e.ray_d; p.thr = e.thr; p.rad = e.rad;
     p.queue[0] = e.q0; p.queue[1] = e.q1;
     p.hits = e.hits; p.sh_env = e.sh_env; p.sh_light = e.sh_light; p.term = e.term;
-    p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm;
+    p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm; p.s_dep = e.s_dep;
     p.ctr = e.ctr; p.traversal = e.trav; p.gstack = e.gstack; p.overflow = e.overflow;
 }
 

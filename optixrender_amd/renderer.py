@@ -175,6 +175,11 @@ class Context:
         self.check(self._lib.oxr_download_rgba8(self.h, abi.ptr(out), out.size), "oxr_download_rgba8")
         return out.reshape(h, w)
 
+    def download_depth(self, w: int, h: int) -> np.ndarray:
+        out = np.zeros((h, w), np.float32)
+        self.check(self._lib.oxr_download_depth(self.h, abi.ptr(out), w * h), "oxr_download_depth")
+        return out
+
     def download_aov(self, which: int, w: int, h: int) -> np.ndarray:
         out = np.zeros((h, w, 4), np.float32)
         self.check(self._lib.oxr_download_aov(self.h, which, abi.ptr(out), w * h), "oxr_download_aov")

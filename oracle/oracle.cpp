@@ -1462,7 +1462,7 @@ Cam make_camera(const orc_camera& c, int W, int H) {
 }
 struct Stats { uint64_t ext = 0, shadow = 0, nodes = 0, tris = 0; };
 
-struct PathOut { V3 radiance, albedo, normal; };
+struct PathOut { V3 radiance, albedo, normal; float depth; };
 
 // __raygen__renderFrame (devicePrograms.cu:170-532) for one (pixel, frame).
 PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, int ix, int iy, int frame,
@@ -1523,6 +1523,7 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
     is.in_medium = is.out_medium = cam.medium;
     bool first = true;
     V3 pix_albedo(0.0f), pix_normal(0.0f);
+    float pix_depth = 1.0f;
     const bool has_env = S.env_w > 0;
 
     for (int bounce = 0; bounce < P.max_depth; bounce++) {
@@ -1551,6 +1552,10 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
         }
         closest_distance = is.distance;
         bool hitLight = light_trace(ray, is.in_medium, is.out_medium, light_radiance, light_pdf, closest_distance);
+        // depth AOV (no reference counterpart; a denoiser input): first surface
+        // or light along the camera ray, as d / (1 + d), 1 for a miss
+        if (bounce == 0)
+            pix_depth = closest_distance < FLT_MAX ? closest_distance / (1.0f + closest_distance) : 1.0f;
 
         bool scattered = false;
         int medium_index = -1;
@@ -1698,6 +1703,7 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
     out.radiance = radiance;
     out.albedo = pix_albedo;
     out.normal = pix_normal;
+    out.depth = pix_depth;
     return out;
 }
 
@@ -1900,7 +1906,7 @@ void orc_tonemap(int64_t n, const float* rgba, uint32_t* out) {
 }
 
 int orc_render(const orc_scene* sc, const orc_camera* cam_in, const orc_render_params* p, float* color,
-               float* albedo, float* normal, float* path_radiance) {
+               float* albedo, float* normal, float* depth, float* path_radiance) {
     if (p->width <= 0 || p->height <= 0 || p->spp < 0) return -1;
     Scene S;
     load_scene(S, sc);
@@ -1909,7 +1915,7 @@ int orc_render(const orc_scene* sc, const orc_camera* cam_in, const orc_render_p
     parallel_rows(H, p->threads, [&](int iy, int) {
         for (int ix = 0; ix < W; ix++) {
             size_t pix = (size_t)iy * W + ix;
-            double dc[4], da[4], dn[4];
+            double dc[4], da[4], dn[4], dd = depth ? depth[pix] : 0.0;
             for (int k = 0; k < 4; k++) { dc[k] = color[4 * pix + k]; da[k] = albedo[4 * pix + k]; dn[k] = normal[4 * pix + k]; }
             for (int s = 0; s < p->spp; s++) {
                 int frame = p->frame_base + s;
@@ -1922,12 +1928,15 @@ int orc_render(const orc_scene* sc, const orc_camera* cam_in, const orc_render_p
                 float c[4] = {o.radiance.x, o.radiance.y, o.radiance.z, 1.0f};
                 float a[4] = {o.albedo.x, o.albedo.y, o.albedo.z, 1.0f};
                 float nn[4] = {o.normal.x, o.normal.y, o.normal.z, 1.0f};
+                float z = o.depth;
                 if (is_nan(o.radiance)) {
                     c[0] = c[1] = c[2] = 0.0f;
                     a[0] = a[1] = a[2] = 0.0f;
                     nn[0] = nn[1] = nn[2] = 0.0f;
+                    z = 0.0f;
                 }
                 if (p->fp64_accum) {
+                    dd = frame > 0 ? (z + (double)frame * dd) / (frame + 1.0) : z;
                     for (int k = 0; k < 4; k++) {
                         if (frame > 0) {
                             dc[k] = (c[k] + (double)frame * dc[k]) / (frame + 1.0);
@@ -1941,6 +1950,13 @@ int orc_render(const orc_scene* sc, const orc_camera* cam_in, const orc_render_p
                     float* pc = color + 4 * pix;
                     float* pa = albedo + 4 * pix;
                     float* pn = normal + 4 * pix;
+                    if (depth) {
+                        if (frame > 0) {
+                            z += (float)frame * depth[pix];
+                            z /= ((float)frame + 1.0f);
+                        }
+                        depth[pix] = z;
+                    }
                     for (int k = 0; k < 4; k++) {
                         if (frame > 0) {
                             c[k] += (float)frame * pc[k];
@@ -1954,6 +1970,7 @@ int orc_render(const orc_scene* sc, const orc_camera* cam_in, const orc_render_p
                     }
                 }
             }
+            if (p->fp64_accum && depth) depth[pix] = (float)dd;
             if (p->fp64_accum)
                 for (int k = 0; k < 4; k++) {
                     color[4 * pix + k] = (float)dc[k];

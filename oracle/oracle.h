@@ -151,7 +151,7 @@ void orc_tonemap(int64_t n, const float* rgba, uint32_t* out);
 /* color/albedo/normal: width*height float4 running means (in/out when
    frame_base > 0).  path_radiance (optional): spp*width*height*3 floats. */
 int orc_render(const orc_scene* scene, const orc_camera* cam, const orc_render_params* p,
-               float* color, float* albedo, float* normal, float* path_radiance);
+               float* color, float* albedo, float* normal, float* depth, float* path_radiance);
 /* counts of extension/shadow rays and traversal work, for measurement:
    out[0] extension rays, out[1] shadow rays, out[2] node records fetched,
    out[3] triangle tests. */

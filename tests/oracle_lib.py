@@ -69,7 +69,7 @@ def lib():
         L.orc_env_cache.argtypes = [C.c_int, C.c_int, _P, _P]
         L.orc_bilinear.argtypes = [C.c_int, C.c_int, _P, C.c_float, C.c_float, _P]
         L.orc_tonemap.argtypes = [C.c_int64, _P, _P]
-        L.orc_render.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+        L.orc_render.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_render.restype = C.c_int
         L.orc_render_stats.argtypes = [_P, _P, _P, _P]
         L.orc_render_stats.restype = C.c_int
@@ -249,17 +249,23 @@ def _scene_struct(scene: SceneData):
 
 
 def render(scene: SceneData, width, height, spp, max_depth=25, rr_start=0, filt=3, frame_base=0, threads=0,
-           fp64=False, keep_paths=False, camera=None, color=None, albedo=None, normal=None):
+           fp64=False, keep_paths=False, camera=None, color=None, albedo=None, normal=None, depth=None,
+           with_depth=False):
+    """(color, albedo, normal, paths), or (color, albedo, normal, depth, paths)
+    with with_depth; the AOVs continue from the given buffers (progressive)."""
     s, keep = _scene_struct(scene)
     prm = OrcRenderParams(width, height, spp, max_depth, rr_start, filt, frame_base, threads, int(fp64), 0)
     color = np.zeros((height, width, 4), np.float32) if color is None else color.copy()
     albedo = np.zeros((height, width, 4), np.float32) if albedo is None else albedo.copy()
     normal = np.zeros((height, width, 4), np.float32) if normal is None else normal.copy()
+    depth = np.zeros((height, width), np.float32) if depth is None else depth.copy()
     paths = np.zeros((spp, height, width, 3), np.float32) if keep_paths else None
     cam = camera or scene.camera
-    rc = lib().orc_render(C.byref(s), C.byref(cam), C.byref(prm), p(color), p(albedo), p(normal),
+    rc = lib().orc_render(C.byref(s), C.byref(cam), C.byref(prm), p(color), p(albedo), p(normal), p(depth),
                           p(paths) if paths is not None else None)
     assert rc == 0
+    if with_depth:
+        return color, albedo, normal, depth, paths
     return color, albedo, normal, paths
 
 

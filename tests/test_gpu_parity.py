@@ -229,10 +229,15 @@ def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None):
         paths = ctx.download_path_radiance(spp, w, h)
         color = ctx.download_aov(0, w, h)
         albedo = ctx.download_aov(1, w, h)
+        dz = ctx.download_depth(w, h)
     finally:
         ctx.set_tuning(base)
         ctx.release(sid)
-    rc, ra, rn, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, keep_paths=True)
+    rc, ra, rn, rd, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, keep_paths=True,
+                                           with_depth=True)
+    # first-hit depth AOV: same traversal, same distance arithmetic
+    close = np.abs(dz - rd) <= 1e-6 * np.abs(rd) + 1e-7
+    assert close.mean() >= PASS_FRACTION, f"depth AOV differs on {np.count_nonzero(~close)} pixels"
     return paths, rp, color, rc, albedo, ra
 
 
