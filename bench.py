@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
+    ap.add_argument("--tuning", default="",
+                    help="comma-separated oxr_tuning fields to set before the steps, e.g. stack_lds=16,pool_paths=4194304")
+    ap.add_argument("--autotune", action="store_true", help="run oxr_autotune on a 4-spp probe before the steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (the known-size reference for HBM counters)")
     return ap.parse_args()
@@ -138,6 +141,16 @@ def main():
     W, H, spp, depth = args.width, args.height, args.spp, args.depth
     print(f"[bench] scene {scene.n_tris} tris generated in {gen_s:.1f}s, BVH build {build_ms:.2f} ms", file=sys.stderr,
           flush=True)
+    if args.tuning:
+        t = ctx.get_tuning()
+        for kv in args.tuning.split(","):
+            k, v = kv.split("=")
+            setattr(t, k.strip(), int(v))
+        ctx.set_tuning(t)
+    if args.autotune:
+        t, ms = ctx.autotune(spp=4, max_depth=depth, rr_start=0)
+        ctx.set_camera(scene.camera)
+        print(f"[bench] autotune: {t.as_dict()} ({ms:.2f} ms per 4-spp probe)", file=sys.stderr, flush=True)
 
     def step(flags=abi.OXR_FLAG_STAGE_TIMING):
         # stage timing: kernels launched one by one with HIP events between stages

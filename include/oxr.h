@@ -115,7 +115,7 @@ typedef struct oxr_render_params {
     int32_t rr_start;     /* first bounce that plays Russian roulette (reference: 0) */
     int32_t progressive;  /* 1: continue frameID; 0: frameID restarts at 0 (Renderer.cpp:642-644) */
     int32_t filter;       /* 0 box 1 tent 2 triangle 3 gaussian (reference default 3) */
-    int32_t denoise;      /* 0 plain accumulation */
+    int32_t denoise;      /* 0 plain accumulation; 1 denoise the accumulated image before the tone map */
     uint32_t flags;       /* OXR_FLAG_* */
     int32_t batch_frames; /* frames per sample window, 0 = automatic (oxr_tuning.window_samples) */
 } oxr_render_params;
@@ -125,6 +125,7 @@ typedef struct oxr_render_stats {
     double extend_ms, shade_ms, connect_ms;
     double generate_ms;      /* window start and path regeneration */
     double accumulate_ms, tonemap_ms;
+    double denoise_ms;       /* KPCNN denoiser (denoise = 1 only) */
     uint64_t extension_rays, shadow_rays;
     uint64_t node_fetches, tri_tests;   /* all traversals; only with OXR_FLAG_COUNT_TRAVERSAL */
     uint64_t ext_node_fetches, ext_tri_tests;  /* extension rays only; same condition */
@@ -174,7 +175,8 @@ int oxr_set_camera(oxr_ctx* ctx, const oxr_camera* cam);
 int oxr_render(oxr_ctx* ctx, const oxr_render_params* params);
 int oxr_get_stats(oxr_ctx* ctx, oxr_render_stats* out);
 int oxr_download_rgba8(oxr_ctx* ctx, uint32_t* out, size_t count);
-/* which: 0 colour, 1 albedo, 2 normal; count = width*height float4. */
+/* which: 0 colour, 1 albedo, 2 normal, 3 denoised colour (after a render
+   with denoise = 1); count = width*height float4. */
 int oxr_download_aov(oxr_ctx* ctx, int which, float* out_rgba, size_t count);
 /* First-hit depth AOV, running mean of d / (1 + d) (1 for a miss): a
    denoiser input with no reference counterpart.  count = width*height. */
@@ -202,7 +204,7 @@ typedef struct oxr_server_config {
 typedef struct oxr_request {
     uint32_t scene_id;
     int32_t width, height, spp, max_depth, rr_start, filter;
-    int32_t reserved;
+    int32_t denoise;           /* 1: denoise the finished image with the context's weights (oxr_denoiser_load) */
     oxr_camera camera;
 } oxr_request;
 /* request states reported by oxr_poll */
@@ -251,6 +253,19 @@ int oxr_autotune(oxr_ctx* ctx, const oxr_render_params* probe, oxr_tuning* best,
 /* ---- building blocks exposed for parity tests ---- */
 /* ComputeFinalPixelColorsKernel (PostProcessing.cu:26-48) over n host float4. */
 int oxr_tonemap_host(oxr_ctx* ctx, const float* rgba, int64_t n, uint32_t* out);
+
+/* ---- denoiser (optixDenoiserCreate/Setup/Invoke, Renderer.cpp:660-785,
+   817-874) ----  The OptiX AI denoiser is closed; this is a kernel-predicting
+   CNN of our own: 5 layers of 3x3 convolutions 10 -> 32 -> 32 -> 32 -> 32 ->
+   25 over log(1 + colour), albedo, normal and depth, whose softmaxed outputs
+   form a 5x5 filter applied to the colour.  BF16 MFMA with FP32 accumulation.
+   weights: per layer l, w[co][ci][3][3] then b[co] as floats (37,881 in all). */
+#define OXR_DENOISER_PARAMS 37881
+int oxr_denoiser_load(oxr_ctx* ctx, const float* weights, int64_t count);
+/* Denoises host AOVs of one w x h image (float4 colour/albedo/normal, float
+   depth) with the loaded weights into out (float4). */
+int oxr_denoise_host(oxr_ctx* ctx, int w, int h, const float* color, const float* albedo, const float* normal,
+                     const float* depth, float* out);
 /* Kulla-Conty tables as uploaded (ComputeKullaConty, Microfacet.h:231-291). */
 int oxr_kulla_conty(float* bsdf_avg /* 64*64 */, float* albedo_avg /* 64 */);
 /* Batched EvaluateMaterial / SampleMaterial (Material.h:783-845) on the device.

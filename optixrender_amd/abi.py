@@ -196,6 +196,7 @@ class RenderStats(C.Structure):
         ("generate_ms", C.c_double),
         ("accumulate_ms", C.c_double),
         ("tonemap_ms", C.c_double),
+        ("denoise_ms", C.c_double),
         ("extension_rays", C.c_uint64),
         ("shadow_rays", C.c_uint64),
         ("node_fetches", C.c_uint64),
@@ -232,7 +233,7 @@ class Request(C.Structure):
         ("max_depth", C.c_int32),
         ("rr_start", C.c_int32),
         ("filter", C.c_int32),
-        ("reserved", C.c_int32),
+        ("denoise", C.c_int32),
         ("camera", Camera),
     ]
 
@@ -294,6 +295,8 @@ SIGNATURES = {
     "oxr_download_rgba8": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_download_aov": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "oxr_download_depth": (C.c_int, [_P, _P, C.c_size_t]),
+    "oxr_denoiser_load": (C.c_int, [_P, _P, C.c_int64]),
+    "oxr_denoise_host": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "oxr_download_path_radiance": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_server_create": (C.c_int, [_P, _P, C.POINTER(C.c_void_p)]),
     "oxr_server_destroy": (C.c_int, [_P]),

@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["api.cpp", "serve.cpp", "wavefront.cpp", "bvh.hip", "render.hip"]
+SOURCES = ["api.cpp", "serve.cpp", "wavefront.cpp", "bvh.hip", "render.hip", "denoise.hip"]
 STAMP = PKG / "liboxr.stamp"
 
 

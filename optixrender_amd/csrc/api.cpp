@@ -156,6 +156,63 @@ void free_scene(SceneDev* s) {
     delete s;
 }
 
+// BF16 round to nearest even (NaN kept), as the device converts activations.
+uint16_t to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Flat KPCNN weights (per layer w[co][ci][3][3], b[co]) -> MFMA lane order
+// (the layout DenoiserDev documents) and padded biases.
+void arrange_denoiser(const float* w, std::vector<uint16_t>& lanes, std::vector<float>& bias, int64_t* offset) {
+    lanes.clear();
+    bias.assign(kDnLayers * 32, 0.0f);
+    size_t pos = 0;
+    int64_t groups = 0;
+    for (int l = 0; l < kDnLayers; l++) {
+        const int ci_n = kDnChannels[l], co_n = kDnChannels[l + 1];
+        const int steps = ci_n <= 16 ? 1 : 2;
+        const float* W = w + pos;
+        pos += (size_t)co_n * ci_n * 9;
+        const float* B = w + pos;
+        pos += co_n;
+        offset[l] = groups;
+        for (int t = 0; t < 9; t++)
+            for (int s = 0; s < steps; s++)
+                for (int q = 0; q < 64; q++)
+                    for (int e = 0; e < 8; e++) {
+                        const int co = q % 32, ci = 16 * s + 8 * (q / 32) + e;
+                        const float v = (co < co_n && ci < ci_n) ? W[((size_t)(co * ci_n + ci) * 3 + t / 3) * 3 + t % 3]
+                                                                 : 0.0f;
+                        lanes.push_back(to_bf16(v));
+                    }
+        groups += 9 * steps * 64;
+        for (int co = 0; co < co_n; co++) bias[l * 32 + co] = B[co];
+    }
+}
+
+int64_t denoiser_params() {
+    int64_t n = 0;
+    for (int l = 0; l < kDnLayers; l++) n += (int64_t)kDnChannels[l + 1] * (kDnChannels[l] * 9 + 1);
+    return n;
+}
+
+// Output and scratch of the denoiser for w*h pixels.
+hipError_t ensure_denoise_buffers(oxr_ctx* c, size_t pixels) {
+    if (pixels <= c->dn_pixels) return hipSuccess;
+    dfree(c->denoised);
+    if (c->dn_scratch) (void)hipFree(c->dn_scratch);
+    c->dn_scratch = nullptr;
+    c->dn_pixels = 0;
+    hipError_t e = dalloc(&c->denoised, pixels);
+    if (e == hipSuccess) e = hipMalloc(&c->dn_scratch, pixels * 160);
+    if (e == hipSuccess) c->dn_pixels = pixels;
+    return e;
+}
+
 // Grids of the stage kernels: the tuned blocks per CU, capped at what fits
 // on a CU at once (a persistent grid must be resident in full), rounded so
 // that every shard gets a wave.
@@ -226,7 +283,9 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.window_samples = 1 << 27;
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
-    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return bail(OXR_ERR_HIP);
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
+        hipEventCreate(&c->td0) != hipSuccess || hipEventCreate(&c->td1) != hipSuccess)
+        return bail(OXR_ERR_HIP);
     *out = c;
     return OXR_OK;
 }
@@ -241,8 +300,10 @@ int oxr_destroy(oxr_ctx* c) {
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
     dfree(c->depth);
     dfree(c->path_radiance); dfree(c->reqs);
-    if (c->t0) (void)hipEventDestroy(c->t0);
-    if (c->t1) (void)hipEventDestroy(c->t1);
+    dfree(c->dn_lanes); dfree(c->dn_bias); dfree(c->denoised);
+    if (c->dn_scratch) (void)hipFree(c->dn_scratch);
+    for (hipEvent_t ev : {c->t0, c->t1, c->td0, c->td1})
+        if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OXR_OK;
@@ -508,6 +569,7 @@ int oxr_resize(oxr_ctx* c, int w, int h) {
     c->height = h;
     c->frame_id = 0;  // Resize calls SetCamera, which resets accumulation (Renderer.cpp:866, :799)
     c->path_radiance_valid = 0;
+    c->denoised_valid = false;
     return OXR_OK;
 }
 
@@ -526,7 +588,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     if (prm->spp < 1) return fail(c, OXR_ERR_INVALID, "spp must be >= 1");
     if (prm->filter < 0 || prm->filter > 3) return fail(c, OXR_ERR_INVALID, "filter out of range");
     if (prm->max_depth < 0) return fail(c, OXR_ERR_INVALID, "max_depth must be >= 0");
-    if (prm->denoise != 0) return fail(c, OXR_ERR_INVALID, "denoise needs the denoiser entry points");
+    if (prm->denoise != 0 && prm->denoise != 1) return fail(c, OXR_ERR_INVALID, "denoise must be 0 or 1");
+    if (prm->denoise && !c->dn_loaded) return fail(c, OXR_ERR_STATE, "denoise = 1 needs oxr_denoiser_load first");
     if (c->cam.medium < -1 || c->cam.medium >= s->n_media) return fail(c, OXR_ERR_INVALID, "camera medium out of range");
     OXR_HIP(c, hipSetDevice(c->device));
     if (!prm->progressive) c->frame_id = 0;  // Renderer.cpp:642-644
@@ -613,7 +676,18 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
         iterations += r.iterations;
         windows++;
     }
-    OXR_HIP(c, tonemap(c->color, c->rgba, npix, c->stream));
+    c->denoised_valid = false;
+    if (prm->denoise) {
+        OXR_HIP(c, ensure_denoise_buffers(c, (size_t)npix));
+        OXR_HIP(c, hipEventRecord(c->td0, c->stream));
+        OXR_HIP(c, denoise_run(c->dn, c->color, c->albedo, c->normal, c->depth, W, H, c->dn_scratch, c->denoised,
+                               c->stream));
+        OXR_HIP(c, hipEventRecord(c->td1, c->stream));
+        OXR_HIP(c, tonemap(c->denoised, c->rgba, npix, c->stream));
+        launches += 6;
+    } else {
+        OXR_HIP(c, tonemap(c->color, c->rgba, npix, c->stream));
+    }
     launches++;
     OXR_HIP(c, hipEventRecord(c->t1, c->stream));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
@@ -624,6 +698,12 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     float total = 0.0f;
     (void)hipEventElapsedTime(&total, c->t0, c->t1);
     st.render_ms = total;
+    if (prm->denoise) {
+        float dms = 0.0f;
+        (void)hipEventElapsedTime(&dms, c->td0, c->td1);
+        st.denoise_ms = dms;
+        c->denoised_valid = true;
+    }
     st.generate_ms = stage[0];
     st.extend_ms = stage[1];
     st.shade_ms = stage[2];
@@ -727,9 +807,10 @@ int oxr_download_rgba8(oxr_ctx* c, uint32_t* out, size_t count) {
 }
 
 int oxr_download_aov(oxr_ctx* c, int which, float* out, size_t count) {
-    if (!c || !out || which < 0 || which > 2) return OXR_ERR_INVALID;
+    if (!c || !out || which < 0 || which > 3) return OXR_ERR_INVALID;
     if (count != (size_t)c->width * c->height) return fail(c, OXR_ERR_INVALID, "count must equal width*height");
-    float4* src = which == 0 ? c->color : (which == 1 ? c->albedo : c->normal);
+    if (which == 3 && !c->denoised_valid) return fail(c, OXR_ERR_STATE, "the last render was not denoised");
+    float4* src = which == 0 ? c->color : (which == 1 ? c->albedo : (which == 2 ? c->normal : c->denoised));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     OXR_HIP(c, hipMemcpy(out, src, 16 * count, hipMemcpyDeviceToHost));
     return OXR_OK;
@@ -749,6 +830,57 @@ int oxr_download_path_radiance(oxr_ctx* c, float* out, size_t count) {
     if (count != c->path_radiance_valid) return fail(c, OXR_ERR_INVALID, "count must equal spp*width*height*3");
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     OXR_HIP(c, hipMemcpy(out, c->path_radiance, 4 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_denoiser_load(oxr_ctx* c, const float* weights, int64_t count) {
+    if (!c || !weights) return OXR_ERR_INVALID;
+    if (count != denoiser_params())
+        return fail(c, OXR_ERR_INVALID, "denoiser weights: expected " + std::to_string(denoiser_params()) + " floats");
+    for (int64_t i = 0; i < count; i++)
+        if (!std::isfinite(weights[i])) return fail(c, OXR_ERR_INVALID, "denoiser weights must be finite");
+    std::vector<uint16_t> lanes;
+    std::vector<float> bias;
+    DenoiserDev d{};
+    arrange_denoiser(weights, lanes, bias, d.lane_offset);
+    OXR_HIP(c, hipSetDevice(c->device));
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    if (!c->dn_lanes) {
+        OXR_HIP(c, dalloc(&c->dn_lanes, lanes.size()));
+        OXR_HIP(c, dalloc(&c->dn_bias, bias.size()));
+    }
+    OXR_HIP(c, hipMemcpy(c->dn_lanes, lanes.data(), lanes.size() * 2, hipMemcpyHostToDevice));
+    OXR_HIP(c, hipMemcpy(c->dn_bias, bias.data(), bias.size() * 4, hipMemcpyHostToDevice));
+    d.lanes = c->dn_lanes;
+    d.bias = c->dn_bias;
+    c->dn = d;
+    c->dn_loaded = true;
+    return OXR_OK;
+}
+
+int oxr_denoise_host(oxr_ctx* c, int w, int h, const float* color, const float* albedo, const float* normal,
+                     const float* depth, float* out) {
+    if (!c || w <= 0 || h <= 0 || (int64_t)w * h > (1LL << 30) || !color || !albedo || !normal || !depth || !out)
+        return OXR_ERR_INVALID;
+    if (!c->dn_loaded) return fail(c, OXR_ERR_STATE, "oxr_denoiser_load first");
+    const size_t n = (size_t)w * h;
+    float4 *dc = nullptr, *da = nullptr, *dn = nullptr, *dout = nullptr;
+    float* dd = nullptr;
+    void* scratch = nullptr;
+    hipError_t e;
+    if ((e = dalloc(&dc, n)) == hipSuccess && (e = dalloc(&da, n)) == hipSuccess && (e = dalloc(&dn, n)) == hipSuccess &&
+        (e = dalloc(&dd, n)) == hipSuccess && (e = dalloc(&dout, n)) == hipSuccess &&
+        (e = hipMalloc(&scratch, denoise_scratch_bytes(w, h))) == hipSuccess &&
+        (e = hipMemcpy(dc, color, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = hipMemcpy(da, albedo, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = hipMemcpy(dn, normal, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = hipMemcpy(dd, depth, 4 * n, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = denoise_run(c->dn, dc, da, dn, dd, w, h, scratch, dout, c->stream)) == hipSuccess &&
+        (e = hipStreamSynchronize(c->stream)) == hipSuccess)
+        e = hipMemcpy(out, dout, 16 * n, hipMemcpyDeviceToHost);
+    dfree(dc); dfree(da); dfree(dn); dfree(dd); dfree(dout);
+    if (scratch) (void)hipFree(scratch);
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("denoise: ") + hipGetErrorString(e));
     return OXR_OK;
 }
 

@@ -127,10 +127,11 @@ struct WaveParams {
     // path pool: `pool` slots, shard k owns slots [k*region, (k+1)*region)
     int64_t pool;
     int region;        // pool / kShards
-    float4* ray_o;     // origin, pre_pdf
-    float4* ray_d;     // direction, rng state bits
-    float4* thr;       // history (throughput), bounce (int bits)
-    float4* rad;       // radiance, sample id (uint bits; kNoSample when the slot holds none)
+    // one 64-byte record per slot (regeneration scatters slots over the
+    // queues, so a slot's fields share a cache line):
+    //   [0] origin, pre_pdf   [1] direction, rng state bits
+    //   [2] throughput, bounce (int bits)   [3] radiance, sample id (uint bits; kNoSample: none)
+    float4* path;
     uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
     float4* hits;      // per queue position (2*kShards regions)
     float4* sh_env;    // kShards regions of `region` records, 3 float4 each
@@ -166,6 +167,22 @@ hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hi
 // s_rad) and starts new samples in their slots, appended to queue x.
 hipError_t wave_regen(const WaveParams& p, int x, int grid, hipStream_t s);
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
+// ---- denoiser (denoise.hip) ----
+// KPCNN 10 -> 32 -> 32 -> 32 -> 32 -> 25, 3x3 convolutions.  lanes: BF16
+// weights in MFMA lane order, layer l at lane_offset[l] (in 8-element groups):
+// for tap t, K step s, lane q, element e: W[co = q % 32][ci = 16 s + 8 (q / 32)
+// + e][t / 3][t % 3], zero where co or ci is padding.  bias: 5 x 32 floats.
+constexpr int kDnLayers = 5;
+constexpr int kDnChannels[kDnLayers + 1] = {10, 32, 32, 32, 32, 25};
+struct DenoiserDev {
+    const uint16_t* lanes;
+    int64_t lane_offset[kDnLayers];
+    const float* bias;
+};
+size_t denoise_scratch_bytes(int w, int h);
+hipError_t denoise_run(const DenoiserDev& d, const float4* color, const float4* albedo, const float4* normal,
+                       const float* depth, int w, int h, void* scratch, float4* out, hipStream_t s);
+
 // Largest resident workgroups per CU of the extend / connect instance for
 // stack_lds, and of shade / regen.
 int wave_trace_occupancy(int stack_lds);

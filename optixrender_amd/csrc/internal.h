@@ -44,7 +44,7 @@ void fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, 
 struct WaveEngine {
     int64_t pool = 0, window = 0;  // allocated slots / samples
     int grid = 0;                  // trace grid the stack spill is sized for
-    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *rad = nullptr;
+    float4* path = nullptr;  // 4 float4 per slot
     float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
     float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
     float* s_dep = nullptr;
@@ -110,6 +110,15 @@ struct oxr_ctx {
     size_t path_radiance_floats = 0;
     size_t path_radiance_valid = 0;
     oxr_render_stats stats;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr, td0 = nullptr, td1 = nullptr;
     oxr::ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
+    // denoiser: weights (oxr_denoiser_load), output and scratch of the last denoised render
+    uint16_t* dn_lanes = nullptr;
+    float* dn_bias = nullptr;
+    oxr::DenoiserDev dn{};
+    bool dn_loaded = false;
+    float4* denoised = nullptr;
+    void* dn_scratch = nullptr;
+    size_t dn_pixels = 0;  // capacity of denoised / dn_scratch
+    bool denoised_valid = false;
 };

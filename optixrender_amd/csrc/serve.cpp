@@ -42,14 +42,17 @@ struct Request {
     State state = kPending;
     int frames_done = 0;
     size_t bytes = 0;
-    float4 *color = nullptr, *albedo = nullptr, *normal = nullptr;
+    float4 *color = nullptr, *albedo = nullptr, *normal = nullptr, *denoised = nullptr;
     float* depth = nullptr;
     uint32_t* rgba = nullptr;
     Clock::time_point submitted, finished;
     std::string error;
 };
 
-size_t request_bytes(const oxr_request& r) { return (size_t)r.width * r.height * (3 * 16 + 4 + 4); }
+// colour, albedo, normal (float4), depth (float), RGBA8, and the denoised colour
+size_t request_bytes(const oxr_request& r) {
+    return (size_t)r.width * r.height * (3 * 16 + 4 + 4 + (r.denoise ? 16 : 0));
+}
 
 }  // namespace
 
@@ -61,6 +64,7 @@ struct oxr_server {
     int64_t tick_paths = 0, cap = 0, pool = 0;
     int trace_grid = 0, shade_grid = 0, stack_lds = 32, iters_per_poll = 4;
     WaveEngine eng;  // shared path pool; a tick is one sample window
+    void* dn_scratch = nullptr;  // denoiser activations for one frame of max_frame_pixels
     ReqDesc* reqs_dev = nullptr;
     ReqDesc* reqs_host = nullptr;  // pinned staging
     int max_reqs = 0;
@@ -95,7 +99,7 @@ void sfree(T*& p) {
 }
 
 void free_request_buffers(oxr_server* s, Request* q) {
-    sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba); sfree(q->depth);
+    sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba); sfree(q->depth); sfree(q->denoised);
     if (q->bytes) s->in_use -= q->bytes;
     q->bytes = 0;
 }
@@ -117,6 +121,7 @@ void admit(oxr_server* s) {
         if ((e = salloc(&q->color, np)) != hipSuccess || (e = salloc(&q->albedo, np)) != hipSuccess ||
             (e = salloc(&q->normal, np)) != hipSuccess || (e = salloc(&q->rgba, np)) != hipSuccess ||
             (e = salloc(&q->depth, np)) != hipSuccess ||
+            (q->r.denoise && (e = salloc(&q->denoised, np)) != hipSuccess) ||
             (e = hipMemsetAsync(q->depth, 0, 4 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->color, 0, 16 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->albedo, 0, 16 * np, s->stream)) != hipSuccess ||
@@ -186,8 +191,16 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     if ((e = wave_run_window(s->eng, p, run, s->stream)) != hipSuccess) return e;
     for (size_t i = 0; i < group.size(); i++) {
         Request* q = group[i];
-        if (q->frames_done + frames[i] >= q->r.spp)
-            if ((e = tonemap(q->color, q->rgba, (int64_t)q->r.width * q->r.height, s->stream)) != hipSuccess) return e;
+        if (q->frames_done + frames[i] < q->r.spp) continue;
+        const int64_t np = (int64_t)q->r.width * q->r.height;
+        if (q->r.denoise) {
+            if ((e = denoise_run(c->dn, q->color, q->albedo, q->normal, q->depth, q->r.width, q->r.height,
+                                 s->dn_scratch, q->denoised, s->stream)) != hipSuccess ||
+                (e = tonemap(q->denoised, q->rgba, np, s->stream)) != hipSuccess)
+                return e;
+        } else if ((e = tonemap(q->color, q->rgba, np, s->stream)) != hipSuccess) {
+            return e;
+        }
     }
     s->paths += (uint64_t)path;
     return hipStreamSynchronize(s->stream);
@@ -282,8 +295,9 @@ extern "C" {
 int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out) {
     if (!c || !cfg || !out) return OXR_ERR_INVALID;
     *out = nullptr;
-    if (cfg->max_active < 1 || cfg->tick_paths < 1 || cfg->max_active > 256) {
-        c->err = "server: max_active in [1, 256] and tick_paths >= 1 required";
+    if (cfg->max_active < 1 || cfg->tick_paths < 1 || cfg->max_active > 256 || cfg->max_frame_pixels < 1 ||
+        cfg->max_frame_pixels > (1LL << 30)) {
+        c->err = "server: max_active in [1, 256], tick_paths >= 1 and max_frame_pixels in [1, 2^30] required";
         return OXR_ERR_INVALID;
     }
     if (hipSetDevice(c->device) != hipSuccess) return OXR_ERR_HIP;
@@ -299,7 +313,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->stack_lds = c->tuning.stack_lds;
     s->iters_per_poll = c->tuning.iters_per_poll;
     s->pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, s->cap));
-    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid);
+    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid) + denoise_scratch_bytes((int)cfg->max_frame_pixels, 1);
     auto bail = [&](int code, const char* m) {
         c->err = m;
         oxr_server_destroy(s);
@@ -308,7 +322,8 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     if (s->wave_bytes >= s->budget) return bail(OXR_ERR_INVALID, "server: budget smaller than the wavefront buffers");
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP, "stream");
     if (s->cap > ((int64_t)1 << 31) - 1) return bail(OXR_ERR_INVALID, "server: tick larger than 2^31 samples");
-    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs))
+    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs) ||
+        hipMalloc(&s->dn_scratch, std::max<size_t>(denoise_scratch_bytes((int)cfg->max_frame_pixels, 1), 256)))
         return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
     if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
         return bail(OXR_ERR_OOM, "server: pinned staging");
@@ -334,6 +349,7 @@ int oxr_server_destroy(oxr_server* s) {
     }
     wave_release(s->eng);
     sfree(s->reqs_dev);
+    if (s->dn_scratch) (void)hipFree(s->dn_scratch);
     if (s->reqs_host) (void)hipHostFree(s->reqs_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -348,6 +364,11 @@ int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id) {
     auto it = s->ctx->scenes.find(r->scene_id);
     if (it == s->ctx->scenes.end() || !it->second->built) {
         s->err = "submit: unknown or unbuilt scene";
+        s->rejected++;
+        return OXR_ERR_INVALID;
+    }
+    if (r->denoise != 0 && (r->denoise != 1 || !s->ctx->dn_loaded)) {
+        s->err = "submit: denoise needs 0 or 1 and the context's denoiser weights (oxr_denoiser_load)";
         s->rejected++;
         return OXR_ERR_INVALID;
     }
@@ -421,7 +442,7 @@ static int copy_result(oxr_server* s, uint64_t id, void* out, size_t bytes_per_p
     Request* q = it->second;
     if (q->state != kDone) return OXR_ERR_STATE;
     if (count != (size_t)q->r.width * q->r.height) return OXR_ERR_INVALID;
-    const void* src = which == 0 ? (const void*)q->rgba : (const void*)q->color;
+    const void* src = which == 0 ? (const void*)q->rgba : (q->r.denoise ? (const void*)q->denoised : (const void*)q->color);
     if (hipMemcpy(out, src, bytes_per_px * count, hipMemcpyDeviceToHost) != hipSuccess) return OXR_ERR_HIP;
     return OXR_OK;
 }

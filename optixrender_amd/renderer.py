@@ -145,8 +145,8 @@ class Context:
         self.check(self._lib.oxr_set_camera(self.h, C.byref(cam)), "oxr_set_camera")
 
     def render(self, spp=1, max_depth=25, rr_start=0, progressive=False, filter=abi.FILTER_GAUSSIAN, flags=0,
-               batch_frames=0):
-        p = abi.RenderParams(spp, max_depth, rr_start, int(progressive), filter, 0, flags, batch_frames)
+               batch_frames=0, denoise=False):
+        p = abi.RenderParams(spp, max_depth, rr_start, int(progressive), filter, int(denoise), flags, batch_frames)
         self.check(self._lib.oxr_render(self.h, C.byref(p)), "oxr_render")
         return self.stats()
 
@@ -174,6 +174,20 @@ class Context:
         out = np.zeros(w * h, np.uint32)
         self.check(self._lib.oxr_download_rgba8(self.h, abi.ptr(out), out.size), "oxr_download_rgba8")
         return out.reshape(h, w)
+
+    def load_denoiser(self, flat_weights: np.ndarray):
+        """oxr_denoiser_load: per layer w[co][ci][3][3] then b[co] (denoise.flat_weights)."""
+        w = np.ascontiguousarray(flat_weights, np.float32)
+        self.check(self._lib.oxr_denoiser_load(self.h, abi.ptr(w), w.size), "oxr_denoiser_load")
+
+    def denoise(self, color, albedo, normal, depth) -> np.ndarray:
+        """oxr_denoise_host on one image: (h, w, 4) AOVs and (h, w) depth -> (h, w, 4)."""
+        h, w = depth.shape
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal, depth)]
+        out = np.zeros((h, w, 4), np.float32)
+        self.check(self._lib.oxr_denoise_host(self.h, w, h, *[abi.ptr(a) for a in arrs], abi.ptr(out)),
+                   "oxr_denoise_host")
+        return out
 
     def download_depth(self, w: int, h: int) -> np.ndarray:
         out = np.zeros((h, w), np.float32)
@@ -294,8 +308,9 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, scene_id, width, height, spp, camera, max_depth=16, rr_start=0, filter=abi.FILTER_GAUSSIAN):
-        r = abi.Request(scene_id, width, height, spp, max_depth, rr_start, filter, 0, camera)
+    def submit(self, scene_id, width, height, spp, camera, max_depth=16, rr_start=0, filter=abi.FILTER_GAUSSIAN,
+               denoise=False):
+        r = abi.Request(scene_id, width, height, spp, max_depth, rr_start, filter, int(denoise), camera)
         rid = C.c_uint64()
         self.check(self._lib.oxr_submit(self.h, C.byref(r), C.byref(rid)), "oxr_submit")
         self.sizes[rid.value] = (width, height)

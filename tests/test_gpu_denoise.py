@@ -1,0 +1,116 @@
+"""The KPCNN denoiser on the GPU (BF16 MFMA) against its float32 torch
+reference with the same BF16 rounding of weights and activations, and the
+invariants of a kernel-predicting network."""
+import numpy as np
+import pytest
+
+from optixrender_amd import abi, denoise, scenes
+
+pytestmark = pytest.mark.gpu
+
+W, H = 72, 40   # 3 tiles of 32 pixels per row, the last one partial
+
+
+def random_layers(seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    layers = []
+    for ci, co in zip(denoise.CHANNELS[:-1], denoise.CHANNELS[1:]):
+        std = scale * np.sqrt(2.0 / (9 * ci))
+        layers.append((rng.normal(0, std, (co, ci, 3, 3)).astype(np.float32),
+                       rng.normal(0, 0.05, co).astype(np.float32)))
+    return layers
+
+
+@pytest.fixture(scope="module")
+def aovs(ctx):
+    sc = scenes.cornell_spheres()
+    sid = ctx.upload(sc)
+    ctx.build_bvh(sid)
+    ctx.select(sid)
+    ctx.resize(W, H)
+    ctx.set_camera(sc.camera)
+    ctx.render(spp=4, max_depth=8, rr_start=3)
+    out = (ctx.download_aov(0, W, H), ctx.download_aov(1, W, H), ctx.download_aov(2, W, H), ctx.download_depth(W, H))
+    yield sc, sid, out
+    ctx.release(sid)
+
+
+def test_parameter_count_matches_the_abi():
+    assert denoise.parameter_count() == 37881
+    assert denoise.flat_weights(random_layers()).size == 37881
+
+
+@pytest.mark.parametrize("seed,scale", [(0, 1.0), (1, 2.0)])
+def test_denoiser_matches_bf16_reference(ctx, aovs, seed, scale):
+    _, _, (c, a, n, d) = aovs
+    layers = random_layers(seed, scale)
+    ctx.load_denoiser(denoise.flat_weights(layers))
+    print(f"[denoise] seed={seed} scale={scale} {W}x{H}", flush=True)
+    got = ctx.denoise(c, a, n, d)
+    ref = denoise.reference_denoise(layers, c, a, n, d, bf16=True)
+    # accumulation order differs (MFMA vs torch) and a BF16 activation can
+    # round the other way: 1e-2 relative, and the mean far tighter
+    err = np.abs(got[..., :3] - ref) / (np.abs(ref) + 1e-3)
+    assert np.mean(err <= 1e-2) >= 0.999, f"max rel err {err.max()}"
+    assert err.mean() < 1e-3
+    assert np.all(got[..., 3] == 1.0)
+
+
+def test_constant_image_is_a_fixed_point(ctx, aovs):
+    # softmax weights sum to one, so any weights keep a constant colour
+    _, _, (c, a, n, d) = aovs
+    ctx.load_denoiser(denoise.flat_weights(random_layers(3, 1.5)))
+    const = np.zeros_like(c)
+    const[..., :3] = (0.25, 1.5, 7.0)
+    got = ctx.denoise(const, a, n, d)
+    assert np.allclose(got[..., :3], const[..., :3], rtol=1e-5, atol=0)
+
+
+def test_render_with_denoise_equals_host_denoise(ctx, aovs):
+    sc, sid, _ = aovs
+    ctx.load_denoiser(denoise.flat_weights(random_layers(4)))
+    ctx.select(sid)
+    ctx.resize(W, H)
+    ctx.set_camera(sc.camera)
+    st = ctx.render(spp=3, max_depth=8, rr_start=3, denoise=True)
+    assert st.denoise_ms > 0
+    dn = ctx.download_aov(3, W, H)
+    c, a, n, d = (ctx.download_aov(0, W, H), ctx.download_aov(1, W, H), ctx.download_aov(2, W, H),
+                  ctx.download_depth(W, H))
+    assert np.array_equal(dn.view(np.uint32), ctx.denoise(c, a, n, d).view(np.uint32))
+    assert np.array_equal(ctx.download_rgba8(W, H).reshape(-1), ctx.tonemap(dn.reshape(-1, 4)))
+    # a plain render leaves no denoised image behind
+    ctx.render(spp=1, max_depth=8, rr_start=3)
+    with pytest.raises(abi.OxrError):
+        ctx.download_aov(3, W, H)
+
+
+def test_full_hd_constant_and_finite(ctx):
+    # one 1920x1080 frame: every tile, the partial edge handling, and no NaN
+    w, h = 1920, 1080
+    rng = np.random.default_rng(5)
+    ctx.load_denoiser(denoise.flat_weights(random_layers(5)))
+    col = np.ones((h, w, 4), np.float32)
+    col[..., :3] = rng.uniform(0.0, 4.0, (h, w, 3))
+    alb = np.ones((h, w, 4), np.float32)
+    nrm = np.ones((h, w, 4), np.float32)
+    dep = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    out = ctx.denoise(col, alb, nrm, dep)
+    assert np.isfinite(out).all()
+    # a convex combination of the 5x5 neighbourhood
+    assert out[..., :3].min() >= 0.0 and out[..., :3].max() <= 4.0 * (1 + 1e-6)
+
+
+def test_trained_weights_beat_plain_accumulation(ctx, aovs):
+    if not denoise.WEIGHTS.exists():
+        pytest.skip("no trained weights in the package")
+    sc, sid, _ = aovs
+    ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
+    ctx.select(sid)
+    ctx.resize(W, H)
+    ctx.set_camera(sc.camera)
+    ctx.render(spp=256, max_depth=8, rr_start=3)
+    ref = ctx.download_aov(0, W, H)
+    ctx.render(spp=4, max_depth=8, rr_start=3, denoise=True)
+    noisy, dn = ctx.download_aov(0, W, H), ctx.download_aov(3, W, H)
+    assert denoise.log_rmse(dn, ref) < denoise.log_rmse(noisy, ref)
