@@ -277,9 +277,9 @@ int oxr_create(int device, oxr_ctx** out) {
         return bail(OXR_ERR_HIP);
     c->tuning.trace_blocks_per_cu = 8;
     c->tuning.shade_blocks_per_cu = 8;
-    c->tuning.stack_lds = 32;
+    c->tuning.stack_lds = 16;
     c->tuning.iters_per_poll = 4;
-    c->tuning.pool_paths = 1 << 23;
+    c->tuning.pool_paths = 1 << 24;
     c->tuning.window_samples = 1 << 27;
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);

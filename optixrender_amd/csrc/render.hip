@@ -237,10 +237,29 @@ OXR_D void start_path(const WaveParams& p, uint32_t pid, int64_t g) {
     float sx = ((float)ix + 0.5f + jit.x) / (float)q.width;
     float sy = ((float)iy + 0.5f + jit.y) / (float)q.height;
     f3 dir = normalize(make3(q.cam_dir) + (sx - 0.5f) * make3(q.cam_h) + (sy - 0.5f) * make3(q.cam_v));
-    p.ray_o[pid] = make_float4(q.cam_pos[0], q.cam_pos[1], q.cam_pos[2], 0.0f);
-    p.ray_d[pid] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
-    p.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
-    p.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)g));
+    p.path[4 * (size_t)pid + 0] = make_float4(q.cam_pos[0], q.cam_pos[1], q.cam_pos[2], 0.0f);
+    p.path[4 * (size_t)pid + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
+    p.path[4 * (size_t)pid + 2] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
+    p.path[4 * (size_t)pid + 3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)g));
+}
+
+// Draws one new sample of shard k for every lane that asks (consecutive
+// positions of the shard's sequence, one atomic per wave), starts it in the
+// lane's slot and appends the slot to the new-path region of queue y.  All
+// lanes of the wave must call it.  Returns whether this lane started a path.
+OXR_D bool restart(const WaveParams& p, int k, int y, bool want, uint32_t pid) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return false;
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr_at(p.ctr, kCtrSamplePos + k), (int)__popcll(m));
+    base = __shfl(base, 0);
+    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
+    const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
+    const bool start = want && g < total;
+    if (start) start_path(p, pid, g);
+    const int o = wave_append(ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + kShards + k), start);
+    if (o >= 0) p.queue[y][(size_t)(kShards + k) * p.region + o] = pid;
+    return start;
 }
 
 __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame0, int offset, int frames,
@@ -262,7 +281,7 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
     }
     if (t < p.pool) {
         p.term[t] = (uint32_t)t;
-        p.rad[t] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoSample));
+        p.path[4 * t + 3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoSample));
     }
 }
 
@@ -274,12 +293,8 @@ __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
     const int n = *ctr_at(p.ctr, kCtrTerm + k);
     const int chunks = (n + 63) >> 6;
     if (chunks == 0) return;
-    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
     int* fetch = ctr_at(p.ctr, kCtrFetchRegen + k);
-    int* drawn = ctr_at(p.ctr, kCtrSamplePos + k);
-    int* qn_count = ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + kShards + k);
     const uint32_t* term = p.term + (size_t)k * p.region;
-    uint32_t* qn = p.queue[y] + (size_t)(kShards + k) * p.region;
     while (true) {
         const int j = wave_next(fetch);
         if (j >= chunks) break;
@@ -288,19 +303,11 @@ __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
         uint32_t pid = 0;
         if (have) {
             pid = term[slot];
-            float4 r = p.rad[pid];
+            float4 r = p.path[4 * (size_t)pid + 3];
             uint32_t g = __float_as_uint(r.w);
             if (g != kNoSample) p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
         }
-        const uint64_t m = __ballot(have);
-        int base = 0;
-        if (lane_id() == 0) base = atomicAdd(drawn, (int)__popcll(m));
-        base = __shfl(base, 0);
-        const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
-        const bool start = have && g < total;
-        if (start) start_path(p, pid, g);
-        const int o = wave_append(qn_count, start);
-        if (o >= 0) qn[o] = pid;
+        restart(p, k, y, have, pid);
     }
 }
 
@@ -342,7 +349,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int x) {
         const int64_t qp = sq.position(j, k, p.region);
         if (qp >= 0) {
             uint32_t pid = q[qp];
-            float4 o = p.ray_o[pid], d = p.ray_d[pid];
+            float4 o = p.path[4 * (size_t)pid + 0], d = p.path[4 * (size_t)pid + 1];
             Hit h = traverse<false, COUNT>(p.bvh, xyz(o), xyz(d), kEps, kFltMax, st, nn, nt, overflow);
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
         }
@@ -383,11 +390,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which
             if (h.leaf == kNoHit) {
                 float4 c = rec[3 * (size_t)slot + 2];
                 uint32_t pid = __float_as_uint(b.w);
-                float4 r = p.rad[pid];
+                float4 r = p.path[4 * (size_t)pid + 3];
                 r.x = r.x + c.x;
                 r.y = r.y + c.y;
                 r.z = r.z + c.z;
-                p.rad[pid] = r;
+                p.path[4 * (size_t)pid + 3] = r;
             }
         }
     }
@@ -425,16 +432,16 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 // continues; NEE shadow rays are returned for the connect stage with their
 // contribution precomputed (it is added only if the ray is unoccluded).
 OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, ShadowOut& sh_env,
-                      ShadowOut& sh_light) {
+                      ShadowOut& sh_light, float4& rad_out) {
     sh_env.emit = false;
     sh_light.emit = false;
-    float4 ro = p.ray_o[pid], rd = p.ray_d[pid], th = p.thr[pid];
+    float4 ro = p.path[4 * (size_t)pid + 0], rd = p.path[4 * (size_t)pid + 1], th = p.path[4 * (size_t)pid + 2];
     f3 org = xyz(ro), dir = xyz(rd);
     float pre_pdf = ro.w;
     Rng rng{__float_as_uint(rd.w)};
     f3 history = xyz(th);
     const int bounce = __float_as_int(th.w);
-    float4 rad4 = p.rad[pid];
+    float4 rad4 = p.path[4 * (size_t)pid + 3];
     f3 radiance = xyz(rad4);
     const uint32_t g = __float_as_uint(rad4.w);
     f3 V = -dir, L = dir;
@@ -631,12 +638,13 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         }
         if (alive && any_nan(history)) alive = false;                       // devicePrograms.cu:487-489
     }
-    p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, rad4.w);
+    rad_out = make_float4(radiance.x, radiance.y, radiance.z, rad4.w);
+    p.path[4 * (size_t)pid + 3] = rad_out;
     const bool cont = alive && bounce + 1 < p.max_depth;
     if (cont) {
-        p.ray_o[pid] = make_float4(pos.x, pos.y, pos.z, pre_pdf);
-        p.ray_d[pid] = make_float4(L.x, L.y, L.z, __uint_as_float(rng.s));
-        p.thr[pid] = make_float4(history.x, history.y, history.z, __int_as_float(bounce + 1));
+        p.path[4 * (size_t)pid + 0] = make_float4(pos.x, pos.y, pos.z, pre_pdf);
+        p.path[4 * (size_t)pid + 1] = make_float4(L.x, L.y, L.z, __uint_as_float(rng.s));
+        p.path[4 * (size_t)pid + 2] = make_float4(history.x, history.y, history.z, __int_as_float(bounce + 1));
     }
     return cont;
 }
@@ -650,7 +658,7 @@ OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) 
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
+__global__ __launch_bounds__(256, 3) void k_shade(WaveParams p, int x) {
     const int k = wave_shard();
     const ShardQueue sq(p.ctr, x, k);
     if (sq.chunks == 0) return;
@@ -675,16 +683,24 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         bool cont = false;
         ShadowOut sev, slv;
         sev.emit = slv.emit = false;
+        float4 rad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (active) {
             pid = q[qp];
-            cont = shade_path(p, lut, pid, p.hits[qp], sev, slv);
+            cont = shade_path(p, lut, pid, p.hits[qp], sev, slv, rad);
         }
+        // A path that ended with no shadow ray in flight is final now: its
+        // radiance goes to its sample and the slot starts the next sample at
+        // once.  One that still waits for a connect is finished by k_regen.
+        const bool ended = active && !cont;
+        const bool final = ended && !sev.emit && !slv.emit;
+        if (final) p.s_rad[__float_as_uint(rad.w)] = make_float4(rad.x, rad.y, rad.z, 0.0f);
         int o = wave_append(qo_count, cont);
         if (o >= 0) qo[o] = pid;
-        o = wave_append(term_count, active && !cont);
+        o = wave_append(term_count, ended && !final);
         if (o >= 0) term[o] = pid;
         emit_shadow(se, se_count, sev, pid);
         emit_shadow(sl, sl_count, slv, pid);
+        restart(p, k, x ^ 1, final, pid);
     }
 }
 

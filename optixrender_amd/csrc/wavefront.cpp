@@ -53,7 +53,7 @@ void wave_drop_graphs(WaveEngine& e) {
 
 void wave_release(WaveEngine& e) {
     wave_drop_graphs(e);
-    wfree(e.ray_o); wfree(e.ray_d); wfree(e.thr); wfree(e.rad); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light);
+    wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light);
     wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.gstack);
     wfree(e.ctr); wfree(e.overflow); wfree(e.trav);
     if (e.host_counts) (void)hipHostFree(e.host_counts);
@@ -82,11 +82,10 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
     }
     if (pool > e.pool) {
         wave_drop_graphs(e);
-        wfree(e.ray_o); wfree(e.ray_d); wfree(e.thr); wfree(e.rad); wfree(e.hits); wfree(e.sh_env);
+        wfree(e.path); wfree(e.hits); wfree(e.sh_env);
         wfree(e.sh_light); wfree(e.q0); wfree(e.q1); wfree(e.term);
         e.pool = 0;
-        if ((r = walloc(&e.ray_o, pool)) != hipSuccess || (r = walloc(&e.ray_d, pool)) != hipSuccess ||
-            (r = walloc(&e.thr, pool)) != hipSuccess || (r = walloc(&e.rad, pool)) != hipSuccess ||
+        if ((r = walloc(&e.path, 4 * pool)) != hipSuccess ||
             (r = walloc(&e.hits, 2 * pool)) != hipSuccess || (r = walloc(&e.sh_env, 3 * pool)) != hipSuccess ||
             (r = walloc(&e.sh_light, 3 * pool)) != hipSuccess || (r = walloc(&e.q0, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.q1, 2 * pool)) != hipSuccess || (r = walloc(&e.term, pool)) != hipSuccess)
@@ -115,7 +114,7 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
 void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p) {
     p.pool = pool;
     p.region = (int)(pool / kShards);
-    p.ray_o = e.ray_o; p.ray_d = e.ray_d; p.thr = e.thr; p.rad = e.rad;
+    p.path = e.path;
     p.queue[0] = e.q0; p.queue[1] = e.q1;
     p.hits = e.hits; p.sh_env = e.sh_env; p.sh_light = e.sh_light; p.term = e.term;
     p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm; p.s_dep = e.s_dep;

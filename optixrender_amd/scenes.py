@@ -16,6 +16,8 @@ PCG64 seeded per scene.
   heightfield_medium (1920x1080, 32 spp, depth 32): fBm heightfield with
       interleaved per-triangle materials and a 4x4x4 homogeneous-medium box
       (sigma_t 0.5, albedo 0.9, HG g 0.3) around a dielectric sphere.
+  sphere_field_5m: 61 icospheres at subdivision 6 (4,997,120 triangles), a
+      resident scene of the serving benchmark.
 """
 from __future__ import annotations
 
@@ -180,7 +182,104 @@ def sphere_field(seed: int = 1, subdiv: int = 6, count: int = 12, env_w: int = 2
     return SceneData(pos, nrm, mid, mats, [], [], env, cam)
 
 
+def fbm(n: int, octaves: int = 6, seed: int = 1, base: int = 4) -> np.ndarray:
+    """(n, n) fractional Brownian motion of smoothstep value noise in about
+    [-1, 1]: octave o has a (base 2^o + 1)^2 lattice and amplitude 2^-o."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, n, dtype=np.float64)
+    h = np.zeros((n, n), np.float32)
+    amp, freq, norm = 1.0, base, 0.0
+    for _ in range(octaves):
+        lat = rng.uniform(-1.0, 1.0, (freq + 1, freq + 1)).astype(np.float32)
+        x = u * freq
+        i = np.minimum(x.astype(np.int64), freq - 1)
+        f = (x - i).astype(np.float32)
+        f = f * f * (3.0 - 2.0 * f)
+        a = lat[i][:, i] * (1 - f)[None, :] + lat[i][:, i + 1] * f[None, :]
+        b = lat[i + 1][:, i] * (1 - f)[None, :] + lat[i + 1][:, i + 1] * f[None, :]
+        h += amp * (a * (1 - f)[:, None] + b * f[:, None])
+        norm += amp
+        amp *= 0.5
+        freq *= 2
+    return h / norm
+
+
+def box(lo, hi) -> np.ndarray:
+    """12 outward-facing triangles of an axis-aligned box."""
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    faces = [
+        quad((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1)),  # -y
+        quad((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0)),  # +y
+        quad((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0)),  # -z
+        quad((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)),  # +z
+        quad((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)),  # -x
+        quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)),  # +x
+    ]
+    return np.concatenate(faces)
+
+
+def heightfield_medium(verts: int = 3163, octaves: int = 6, seed: int = 1, extent: float = 20.0,
+                       height: float = 3.0, sphere_subdiv: int = 5, env_w: int = 2048,
+                       env_h: int = 1024) -> SceneData:
+    """fBm heightfield (2 (verts-1)^2 triangles: 19,996,488 at 3163 vertices a
+    side) with four materials interleaved per triangle, a 4x4x4 box of
+    homogeneous medium (sigma_s 0.45, sigma_a 0.05, HG g 0.3) resting on it,
+    bounded by an index-matched thin dielectric (the reference has no
+    pass-through interface; ThinDielectric at IOR 1/1 transmits straight on),
+    and a GGX dielectric sphere (IOR 1.5) inside the box, under the HDR sky."""
+    n = verts
+    h = fbm(n, octaves, seed) * height
+    g = np.linspace(-extent, extent, n, dtype=np.float32)
+    X, Z = np.meshgrid(g, g)                       # X varies along columns, Z along rows
+    P = np.stack([X, h, Z], -1).reshape(-1, 3)
+    dx = 2.0 * extent / (n - 1)
+    gz, gx = np.gradient(h, dx)
+    N = np.stack([-gx, np.ones_like(h), -gz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    N = N.reshape(-1, 3).astype(np.float32)
+    r = np.arange(n - 1)
+    v00 = (r[:, None] * n + r[None, :]).reshape(-1)
+    v01, v10, v11 = v00 + 1, v00 + n, v00 + n + 1
+    idx = np.stack([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)], 1).reshape(-1, 3)
+    del v00, v01, v10, v11
+    terrain = P[idx].astype(np.float32)
+    tnorm = N[idx]
+    del idx, P, N
+    tmat = (np.arange(len(terrain)) % 4).astype(np.uint32)
+    # medium box resting on the terrain at the centre, sphere inside it
+    c = n // 2
+    y0 = float(h[c, c]) + 0.25
+    bx = box((-2.0, y0, -2.0), (2.0, y0 + 4.0, 2.0))
+    sph = icosphere(sphere_subdiv) * 1.2 + np.array([0.0, y0 + 2.0, 0.0])
+    pos = np.concatenate([terrain, bx.astype(np.float32), sph.astype(np.float32)])
+    nrm = np.concatenate([tnorm, flat_normals(bx).astype(np.float32),
+                          (icosphere(sphere_subdiv)).astype(np.float32)])
+    mid = np.concatenate([tmat, np.full(len(bx), 4, np.uint32), np.full(len(sph), 5, np.uint32)])
+    del terrain, tnorm
+    mats = [
+        abi.material(abi.DIFFUSE, albedo=(0.45, 0.40, 0.33), roughness=0.5),
+        abi.material(abi.DIFFUSE, albedo=(0.20, 0.45, 0.15), roughness=0.3),
+        abi.material(abi.PLASTIC, albedo=(0.55, 0.50, 0.45), roughness=0.4, int_ior=1.5, ext_ior=1.0),
+        abi.material(abi.CONDUCTOR, roughness=0.35),
+        abi.material(abi.THIN_DIELECTRIC, albedo=(1.0, 1.0, 1.0), roughness=0.05, int_ior=1.0, ext_ior=1.0),
+        abi.material(abi.DIELECTRIC, roughness=0.1, int_ior=1.5, ext_ior=1.0),
+    ]
+    mats[4].in_medium, mats[4].out_medium = 0, -1
+    mats[5].in_medium, mats[5].out_medium = -1, 0
+    med = abi.medium(phase_type=1, g=(0.3, 0.3, 0.3), sigma_s=(0.45, 0.45, 0.45), sigma_a=(0.05, 0.05, 0.05),
+                     scale=1.0)
+    env = sky_env(env_w, env_h)
+    cam = abi.camera((7.0, y0 + 5.5, 12.0), (0.0, y0 + 2.0, 0.0))
+    return SceneData(pos, nrm, mid, mats, [], [med], env, cam)
+
+
+def sphere_field_5m(seed: int = 7) -> SceneData:
+    """Random-sphere field of 61 icospheres at subdivision 6: 4,997,120 triangles."""
+    return sphere_field(seed=seed, subdiv=6, count=61)
+
+
 CONFIGS = {
     "cornell_spheres": dict(width=256, height=256, spp=16, max_depth=8, rr_start=3),
     "sphere_field": dict(width=1920, height=1080, spp=64, max_depth=16, rr_start=0),
+    "heightfield_medium": dict(width=1920, height=1080, spp=32, max_depth=32, rr_start=0),
 }
