@@ -236,6 +236,7 @@ void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_b
     p.normals = s->normals_sorted;
     p.mat_ids = s->mat_sorted;
     p.mats = s->mats;
+    p.class_mask = s->class_mask;
     p.lights = s->lights;
     p.n_lights = s->n_lights;
     p.media = s->media;
@@ -376,6 +377,8 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
     const int64_t n = d->n_tris;
     s->n = n;
     s->n_mats = d->n_materials;
+    s->class_mask = 1u;  // rays that leave the scene
+    for (int i = 0; i < d->n_materials; i++) s->class_mask |= 1u << (1 + d->materials[i].type);
     s->n_lights = d->n_lights;
     s->n_media = d->n_media;
     s->env_w = d->env_w;

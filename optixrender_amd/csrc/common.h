@@ -102,9 +102,12 @@ enum CtrId {
     kCtrFetchRegen = kCtrFetchConnLight + kShards,
     kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
     kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
-    kCtrIds = kCtrWindowTotal + 1
+    kCtrClass = kCtrWindowTotal + 1,           // [class][shard]: entries of the shade-class queues
+    kCtrFetchClass = kCtrClass + 9 * kShards,  // [class][shard]: chunks taken by the shade kernels
+    kCtrIds = kCtrFetchClass + 9 * kShards
 };
 constexpr size_t ctr_ints() { return (size_t)kCtrIds * kCtrStride; }
+constexpr int kShadeClasses = 9;
 
 // Wavefront integrator state for one context.
 struct WaveParams {
@@ -137,6 +140,10 @@ struct WaveParams {
     float4* sh_env;    // kShards regions of `region` records, 3 float4 each
     float4* sh_light;
     uint32_t* term;    // kShards regions of `region` slot ids
+    // shade classes: 0 = the ray left the scene, 1 + t = it hit a surface of
+    // material type t.  cls_queue[class][shard] holds `region` queue positions.
+    uint32_t* cls_queue;
+    uint32_t class_mask;  // classes the scene can produce (bit c): the shade launches
     // per-sample outputs of the window
     float4* s_rad;
     float4* s_alb;
@@ -161,6 +168,7 @@ hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int 
 // One iteration at queue parity x: extend -> shade -> connect(env) ->
 // connect(lights) -> regen.  Each stage is its own launch.
 hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStream_t s);
+// One launch per shade class present in p.class_mask.
 hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s);
 hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
 // Finishes the paths that ended in the last shade (writes their radiance to

@@ -20,6 +20,7 @@ struct SceneDev {
     uint32_t* mat_sorted = nullptr;
     oxr_material* mats = nullptr;
     int n_mats = 0;
+    uint32_t class_mask = 1;  // shade classes: bit 0 rays that leave the scene, bit 1 + t material type t
     oxr_light* lights = nullptr;
     int n_lights = 0;
     oxr_medium* media = nullptr;
@@ -48,7 +49,7 @@ struct WaveEngine {
     float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
     float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
     float* s_dep = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr, *cls_queue = nullptr;
     int* ctr = nullptr;
     int* overflow = nullptr;
     unsigned long long* trav = nullptr;

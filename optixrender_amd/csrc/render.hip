@@ -288,7 +288,10 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
 // Finish the paths that ended in the last shade, then start new samples in
 // their slots (queue y, new-path region of the shard).
 __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
-    if (blockIdx.x == 0 && threadIdx.x < kShards) *ctr_at(p.ctr, kCtrFetchExtend + threadIdx.x) = 0;
+    if (blockIdx.x == 0) {  // counters of the next extend, idle now
+        if (threadIdx.x < kShards) *ctr_at(p.ctr, kCtrFetchExtend + threadIdx.x) = 0;
+        for (int i = threadIdx.x; i < kShadeClasses * kShards; i += blockDim.x) *ctr_at(p.ctr, kCtrClass + i) = 0;
+    }
     const int k = wave_shard();
     const int n = *ctr_at(p.ctr, kCtrTerm + k);
     const int chunks = (n + 63) >> 6;
@@ -329,6 +332,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int x) {
             *ctr_at(p.ctr, kCtrFetchConnLight + t) = 0;
             *ctr_at(p.ctr, kCtrFetchRegen + t) = 0;
         }
+        for (int i = t; i < kShadeClasses * kShards; i += blockDim.x) *ctr_at(p.ctr, kCtrFetchClass + i) = 0;
         if (t == 0) {
             unsigned long long rays = 0;
             for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, kCtrQueue + x * 2 * kShards + r);
@@ -347,11 +351,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int x) {
         const int j = wave_next(fetch);
         if (j >= sq.chunks) break;
         const int64_t qp = sq.position(j, k, p.region);
+        int cls = -1;
         if (qp >= 0) {
             uint32_t pid = q[qp];
             float4 o = p.path[4 * (size_t)pid + 0], d = p.path[4 * (size_t)pid + 1];
             Hit h = traverse<false, COUNT>(p.bvh, xyz(o), xyz(d), kEps, kFltMax, st, nn, nt, overflow);
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
+            cls = h.leaf == kNoHit ? 0 : 1 + p.mats[p.mat_ids[h.leaf]].type;
+        }
+        // list the queue position under its shade class: one append per
+        // class present in the wave
+        uint64_t left = __ballot(cls >= 0);
+        while (left) {
+            const int c = __shfl(cls, (int)__builtin_ctzll(left));
+            const bool mine = cls == c;
+            const int o = wave_append(ctr_at(p.ctr, kCtrClass + c * kShards + k), mine);
+            if (mine) p.cls_queue[((size_t)c * kShards + k) * p.region + o] = (uint32_t)qp;
+            left &= ~__ballot(mine);
         }
     }
     if (COUNT) {
@@ -431,6 +447,10 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 // for one path, after its radiance ray was traced.  Returns whether the path
 // continues; NEE shadow rays are returned for the connect stage with their
 // contribution precomputed (it is added only if the ray is unoccluded).
+// MT: the material type of the hit surface, fixed by the shade class; < 0 for
+// rays that left the scene.  MEDIA: the scene has participating media (the
+// medium code is compiled only then).
+template <int MT, bool MEDIA>
 OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, ShadowOut& sh_env,
                       ShadowOut& sh_light, float4& rad_out) {
     sh_env.emit = false;
@@ -448,7 +468,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     float mult_trans_pdf = 1.0f;
 
     // __closesthit__radiance / __miss__radiance (devicePrograms.cu:71-159)
-    uint32_t leaf = __float_as_uint(hit.y);
+    const uint32_t leaf = MT < 0 ? kNoHit : __float_as_uint(hit.y);
     Surf s;
     s.m = nullptr;
     s.front = true;
@@ -494,7 +514,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     bool scattered = false;
     int medium_index = -1;
     bool alive = true;
-    if (p.n_media != 0) {                                                   // devicePrograms.cu:276-338
+    if (MEDIA && p.n_media != 0) {                                          // devicePrograms.cu:276-338
         medium_index = (hitLight || s.front) ? mi_out : mi_in;
         if (medium_index != -1) {
             const oxr_medium* md = p.media + medium_index;
@@ -563,6 +583,10 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
                 radiance = radiance + mw * history * light_radiance;
             }
             alive = false;
+        } else if constexpr (MT < 0) {
+            // unreachable: a ray that left the scene has closest == FLT_MAX
+            // unless it hit a light, and both cases are handled above
+            alive = false;
         } else {
             if (p.env_w > 0) {                                              // devicePrograms.cu:376-424
                 float x0 = rng(), x1 = rng();
@@ -577,7 +601,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
                 float cd = kFltMax;
                 if (!light_trace(p, pos, sd, tm, tv, tp, cd)) {
                     float bpdf = 0.0f;
-                    f3 f = eval_material(s, lut, V, sd, bpdf);
+                    f3 f = eval_material_t<MT>(s, lut, V, sd, bpdf);
                     float ct = fabsf(dot(s.n, sd));
                     float mw = power_heuristic(lp, bpdf);
                     f3 thh(1.0f);
@@ -606,7 +630,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
                 float cd = ld - kEps;
                 if (!light_trace(p, pos, sd, tm, tv, tp, cd)) {
                     float bpdf = 0.0f;
-                    f3 f = eval_material(s, lut, V, sd, bpdf);
+                    f3 f = eval_material_t<MT>(s, lut, V, sd, bpdf);
                     float ct = fabsf(dot(s.n, sd));
                     float mw = power_heuristic(light_pdf, bpdf);
                     f3 thh(1.0f);
@@ -624,7 +648,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
                 }
             }
             float bpdf = 0.0f;                                              // devicePrograms.cu:474-477
-            f3 f = sample_material(s, lut, rng, V, L, bpdf);
+            f3 f = sample_material_t<MT>(s, lut, rng, V, L, bpdf);
             float ct = fabsf(dot(s.n, L));
             history = history * (f * ct / f3(bpdf));
             pre_pdf = bpdf;
@@ -658,11 +682,18 @@ OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) 
     }
 }
 
-__global__ __launch_bounds__(256, 3) void k_shade(WaveParams p, int x) {
+// One shade class: the paths of queue x whose extension ray left the scene
+// (CLS 0) or hit a surface of material type CLS - 1, listed by k_extend in
+// cls_queue.  Each class compiles only its own BSDF, so the common case (a
+// ray that left the scene) runs at full occupancy.
+template <int CLS, bool MEDIA>
+__global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
     const int k = wave_shard();
-    const ShardQueue sq(p.ctr, x, k);
-    if (sq.chunks == 0) return;
-    int* fetch = ctr_at(p.ctr, kCtrFetchShade + k);
+    const int n = *ctr_at(p.ctr, kCtrClass + CLS * kShards + k);
+    const int chunks = (n + 63) >> 6;
+    if (chunks == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchClass + CLS * kShards + k);
+    const uint32_t* cq = p.cls_queue + ((size_t)CLS * kShards + k) * p.region;
     const uint32_t* q = p.queue[x];
     const size_t rbase = (size_t)k * p.region;
     uint32_t* qo = p.queue[x ^ 1] + rbase;  // survivors region of shard k
@@ -676,17 +707,18 @@ __global__ __launch_bounds__(256, 3) void k_shade(WaveParams p, int x) {
     Lut lut{p.lut_bsdf, p.lut_albedo};
     while (true) {
         const int j = wave_next(fetch);
-        if (j >= sq.chunks) break;
-        const int64_t qp = sq.position(j, k, p.region);
-        const bool active = qp >= 0;
+        if (j >= chunks) break;
+        const int slot = (j << 6) + lane_id();
+        const bool active = slot < n;
         uint32_t pid = 0;
         bool cont = false;
         ShadowOut sev, slv;
         sev.emit = slv.emit = false;
         float4 rad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (active) {
+            const uint32_t qp = cq[slot];
             pid = q[qp];
-            cont = shade_path(p, lut, pid, p.hits[qp], sev, slv, rad);
+            cont = shade_path<CLS - 1, MEDIA>(p, lut, pid, p.hits[qp], sev, slv, rad);
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
@@ -831,8 +863,23 @@ hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStre
     return hipGetLastError();
 }
 
+template <bool MEDIA>
+void launch_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
+    const uint32_t m = p.class_mask;
+    if (m & 1u) k_shade<0, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 2u) k_shade<1, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 4u) k_shade<2, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 8u) k_shade<3, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 16u) k_shade<4, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 32u) k_shade<5, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 64u) k_shade<6, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 128u) k_shade<7, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 256u) k_shade<8, MEDIA><<<grid, 256, 0, s>>>(p, x);
+}
+
 hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
-    k_shade<<<grid, 256, 0, s>>>(p, x);
+    if (p.n_media != 0) launch_shade<true>(p, x, grid, s);
+    else launch_shade<false>(p, x, grid, s);
     return hipGetLastError();
 }
 
@@ -857,7 +904,10 @@ int wave_trace_occupancy(int stack_lds) {
     return std::min(occupancy(k_extend<false, 32>, kTraceBlock), occupancy(k_connect<false, 32>, kTraceBlock));
 }
 
-int wave_shade_occupancy() { return std::min(occupancy(k_shade, 256), occupancy(k_regen, 256)); }
+int wave_shade_occupancy() {
+    // the grid serves every class kernel; each is capped by what fits anyway
+    return std::max({occupancy(k_shade<0, false>, 256), occupancy(k_shade<0, true>, 256), occupancy(k_regen, 256)});
+}
 
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s) {
     k_accumulate<<<blocks(npix, 256), 256, 0, s>>>(p, npix);

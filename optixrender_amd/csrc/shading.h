@@ -535,6 +535,36 @@ OXR_D f3 sample_material(const Surf& s, const Lut& lut, Rng& rng, f3 V, f3& L, f
     return f3(0.0f);
 }
 
+// The same dispatch fixed at compile time, for a shade kernel that only sees
+// material type MT.  MT < 0 is the class of rays that left the scene, which
+// never evaluate a BSDF: the calls compile to nothing.
+template <int MT>
+OXR_D f3 eval_material_t(const Surf& s, const Lut& lut, f3 V, f3 L, float& pdf) {
+    pdf = 0.0f;
+    if constexpr (MT == 0) return eval_diffuse(s, V, L, pdf);
+    else if constexpr (MT == 1) return eval_conductor(s, lut, V, L, pdf);
+    else if constexpr (MT == 2) return eval_dielectric(s, V, L, pdf);
+    else if constexpr (MT == 3) return eval_plastic(s, lut, V, L, pdf);
+    else if constexpr (MT == 4) return eval_metal(s, V, L, pdf);
+    else if constexpr (MT == 5) return eval_thin(s, V, L, pdf);
+    else if constexpr (MT == 6) return eval_clearcoat(s, lut, V, L, pdf);
+    else if constexpr (MT == 7) return eval_transmitter(s, V, L, pdf);
+    else return f3(0.0f);
+}
+template <int MT>
+OXR_D f3 sample_material_t(const Surf& s, const Lut& lut, Rng& rng, f3 V, f3& L, float& pdf) {
+    pdf = 0.0f;
+    if constexpr (MT == 0) return sample_diffuse(s, rng, V, L, pdf);
+    else if constexpr (MT == 1) return sample_conductor(s, lut, rng, V, L, pdf);
+    else if constexpr (MT == 2) return sample_dielectric(s, rng, V, L, pdf);
+    else if constexpr (MT == 3) return sample_plastic(s, lut, rng, V, L, pdf);
+    else if constexpr (MT == 4) return sample_metal(s, rng, V, L, pdf);
+    else if constexpr (MT == 5) return sample_thin(s, rng, V, L, pdf);
+    else if constexpr (MT == 6) return sample_clearcoat(s, lut, rng, V, L, pdf);
+    else if constexpr (MT == 7) return sample_transmitter(s, rng, V, L, pdf);
+    else return f3(0.0f);
+}
+
 // ---------------------------------------------------------------- media
 OXR_D float hg_channel(float g, float c) {
     float t = 1.0f + g * g + 2.0f * g * c;
