@@ -101,18 +101,24 @@ def copy_calibration():
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_k_extend_sphere_field_1080p_64spp.md"
 
 
-def measured_traffic(workload: str):
+def tuning_key(t) -> str:
+    return json.dumps(t.as_dict(), sort_keys=True)
+
+
+def measured_traffic(workload: str, tuning: str):
     """HBM bytes per k_extend launch from the committed counter summary, if
-    it was measured on this workload (a line `traffic_bytes_per_launch: N`
-    under a line `workload: ...`), else None."""
+    it was measured on this workload with this tuning (lines `workload: ...`,
+    `tuning: ...`, `traffic_bytes_per_launch: N`), else None."""
     if not TRAFFIC_FILE.exists():
         return None
     fields = {}
     for line in TRAFFIC_FILE.read_text().splitlines():
-        if ":" in line and not line.startswith(("|", "#", " ")):
+        if ":" in line and not line.startswith(("|", "#", " ", "`")):
             k, v = line.split(":", 1)
-            fields[k.strip().strip("`")] = v.strip().strip("`")
-    if fields.get("workload") != workload or "traffic_bytes_per_launch" not in fields:
+            fields[k.strip()] = v.strip()
+    if fields.get("workload") != workload or fields.get("tuning") != tuning:
+        return None
+    if "traffic_bytes_per_launch" not in fields:
         return None
     return int(float(fields["traffic_bytes_per_launch"]))
 
@@ -227,7 +233,8 @@ def main():
         both = (ext_bytes + conn_bytes) / ((float(np.mean(ext_ms)) + float(np.mean(conn_ms))) / 1e3) / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(out["config"]["workload"]),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(out["config"]["workload"], tuning_key(ctx.get_tuning())),
             "kernel": "k_extend (all launches of one step)",
             "launches_per_step": int(st.iterations),
             "bytes_per_step": int(ext_bytes),
