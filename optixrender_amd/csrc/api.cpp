@@ -230,6 +230,19 @@ SceneDev* selected(oxr_ctx* c) {
 
 }  // namespace
 
+// HdrTexture::CalculateHdrCache's luminance total (Scene.cpp:217-313): a
+// single float accumulator over the texels in row-major order, Luminance as
+// in Utils.h:76-78 with the same operation order as the device's k_env_lum.
+float oxr::env_luminance_sum(const float* rgba, int64_t texels) {
+    float s = 0.0f;
+    for (int64_t i = 0; i < texels; i++) {
+        const float* t = rgba + 4 * i;
+        const float lum = 0.299f * t[0] + 0.587f * t[1] + 0.114f * t[2];
+        s += lum;
+    }
+    return s;
+}
+
 void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, const float* lut_albedo) {
     std::memset(&p, 0, sizeof(p));
     p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
@@ -383,6 +396,7 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
     s->n_media = d->n_media;
     s->env_w = d->env_w;
     s->env_h = d->env_h;
+    if (d->env_w > 0) s->env_lum_sum = env_luminance_sum(d->env_rgba, (int64_t)d->env_w * d->env_h);
     const size_t env_px = (size_t)d->env_w * d->env_h;
     if ((e = dalloc(&s->positions, 9 * n)) == hipSuccess && (e = dalloc(&s->normals, 9 * n)) == hipSuccess &&
         (e = dalloc(&s->mat_ids, n)) == hipSuccess && (e = dalloc(&s->mats, d->n_materials)) == hipSuccess &&
@@ -450,7 +464,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         es = hipMemcpy(&bad, bvh_error_word(scratch, n), sizeof(bad), hipMemcpyDeviceToHost);
     // the environment cache reuses the scratch only after the BVH error word was read
     if (e == hipSuccess && es == hipSuccess && s->env_w > 0) {
-        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_cache, (float*)scratch, c->stream);
+        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_lum_sum, s->env_cache, (float*)scratch, c->stream);
         if (e == hipSuccess) es = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(scratch);

@@ -54,7 +54,11 @@ hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void*
 hipError_t trace_rays(const BvhView& b, const float* d_rays, int64_t n, int any, float4* d_hits, uint8_t* d_occ,
                       uint32_t* d_counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s);
 inline size_t gstack_words(int grid) { return (size_t)(kStackTotal - kStackLdsMin) * grid * kTraceBlock; }
-hipError_t env_cache_build(const float4* d_env, int w, int h, float4* d_cache, float* d_scratch, hipStream_t s);
+// lum_sum: the sequential float sum of the texel luminances in row-major
+// order (the reference's host loop), computed by env_luminance_sum.
+hipError_t env_cache_build(const float4* d_env, int w, int h, float lum_sum, float4* d_cache, float* d_scratch,
+                           hipStream_t s);
+float env_luminance_sum(const float* rgba, int64_t texels);
 size_t env_scratch_floats(int w, int h);
 hipError_t tonemap(const float4* d_in, uint32_t* d_out, int64_t n, hipStream_t s);
 hipError_t material_batch(const oxr_material* d_mat, const float* d_lut_bsdf, const float* d_lut_albedo, int mode,

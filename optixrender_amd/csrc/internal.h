@@ -28,6 +28,7 @@ struct SceneDev {
     float4* env = nullptr;
     float4* env_cache = nullptr;
     int env_w = 0, env_h = 0;
+    float env_lum_sum = 0.0f;  // sequential luminance sum of the environment (host, at upload)
     BvhBuffers bvh;
     bool built = false;
     float build_ms = 0.0f;

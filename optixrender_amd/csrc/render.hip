@@ -74,23 +74,10 @@ __global__ void k_env_lum(const float4* __restrict__ env, int64_t m, float* lum)
     float4 t = env[i];
     lum[i] = 0.299f * t.x + 0.587f * t.y + 0.114f * t.z;  // Utils.h:76-78
 }
-__global__ void k_env_sum(const float* __restrict__ lum, int64_t m, float* out) {
-    if (threadIdx.x != 0) return;
-    float s = 0.0f;
-    int64_t i = 0;
-    for (; i + 8 <= m; i += 8) {
-        float v0 = lum[i], v1 = lum[i + 1], v2 = lum[i + 2], v3 = lum[i + 3];
-        float v4 = lum[i + 4], v5 = lum[i + 5], v6 = lum[i + 6], v7 = lum[i + 7];
-        s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
-    }
-    for (; i < m; i++) s += lum[i];
-    *out = s;
-}
-__global__ void k_env_columns(const float* __restrict__ lum, const float* __restrict__ sum, int w, int h, float* pdf,
-                              float* marg, float* cdfy) {
+__global__ void k_env_columns(const float* __restrict__ lum, float total, int w, int h, float* pdf, float* marg,
+                              float* cdfy) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= w) return;
-    float total = *sum;
     float m = 0.0f;
     for (int i = 0; i < h; i++) {
         float p = lum[(size_t)i * w + j] / total;
@@ -795,17 +782,16 @@ hipError_t trace_rays(const BvhView& b, const float* rays, int64_t n, int any, f
 
 size_t env_scratch_floats(int w, int h) { return (size_t)3 * w * h + 2 * (size_t)w + 16; }
 
-hipError_t env_cache_build(const float4* env, int w, int h, float4* cache, float* scratch, hipStream_t s) {
+hipError_t env_cache_build(const float4* env, int w, int h, float lum_sum, float4* cache, float* scratch,
+                           hipStream_t s) {
     int64_t m = (int64_t)w * h;
     float* lum = scratch;
     float* pdf = lum + m;
     float* cdfy = pdf + m;
     float* marg = cdfy + m;
     float* cdfx = marg + w;
-    float* sum = cdfx + w;
     k_env_lum<<<blocks(m, 256), 256, 0, s>>>(env, m, lum);
-    k_env_sum<<<1, 64, 0, s>>>(lum, m, sum);
-    k_env_columns<<<blocks(w, 64), 64, 0, s>>>(lum, sum, w, h, pdf, marg, cdfy);
+    k_env_columns<<<blocks(w, 64), 64, 0, s>>>(lum, lum_sum, w, h, pdf, marg, cdfy);
     k_env_cdfx<<<1, 64, 0, s>>>(marg, w, cdfx);
     k_env_search<<<blocks(m, 256), 256, 0, s>>>(pdf, cdfx, cdfy, w, h, cache);
     return hipGetLastError();
