@@ -5,9 +5,15 @@
 // (Renderer.cpp:584-627).
 #include "Renderer.h"
 
+#include <unistd.h>
+
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <mutex>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 namespace oxr_cpp {
 oxr_ctx* default_context() {
@@ -133,9 +139,31 @@ void Renderer::Render() {
     p.rr_start = launchParams.rrStart;
     p.progressive = progressive ? 1 : 0;
     p.filter = (int32_t)launchParams.filterType;
-    p.denoise = 0;  // the denoiser is a separate stage; plain accumulation here
+    if (denoiserOn && !denoiserLoaded) {
+        const char* env = std::getenv("OXR_DENOISER_WEIGHTS");
+        std::string path = env ? env : "";
+        if (path.empty()) {  // next to the executable
+            char exe[4096];
+            ssize_t n = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+            std::string dir = n > 0 ? std::string(exe, (size_t)n) : std::string("./");
+            path = dir.substr(0, dir.find_last_of('/') + 1) + "denoiser_kpcnn.f32";
+        }
+        LoadDenoiser(path);
+    }
+    p.denoise = denoiserOn ? 1 : 0;  // Renderer.cpp:660-785 (our KPCNN in place of the OptiX model)
     OXR_CHECK(oxr_render(ctx, &p));
     launchParams.frame.frameID = LastStats().frame_id;
+}
+
+void Renderer::LoadDenoiser(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open denoiser weights " + path);
+    std::vector<float> w(OXR_DENOISER_PARAMS);
+    f.read(reinterpret_cast<char*>(w.data()), (std::streamsize)(w.size() * sizeof(float)));
+    if (f.gcount() != (std::streamsize)(w.size() * sizeof(float)))
+        throw std::runtime_error("denoiser weights " + path + ": expected " + std::to_string(w.size()) + " floats");
+    OXR_CHECK(oxr_denoiser_load(oxr_cpp::default_context(), w.data(), (int64_t)w.size()));
+    denoiserLoaded = true;
 }
 
 void Renderer::SetCamera(const Camera& camera) {

@@ -38,6 +38,11 @@ public:
     float BvhBuildMs() const { return buildMs; }
 
     bool denoiserOn = false;
+    // Loads the KPCNN weights (OXR_DENOISER_PARAMS raw floats, the layout
+    // oxr_denoiser_load takes).  Render() calls it on the first denoised
+    // frame with $OXR_DENOISER_WEIGHTS, or denoiser_kpcnn.f32 next to the
+    // executable.
+    void LoadDenoiser(const std::string& path);
     bool progressive = false;
     LaunchParams launchParams;
 
@@ -47,4 +52,5 @@ protected:
     float buildMs = 0.0f;
     Camera lastSetCamera;
     bool cameraSet = false;
+    bool denoiserLoaded = false;
 };

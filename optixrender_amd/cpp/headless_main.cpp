@@ -4,7 +4,9 @@
 // (DownloadPixels) as a binary PPM.
 //
 //   headless [settings.json] [out.ppm] [frames] [obj...]
-// Without OBJ arguments the scene is a procedural Cornell box.
+// Without OBJ arguments the scene is a procedural Cornell box.  With
+// OXR_HEADLESS_DENOISE=1 in the environment the frames are denoised
+// (denoiserOn, the viewer's 'D' key).
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -74,6 +76,8 @@ int main(int ac, char** av) {
         renderer.Resize(vec2i(parser.width, parser.height));
         renderer.SetCamera(camera);
         renderer.progressive = true;
+        const char* dn = std::getenv("OXR_HEADLESS_DENOISE");
+        renderer.denoiserOn = dn && std::string(dn) == "1";
         renderer.launchParams.filterType = parser.filterType;
         std::vector<uint32_t> pixels((size_t)parser.width * parser.height);
         for (int f = 0; f < frames; f++) renderer.Render();

@@ -47,8 +47,13 @@ def flops_per_pixel() -> int:
     return conv + 2 * KERNEL * KERNEL * 3
 
 
+RAW_WEIGHTS = WEIGHTS.with_suffix(".f32")
+
+
 def save_weights(layers, path: Path = WEIGHTS, meta: dict | None = None):
-    """layers: list of (w[co, ci, 3, 3], b[co]) float arrays."""
+    """layers: list of (w[co, ci, 3, 3], b[co]) float arrays.  Writes the
+    .npz and, beside it, the flat float32 file (.f32) the C++ facade loads."""
+    path = Path(path)
     arrs = {}
     for i, (w, b) in enumerate(layers):
         arrs[f"w{i}"] = np.asarray(w, np.float32)
@@ -56,6 +61,7 @@ def save_weights(layers, path: Path = WEIGHTS, meta: dict | None = None):
     for k, v in (meta or {}).items():
         arrs[f"meta_{k}"] = np.asarray(v)
     np.savez(path, **arrs)
+    flat_weights(layers).astype("<f4").tofile(path.with_suffix(".f32"))
 
 
 def load_weights(path: Path = WEIGHTS):

@@ -2,9 +2,9 @@
 
 Data: seeded scenes from the generators (sphere fields under the HDR sky seen
 from random cameras, Cornell boxes with random sphere materials), each
-rendered by the HIP library twice -- a noisy input at `--spp-in` samples
-(colour, albedo, normal, depth AOVs) and a converged target at `--spp-ref`
-samples.  Training: PyTorch on the GPU, float32, L1 loss on log(1 + colour)
+rendered by the HIP library twice -- a noisy input at one of `--spp-in`
+samples (colour, albedo, normal, depth AOVs) and a converged target at
+`--spp-ref` samples.  Training: PyTorch on the GPU, float32, L1 loss on log(1 + colour)
 over random crops, Adam.  Evaluation: held-out scenes, RMSE of log(1 + colour)
 of the plain input, of the denoised input, and of plain renders at more
 samples, to find the sample count plain accumulation needs to match the
@@ -80,15 +80,17 @@ def render_aovs(ctx: Context, sid: int, sc: SceneData, w: int, h: int, spp: int,
 
 
 def dataset(ctx, seeds, w, h, spp_in, spp_ref, depth):
-    """Per scene: features (10, h, w), noisy colour (3, h, w), target (3, h, w)."""
+    """Per scene: features (10, h, w), noisy colour (3, h, w), target (3, h, w).
+    spp_in: the input sample counts, used in turn over the scenes."""
     out = []
     for i, seed in enumerate(seeds):
+        spp = spp_in[i % len(spp_in)]
         rng = np.random.default_rng(seed)
         sc = make_scene(seed, rng)
         sid = ctx.upload(sc)
         try:
             ctx.build_bvh(sid)
-            c, a, n, d = render_aovs(ctx, sid, sc, w, h, spp_in, depth)
+            c, a, n, d = render_aovs(ctx, sid, sc, w, h, spp, depth)
             ref, _, _, _ = render_aovs(ctx, sid, sc, w, h, spp_ref, depth)
         finally:
             ctx.release(sid)
@@ -107,15 +109,16 @@ def lrmse(a, b):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(denoise.WEIGHTS))
-    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--scenes", type=int, default=128)
     ap.add_argument("--eval-scenes", type=int, default=6)
     ap.add_argument("--res", type=int, default=256)
-    ap.add_argument("--spp-in", type=int, default=4)
-    ap.add_argument("--spp-ref", type=int, default=1024)
+    ap.add_argument("--spp-in", default="2,4,8", help="input sample counts, used in turn over the training scenes")
+    ap.add_argument("--eval-spp", type=int, default=4)
+    ap.add_argument("--spp-ref", type=int, default=4096)
     ap.add_argument("--depth", type=int, default=16)
-    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--crop", type=int, default=96)
+    ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
@@ -130,8 +133,9 @@ def main(argv=None):
     ctx = Context(0)
     train_seeds = list(range(args.scenes))
     eval_seeds = list(range(1000, 1000 + args.eval_scenes))
-    train = dataset(ctx, train_seeds, args.res, args.res, args.spp_in, args.spp_ref, args.depth)
-    held = dataset(ctx, eval_seeds, args.res, args.res, args.spp_in, args.spp_ref, args.depth)
+    spp_in = [int(x) for x in args.spp_in.split(",")]
+    train = dataset(ctx, train_seeds, args.res, args.res, spp_in, args.spp_ref, args.depth)
+    held = dataset(ctx, eval_seeds, args.res, args.res, [args.eval_spp], args.spp_ref, args.depth)
     log(f"data ready in {time.perf_counter() - t0:.1f}s")
 
     feats = torch.from_numpy(np.stack([f for f, _, _, _ in train])).to(dev)
@@ -159,7 +163,7 @@ def main(argv=None):
         loss.backward()
         opt.step()
         sched.step()
-        if (it + 1) % 250 == 0 or it == 0:
+        if (it + 1) % 1000 == 0 or it == 0:
             log(f"step {it + 1}/{args.iters} loss {loss.item():.5f} ({time.perf_counter() - t1:.1f}s)")
 
     layers = denoise.torch_layers(model)
@@ -170,7 +174,8 @@ def main(argv=None):
         ev["plain"].append(lrmse(c, ref))
         ev["denoised"].append(lrmse(dn, ref))
     meta = {
-        "train_scenes": args.scenes, "eval_scenes": args.eval_scenes, "res": args.res, "spp_in": args.spp_in,
+        "train_scenes": args.scenes, "eval_scenes": args.eval_scenes, "res": args.res, "spp_in": spp_in,
+        "eval_spp": args.eval_spp,
         "spp_ref": args.spp_ref, "iters": args.iters, "batch": args.batch, "crop": args.crop,
         "eval_log_rmse_plain": float(np.mean(ev["plain"])), "eval_log_rmse_denoised": float(np.mean(ev["denoised"])),
         "seconds": round(time.perf_counter() - t0, 1),

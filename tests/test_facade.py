@@ -42,3 +42,16 @@ def test_missing_obj_is_reported(tmp_path):
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "FATAL ERROR" in r.stdout
+
+
+@pytest.mark.gpu
+def test_headless_denoised_frames(tmp_path):
+    # denoiserOn through the facade: weights from denoiser_kpcnn.f32 next to the binary
+    import os
+    out = tmp_path / "dn.ppm"
+    env = dict(os.environ, OXR_HEADLESS_DENOISE="1")
+    r = subprocess.run([str(HEADLESS), str(ROOT / "tests" / "data" / "settings.json"), str(out), "2"],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    img = read_ppm(out)
+    assert img.shape == (48, 64, 3) and img.mean() > 10

@@ -4,7 +4,7 @@ with; admission respects the HBM budget; impossible requests are refused."""
 import numpy as np
 import pytest
 
-from optixrender_amd import abi, scenes
+from optixrender_amd import abi, denoise, scenes
 from optixrender_amd.renderer import Server
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +93,24 @@ def test_cancel(ctx, cornell):
         assert srv.stats().cancelled == 1
     finally:
         srv.close()
+
+
+def test_denoised_request_equals_denoised_render(ctx, cornell):
+    sc, sid = cornell
+    ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
+    w, h, spp, cam = REQS[1]
+    srv = Server(ctx, budget_bytes=1 << 30, max_active=4, tick_paths=2000, max_frame_pixels=64 * 64)
+    try:
+        a = srv.submit(sid, w, h, spp, abi.camera(*cam), max_depth=8, denoise=True)
+        b = srv.submit(sid, *REQS[0][:3], abi.camera(*REQS[0][3]), max_depth=8)   # shares the ticks, plain
+        assert srv.wait(a, 60000) == 0 and srv.wait(b, 60000) == 0
+        col, px = srv.result_color(a), srv.result_rgba8(a)
+    finally:
+        srv.close()
+    ctx.select(sid)
+    ctx.resize(w, h)
+    ctx.set_camera(abi.camera(*cam))
+    ctx.render(spp=spp, max_depth=8, rr_start=0, denoise=True)
+    assert np.array_equal(col.view(np.uint32), ctx.download_aov(3, w, h).view(np.uint32))
+    assert np.array_equal(px, ctx.download_rgba8(w, h))
+
