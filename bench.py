@@ -28,7 +28,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES = 32, 36, 48   # 32-B node record, 36-B triangle, 32-B ray in + 16-B hit out
+# 112-B wide node (four slot boxes + refs), 36-B triangle, 32-B ray in + 16-B hit out
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 112, 36, 48
 
 
 def parse():
@@ -42,6 +43,9 @@ def parse():
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg (p50/p99 latency at equal RMSE)")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
     ap.add_argument("--tuning", default="",
                     help="comma-separated oxr_tuning fields to set before the steps, e.g. stack_lds=16,pool_paths=4194304")
@@ -78,6 +82,34 @@ def cpu_baseline(scene, args):
 
 def spp_total(args):
     return args.spp
+
+
+def serving(ctx, args):
+    """The latency half of the metric: Poisson arrivals over four resident
+    scenes (Cornell box, this sphere field, the 20 M-triangle heightfield
+    with the medium box, a 5 M-triangle sphere field) served by the
+    continuous-batching server, plain accumulation against the KPCNN
+    denoiser at equal RMSE of the tone-mapped image
+    (optixrender_amd.serve_bench)."""
+    from optixrender_amd import denoise, serve_bench
+
+    print("[bench] serving: uploading the resident scenes", file=sys.stderr, flush=True)
+    ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
+    resident = serve_bench.upload_scenes(ctx)
+    _, sid, cam = resident[1]
+    eq = serve_bench.equal_rmse_factor(ctx, sid, cam)
+    print(f"[bench] serving: equal RMSE {json.dumps(eq)}", file=sys.stderr, flush=True)
+    stream = serve_bench.request_stream(args.serve_requests, args.serve_rate, len(resident))
+    plain = serve_bench.run_arm(ctx, resident, stream, False, 1.0)
+    print(f"[bench] serving: plain {json.dumps(plain)}", file=sys.stderr, flush=True)
+    dn = serve_bench.run_arm(ctx, resident, stream, True, eq["factor"])
+    print(f"[bench] serving: denoised {json.dumps(dn)}", file=sys.stderr, flush=True)
+    for _, s, _ in resident:
+        ctx.release(s)
+    return {"requests": args.serve_requests, "rate_per_s": args.serve_rate,
+            "scenes": [r[0] for r in resident], "equal_rmse": eq,
+            "plain_p50_ms": plain["p50_ms"], "plain_p99_ms": plain["p99_ms"],
+            "denoised_p50_ms": dn["p50_ms"], "denoised_p99_ms": dn["p99_ms"], "plain": plain, "denoised": dn}
 
 
 COPY_BYTES = 1 << 30
@@ -244,6 +276,8 @@ def main():
         out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
                               "rays_per_s": round((counted.extension_rays + counted.shadow_rays) /
                                                   (ms_per_step / 1e3), 0)})
+    if not args.no_serving and not args.profile_steps:
+        out["config"]["serving"] = serving(ctx, args)
     if not args.no_cpu_baseline and not args.profile_steps:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args)

@@ -12,8 +12,9 @@ Two arms over the same request stream:
   * plain: every request at its spp;
   * denoised: every request at spp / k with the KPCNN, where k is the
     equal-RMSE factor -- plain accumulation needs k times the samples of a
-    denoised render to reach the same log-space RMSE against a converged
-    reference (measured by `equal_rmse_factor` on the sphere field).
+    denoised render to reach the same RMSE of the tone-mapped image against
+    a converged reference (measured by `equal_rmse_factor` on the sphere
+    field at 1080p, by interpolation between measured plain sample counts).
 
 Latency is the server's own submit-to-done time per request (p50/p99).
 
@@ -68,7 +69,7 @@ def request_stream(n: int, rate: float, n_scenes: int, seed: int = 1):
 
 
 def run_arm(ctx: Context, resident, stream, denoised: bool, k: float, budget_gb: float = 160.0,
-            tick_paths: int = 1 << 24, depth: int = 16):
+            tick_paths: int = 1 << 26, depth: int = 16):
     srv = Server(ctx, budget_bytes=int(budget_gb * (1 << 30)), max_active=64, tick_paths=tick_paths,
                  max_frame_pixels=1920 * 1080)
     ids = []
@@ -101,9 +102,24 @@ def run_arm(ctx: Context, resident, stream, denoised: bool, k: float, budget_gb:
         srv.close()
 
 
-def equal_rmse_factor(ctx: Context, sid: int, cam, w=1920, h=1080, spp_dn=4, ref_spp=1024, depth=16):
+def display(rgb: np.ndarray) -> np.ndarray:
+    """The tone map of ComputeFinalPixelColors (PostProcessing.cu:26-48) in
+    float, before the 8-bit quantisation: what a viewer of the image sees."""
+    c = np.asarray(rgb, np.float64)[..., :3]
+    lum = 0.299 * c[..., 0] + 0.587 * c[..., 1] + 0.114 * c[..., 2]
+    f = 1.0 / (1.0 + lum / 1.5)
+    return np.clip(np.maximum(c * f[..., None], 0.0) ** (1.0 / 2.2), 0.0, 1.0)
+
+
+def display_rmse(a, b) -> float:
+    return float(np.sqrt(np.mean((display(a) - display(b)) ** 2)))
+
+
+def equal_rmse_factor(ctx: Context, sid: int, cam, w=1920, h=1080, spp_dn=4, ref_spp=4096, depth=16,
+                      plain_spp=(4, 8, 16, 32, 64, 128, 256)):
     """Samples plain accumulation needs, relative to spp_dn denoised samples,
-    for the same RMSE of log(1 + colour) against a ref_spp render."""
+    for the same RMSE of the displayed (tone-mapped) image against a ref_spp
+    render: log-log interpolation between the measured plain sample counts."""
     def render(spp, dn=False):
         ctx.select(sid)
         ctx.resize(w, h)
@@ -112,15 +128,24 @@ def equal_rmse_factor(ctx: Context, sid: int, cam, w=1920, h=1080, spp_dn=4, ref
         return ctx.download_aov(3 if dn else 0, w, h)
 
     ref = render(ref_spp)
-    plain = {s: denoise.log_rmse(render(s), ref) for s in (2, 4, 8, 16, 32, 64)}
-    dn = denoise.log_rmse(render(spp_dn, True), ref)
-    xs = np.log(np.array(list(plain.keys()), float))
-    ys = np.log(np.array(list(plain.values()), float))
-    slope, icpt = np.polyfit(xs, ys, 1)
-    spp_equal = float(np.exp((math.log(dn) - icpt) / slope))
-    return {"ref_spp": ref_spp, "denoised_spp": spp_dn, "log_rmse_denoised": round(dn, 5),
-            "log_rmse_plain": {str(k): round(v, 5) for k, v in plain.items()}, "fit_slope": round(float(slope), 3),
-            "plain_spp_at_equal_rmse": round(spp_equal, 1), "factor": round(spp_equal / spp_dn, 2)}
+    plain = {s: display_rmse(render(s), ref) for s in plain_spp}
+    dn = display_rmse(render(spp_dn, True), ref)
+    xs = np.log(np.array(plain_spp, float))
+    ys = np.log(np.array([plain[s] for s in plain_spp]))
+    ly = math.log(dn)
+    if ly >= ys[0]:
+        spp_equal = float(plain_spp[0]) * math.exp((ys[0] - ly) / (ys[1] - ys[0]) * (xs[1] - xs[0]))
+    elif ly <= ys[-1]:
+        spp_equal = float("inf")  # better than the largest plain count measured
+    else:
+        i = int(np.searchsorted(-ys, -ly))  # ys decreasing
+        f = (ly - ys[i - 1]) / (ys[i] - ys[i - 1])
+        spp_equal = float(math.exp(xs[i - 1] + f * (xs[i] - xs[i - 1])))
+    factor = spp_equal / spp_dn if math.isfinite(spp_equal) else float(plain_spp[-1]) / spp_dn
+    return {"metric": "RMSE of the tone-mapped image", "ref_spp": ref_spp, "denoised_spp": spp_dn,
+            "rmse_denoised": round(dn, 5), "rmse_plain": {str(k): round(v, 5) for k, v in plain.items()},
+            "plain_spp_at_equal_rmse": round(spp_equal, 1) if math.isfinite(spp_equal) else f">{plain_spp[-1]}",
+            "factor": round(factor, 2)}
 
 
 def main(argv=None):

@@ -55,6 +55,8 @@ def lib():
         L.orc_bvh_create.restype = _P
         L.orc_bvh_create.argtypes = [C.c_int64, _P]
         L.orc_bvh_destroy.argtypes = [_P]
+        L.orc_bvh_wide.argtypes = [_P, C.c_int64, _P, _P]
+        L.orc_bvh_wide.restype = C.c_int64
         L.orc_bvh_closest.argtypes = [_P, C.c_int64, _P, _P, _P]
         L.orc_bvh_any.argtypes = [_P, C.c_int64, _P, _P, _P]
         L.orc_brute_closest.argtypes = [C.c_int64, _P, C.c_int64, _P, _P]
@@ -136,6 +138,14 @@ class Bvh:
         counts = np.zeros((rays.shape[0], 2), np.uint32)
         lib().orc_bvh_closest(self.h, rays.shape[0], p(rays), p(hits), p(counts))
         return hits, counts
+
+    def wide(self):
+        """(refs (m, 4) uint32, boxes (m, 4) int64) of the 4-wide hierarchy."""
+        m = lib().orc_bvh_wide(self.h, 0, None, None)
+        refs = np.zeros((m, 4), np.uint32)
+        boxes = np.zeros((m, 4), np.int64)
+        lib().orc_bvh_wide(self.h, m, p(refs), p(boxes))
+        return refs, boxes
 
     def any(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -273,7 +283,8 @@ def render_stats(scene: SceneData, width, height, spp, max_depth=25, rr_start=0,
                  camera=None):
     s, keep = _scene_struct(scene)
     prm = OrcRenderParams(width, height, spp, max_depth, rr_start, filt, frame_base, threads, 0, 0)
-    out = np.zeros(4, np.uint64)
+    out = np.zeros(6, np.uint64)
     cam = camera or scene.camera
     assert lib().orc_render_stats(C.byref(s), C.byref(cam), C.byref(prm), p(out)) == 0
-    return dict(extension_rays=int(out[0]), shadow_rays=int(out[1]), node_fetches=int(out[2]), tri_tests=int(out[3]))
+    return dict(extension_rays=int(out[0]), shadow_rays=int(out[1]), node_fetches=int(out[2]), tri_tests=int(out[3]),
+                ext_node_fetches=int(out[4]), ext_tri_tests=int(out[5]))

@@ -258,6 +258,17 @@ def test_render_sphere_field_per_path(ctx, pool):
     assert per_path_check(paths, rp, "sphere_field") >= PASS_FRACTION
 
 
+@pytest.mark.parametrize("pool", [None, 2048])
+def test_render_heightfield_medium_per_path(ctx, pool):
+    # the heightfield-with-medium-box configuration at a small size: per-
+    # triangle materials, ThinDielectric boundary into a homogeneous HG medium,
+    # a dielectric sphere inside, HDR sky, depth 32
+    sc = scenes.heightfield_medium(verts=33, sphere_subdiv=2, env_w=256, env_h=128)
+    print(f"[render] heightfield_medium {sc.n_tris} tris 64x40x2 pool={pool}", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 40, 2, 32, 0, pool)
+    assert per_path_check(paths, rp, "heightfield_medium") >= PASS_FRACTION
+
+
 @pytest.mark.parametrize("phase,g", [(0, 0.0), (1, 0.3), (1, -0.6)])
 def test_render_fog_per_path(ctx, phase, g):
     sc = scenes.foggy_cornell(phase_type=phase, g=g)
