@@ -28,8 +28,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# 112-B wide node (four slot boxes + refs), 36-B triangle, 32-B ray in + 16-B hit out
-NODE_BYTES, TRI_BYTES, RAY_BYTES = 112, 36, 48
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 32, 36, 48   # 32-B node record, 36-B triangle, 32-B ray in + 16-B hit out
 
 
 def parse():

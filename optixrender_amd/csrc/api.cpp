@@ -152,7 +152,7 @@ void free_scene(SceneDev* s) {
     dfree(s->positions); dfree(s->normals); dfree(s->mat_ids); dfree(s->normals_sorted); dfree(s->mat_sorted);
     dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache);
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
-    dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris); dfree(s->bvh.wide);
+    dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
     delete s;
 }
 
@@ -245,7 +245,7 @@ float oxr::env_luminance_sum(const float* rgba, int64_t texels) {
 
 void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, const float* lut_albedo) {
     std::memset(&p, 0, sizeof(p));
-    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n, s->bvh.wide, s->bvh.n_wide};
+    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
     p.normals = s->normals_sorted;
     p.mat_ids = s->mat_sorted;
     p.mats = s->mats;
@@ -440,7 +440,6 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         OXR_HIP(c, dalloc(&b.lo, 3 * nodes));
         OXR_HIP(c, dalloc(&b.hi, 3 * nodes));
         OXR_HIP(c, dalloc(&b.pairs, 4 * ni));
-        OXR_HIP(c, dalloc(&b.wide, 8 * ni));
         OXR_HIP(c, dalloc(&b.tris, 3 * n));
         OXR_HIP(c, dalloc(&s->normals_sorted, 9 * n));
         OXR_HIP(c, dalloc(&s->mat_sorted, n));
@@ -550,7 +549,7 @@ int oxr_trace_rays(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float*
         cleanup();
         return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
     }
-    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n, s->bvh.wide, s->bvh.n_wide};
+    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
     if ((e = wave_reserve(c->eng, 0, 0, c->trace_grid)) == hipSuccess &&
         (e = hipMemsetAsync(c->eng.overflow, 0, sizeof(int), c->stream)) == hipSuccess)
         e = trace_rays(bv, dr, n, any_hit, dh, dOcc, dc, c->eng.gstack, c->trace_grid, c->eng.overflow, c->stream);

@@ -301,76 +301,6 @@ __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, in
     }
 }
 
-// One level of the 4-wide collapse: each listed binary internal node b (wide
-// node w) gets its slots -- b's left child if a leaf, else its two children,
-// then the same for the right child -- and every internal slot becomes a
-// wide node of the next level, numbered alloc + its position in that list
-// (one atomic per wave for the positions).
-__global__ __launch_bounds__(256) void k_collapse4(const float4* __restrict__ pairs, const uint32_t* __restrict__ in_bin,
-                                                   const uint32_t* __restrict__ in_wide, int count, uint32_t* out_bin,
-                                                   uint32_t* out_wide, uint32_t* out_count, uint32_t alloc,
-                                                   float4* wide) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const bool live = i < count;
-    uint32_t ref[4] = {kNoSlot, kNoSlot, kNoSlot, kNoSlot};
-    float4 lo[4], hi[4];
-    uint32_t newbin[4] = {0, 0, 0, 0};
-    int newslot[4] = {0, 0, 0, 0};
-    int ns = 0, nnew = 0;
-    uint32_t w = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) lo[k] = hi[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (live) {
-        const uint32_t b = in_bin[i];
-        w = in_wide[i];
-#pragma unroll
-        for (int side = 0; side < 2; side++) {
-            const float4 lc = pairs[4 * (size_t)b + 2 * side], hc = pairs[4 * (size_t)b + 2 * side + 1];
-            const uint32_t c = __float_as_uint(lc.w);
-            if (c & kLeafBit) {
-                ref[ns] = c; lo[ns] = lc; hi[ns] = hc; ns++;
-            } else {
-#pragma unroll
-                for (int g = 0; g < 2; g++) {
-                    const float4 lg = pairs[4 * (size_t)c + 2 * g], hg = pairs[4 * (size_t)c + 2 * g + 1];
-                    const uint32_t gg = __float_as_uint(lg.w);
-                    ref[ns] = gg; lo[ns] = lg; hi[ns] = hg;
-                    if (!(gg & kLeafBit)) { newbin[nnew] = gg; newslot[nnew] = ns; nnew++; }
-                    ns++;
-                }
-            }
-        }
-    }
-    int incl = nnew;  // inclusive scan over the wave
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(incl, off);
-        if (lane >= off) incl += t;
-    }
-    const int total = __shfl(incl, 63);
-    int base = 0;
-    if (lane == 0 && total) base = (int)atomicAdd(out_count, (uint32_t)total);
-    base = __shfl(base, 0);
-    const int pos = base + incl - nnew;
-    for (int j = 0; j < nnew; j++) {
-        out_bin[pos + j] = newbin[j];
-        out_wide[pos + j] = alloc + (uint32_t)(pos + j);
-        ref[newslot[j]] = alloc + (uint32_t)(pos + j);
-    }
-    if (!live) return;
-    float4* nd = wide + 8 * (size_t)w;
-    nd[0] = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-    nd[1] = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-    nd[2] = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-    nd[3] = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-    nd[4] = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-    nd[5] = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-    nd[6] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
-                        __uint_as_float(ref[3]));
-    nd[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -421,35 +351,7 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
     }
     k_refit<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.left, o.right, o.parent, flags, o.lo, o.hi,
                                                o.pairs, o.tris, bounds + 6);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    o.n_wide = 0;
-    if (n < 2) return hipStreamSynchronize(s);
-    // 4-wide collapse, level by level from the root; the sort's key/value
-    // buffers hold the level lists, flags[0] the next level's size
-    uint32_t* bin[2] = {k0, k1};
-    uint32_t* wid[2] = {v0, v1};
-    const uint32_t root[2] = {0u, 0u};
-    if ((e = hipMemcpyAsync(bin[0], &root[0], 4, hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(wid[0], &root[1], 4, hipMemcpyHostToDevice, s)) != hipSuccess)
-        return e;
-    uint32_t count = 1, alloc = 1;
-    int cur = 0;
-    while (count > 0) {
-        if ((e = hipMemsetAsync(flags, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
-        k_collapse4<<<blocks_for(count, 256), 256, 0, s>>>(o.pairs, bin[cur], wid[cur], (int)count, bin[cur ^ 1],
-                                                          wid[cur ^ 1], flags, alloc, o.wide);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        uint32_t next = 0;
-        if ((e = hipMemcpyAsync(&next, flags, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-            (e = hipStreamSynchronize(s)) != hipSuccess)
-            return e;
-        if ((uint64_t)alloc + next > (uint64_t)(n - 1)) return hipErrorInvalidValue;  // malformed hierarchy
-        alloc += next;
-        count = next;
-        cur ^= 1;
-    }
-    o.n_wide = alloc;
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 }  // namespace oxr

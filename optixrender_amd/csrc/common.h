@@ -9,7 +9,7 @@ namespace oxr {
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kNoHit = 0xffffffffu;
-constexpr int kStackTotal = 128;  // traversal stack entries per lane (up to 3 pushes per wide level of the LBVH)
+constexpr int kStackTotal = 64;   // traversal stack entries per lane (LBVH depth <= 30 + log2 n < 64)
 constexpr int kStackLdsMin = 8;   // fewest entries an instance keeps in LDS; the rest spill to HBM
 constexpr int kTraceBlock = 256;
 
@@ -17,19 +17,11 @@ constexpr int kTraceBlock = 256;
 // records at pairs[4i .. 4i+3]: (lo.xyz, ref), (hi.xyz, 0) for the left
 // child then the right child; ref = internal id, or kLeafBit | sorted leaf.
 // tris: 3 float4 per sorted leaf: (v0, triangle id bits), (v1, 0), (v2, 0).
-// wide: the 4-wide traversal hierarchy collapsed from the binary one, 8
-// float4 (128 B) per node: [0..2] lo.x/y/z of slots 0..3, [3..5] hi.x/y/z,
-// [6] slot refs (kLeafBit | sorted leaf, a wide node, or kNoSlot), [7] unused.
-// The wide node of binary internal node b holds b's left child if it is a
-// leaf or else that child's two children, then the same for the right child.
 struct BvhView {
     const float4* pairs;
     const float4* tris;
     int64_t n;
-    const float4* wide;
-    int64_t n_wide;
 };
-constexpr uint32_t kNoSlot = 0xffffffffu;
 
 struct BvhBuffers {
     // Karras layout kept for parity checks and refits
@@ -42,8 +34,6 @@ struct BvhBuffers {
     float* hi = nullptr;           // 3*(2n-1)
     float4* pairs = nullptr;       // 4*(n-1)
     float4* tris = nullptr;        // 3n
-    float4* wide = nullptr;        // 8 per wide node, at most n-1 nodes
-    int64_t n_wide = 0;            // set by bvh_build
 };
 
 struct Counters64 {
@@ -56,7 +46,6 @@ struct Counters64 {
 size_t bvh_scratch_bytes(int64_t n);
 // Device word set non-zero by the build if the hierarchy came out malformed.
 uint32_t* bvh_error_word(void* d_scratch, int64_t n);
-// Synchronises the stream (the wide collapse reads each level's size back).
 hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void* d_scratch, hipStream_t s);
 
 // ---- launchers (render.hip) ----

@@ -2,11 +2,11 @@
 // (core/cuda/devicePrograms.cu:258-269 closest hit, :303-320/:389-405/:437-453
 // any hit with TERMINATE_ON_FIRST_HIT).
 //
-// One ray per lane over the 4-wide hierarchy (BvhView::wide).  Each visited
-// node costs one 112-byte fetch (7 x dwordx4) of its four slot boxes and
-// refs; leaf slots are tested at once with a watertight ray/triangle test
-// (Woop, Benthin, Wald 2013) on a 48-byte triangle record, then the hit
-// internal slots are visited nearest first.  The stack lives in LDS
+// One ray per lane.  Each visited internal node costs one 64-byte fetch of
+// its two child records (4 x dwordx4), leaves are tested at once in
+// (left, right) order with a watertight ray/triangle test (Woop, Benthin,
+// Wald 2013) on a 48-byte triangle record, then the nearer hit internal
+// child is descended and the farther pushed.  The stack lives in LDS
 // (S entries per lane, [entry][lane] so a wave touches consecutive banks) and
 // spills the deeper entries to a per-lane HBM region.
 // Closest-hit ties in t go to the lower triangle id, so the result does not
@@ -100,12 +100,6 @@ struct Stack {
     }
 };
 
-// Ordered traversal of the 4-wide hierarchy.  A visited node is one fetch
-// of 7 x dwordx4 (the four slot boxes SoA and the slot refs); its leaf slots
-// are tested at once in slot order, then its hit internal slots are ranked
-// by entry distance (ties: lower slot first), the nearest descended and the
-// others pushed farthest first.  Counting mode counts wide nodes fetched and
-// triangles tested, which equal the oracle's canonical counts.
 template <bool ANY, bool COUNT, class StackT>
 OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT& st, uint32_t& nn, uint32_t& nt,
                    int* overflow) {
@@ -134,64 +128,38 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
     }
     st.sp = 0;
     uint32_t node = 0;
-    // each wide node is visited at most once; the cap only guards against a
-    // malformed hierarchy
-    int64_t budget = b.n_wide + 64;
+    // A tree traversal visits each internal node at most once; the cap only
+    // guards against a malformed hierarchy.
+    int64_t budget = b.n + 64;
     while (true) {
-        if (--budget < 0 || node >= (uint64_t)b.n_wide) { *overflow = 2; break; }
-        const float4* nd = b.wide + 8 * (size_t)node;
-        const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5], rf = nd[6];
-        if (COUNT) nn++;
-        const uint32_t ref[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z),
-                                 __float_as_uint(rf.w)};
-        const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w};
-        const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
-        float tn[4];
-        bool hit[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            hit[k] = ref[k] != kNoSlot &&
-                     box_test(r, make_float4(lox[k], loy[k], loz[k], 0.0f), make_float4(hix[k], hiy[k], hiz[k], 0.0f),
-                              tmin, h.t, tn[k]);
+        if (--budget < 0 || node >= (uint32_t)(b.n - 1)) { *overflow = 2; break; }
+        const float4* pr = b.pairs + 4 * (size_t)node;
+        float4 lo0 = pr[0], hi0 = pr[1], lo1 = pr[2], hi1 = pr[3];
+        if (COUNT) nn += 2;
+        uint32_t c0 = __float_as_uint(lo0.w), c1 = __float_as_uint(lo1.w);
+        float tn0, tn1;
+        bool h0 = box_test(r, lo0, hi0, tmin, h.t, tn0);
+        bool h1 = box_test(r, lo1, hi1, tmin, h.t, tn1);
+        if (h0 && (c0 & kLeafBit)) {
+            h0 = false;
+            if (leaf_test(c0 & ~kLeafBit) && ANY) break;
         }
-        bool done = false;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {  // any hit stops at the first occluder
-            if (!done && hit[k] && (ref[k] & kLeafBit)) {
-                hit[k] = false;
-                if (leaf_test(ref[k] & ~kLeafBit) && ANY) done = true;
-            }
+        if (h1 && (c1 & kLeafBit)) {
+            h1 = false;
+            if (leaf_test(c1 & ~kLeafBit) && ANY) break;
         }
-        if (done) break;
-        // rank of each hit internal slot by (entry distance, slot)
-        int rank[4], m = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            rank[k] = 0;
-            m += hit[k] ? 1 : 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (j != k && hit[j] && (tn[j] < tn[k] || (tn[j] == tn[k] && j < k))) rank[k]++;
-        }
-        if (m == 0) {
+        if (h0 && h1) {
+            bool second = tn1 < tn0;
+            if (!st.push(second ? c0 : c1)) { *overflow = 1; break; }
+            node = second ? c1 : c0;
+        } else if (h0) {
+            node = c0;
+        } else if (h1) {
+            node = c1;
+        } else {
             if (st.sp == 0) break;
             node = st.pop();
-            continue;
         }
-        uint32_t nearest = 0;
-        bool ok = true;
-#pragma unroll
-        for (int q = 3; q >= 0; q--) {  // push ranks m-1 .. 1, take rank 0
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (!hit[k] || rank[k] != q) continue;
-                if (q == 0) nearest = ref[k];
-                else if (ok && !st.push(ref[k])) ok = false;
-            }
-        }
-        if (!ok) { *overflow = 1; break; }
-        node = nearest;
     }
     return h;
 }

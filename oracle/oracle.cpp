@@ -1231,10 +1231,6 @@ struct Bvh {
     std::vector<float> lo, hi;          // (n-1) internal then n leaves, 3 floats each
     std::vector<float> tri;             // sorted triangles, 9 floats
     std::vector<uint32_t> prim;         // sorted position -> triangle id
-    struct WideRefs {
-        std::vector<uint32_t> ref;
-        std::vector<int64_t> box;
-    } wide;                             // the 4-wide traversal hierarchy (build_wide)
 };
 void build_bvh(Bvh& b, int64_t n, const float* pos) {
     b.n = n;
@@ -1375,63 +1371,16 @@ inline bool box_test(const RayPre& r, const float* lo, const float* hi, float tm
 struct Hit { float t; uint32_t prim; float u, v; };
 constexpr uint32_t kNoHit = 0xffffffffu;
 
-// 4-wide hierarchy collapsed from the binary LBVH (the traversal layout of the
-// GPU; no reference counterpart: optixTrace's hierarchy is closed).  The wide
-// node of binary internal node b holds, in slot order, b's left child if it
-// is a leaf or else that child's two children, then the same for b's right
-// child: 2 to 4 slots.  An internal slot is the wide node of that binary
-// node.  Slot boxes are the binary nodes' boxes, bit for bit.
-// Bvh::WideRefs: ref, 4 per node: kLeaf | sorted leaf, a wide node index, or
-// kNoSlot; box, 4 per node: index into Bvh::lo / Bvh::hi.
-using Wide = Bvh::WideRefs;
-constexpr uint32_t kNoSlot = 0xffffffffu;
-void build_wide(const Bvh& b, Wide& w) {
-    w.ref.clear();
-    w.box.clear();
-    const int64_t ni = b.n - 1;
-    if (ni <= 0) return;
-    std::vector<uint32_t> todo{0};  // binary internal node of each wide node, in index order
-    for (size_t wi = 0; wi < todo.size(); wi++) {
-        const uint32_t bn = todo[wi];
-        uint32_t refs[4] = {kNoSlot, kNoSlot, kNoSlot, kNoSlot};
-        int64_t boxes[4] = {-1, -1, -1, -1};
-        int s = 0;
-        auto slot = [&](uint32_t c) {
-            if (c & kLeaf) {
-                refs[s] = c;
-                boxes[s] = ni + (c & ~kLeaf);
-            } else {
-                refs[s] = (uint32_t)todo.size();
-                boxes[s] = c;
-                todo.push_back(c);
-            }
-            s++;
-        };
-        for (uint32_t c : {b.left[bn], b.right[bn]}) {
-            if (c & kLeaf) {
-                slot(c);
-            } else {
-                slot(b.left[c]);
-                slot(b.right[c]);
-            }
-        }
-        for (int k = 0; k < 4; k++) {
-            w.ref.push_back(refs[k]);
-            w.box.push_back(boxes[k]);
-        }
-    }
-}
-
-// Canonical ordered traversal of the wide hierarchy: a visited node is one
-// fetch (all four slots), its leaf slots are tested at once in slot order,
-// then its hit internal slots are ordered by entry distance (ties: lower
-// slot first), the nearest descended and the others pushed farthest first.
+// Canonical ordered traversal: both children of a visited node are fetched
+// (2 records), leaves are tested at once in (left, right) order, then the
+// nearer internal child is descended and the farther pushed.
 template <bool ANY>
 Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
     Hit h{tmax, kNoHit, 0.0f, 0.0f};
     uint32_t nn = 0, nt = 0;
     if (b.n == 0) { if (cnt) { cnt[0] = 0; cnt[1] = 0; } return h; }
     RayPre r = prep_ray(o, d);
+    const int64_t ni = b.n - 1;
     auto try_leaf = [&](uint32_t k) -> bool {
         float t, u, v;
         nt++;
@@ -1441,47 +1390,43 @@ Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
         if (take) { h.t = t; h.prim = pid; h.u = u; h.v = v; return true; }
         return false;
     };
-    if (b.n == 1) {
+    if (ni == 0) {
         try_leaf(0);
         if (cnt) { cnt[0] = nn; cnt[1] = nt; }
         return h;
     }
-    const Wide& w = b.wide;
-    uint32_t stack[256];
+    uint32_t stack[128];
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        nn++;
-        float tn[4];
-        bool hit[4];
-        const uint32_t* ref = &w.ref[4 * (size_t)node];
-        for (int k = 0; k < 4; k++) {
-            hit[k] = false;
-            if (ref[k] == kNoSlot) continue;
-            int64_t idx = w.box[4 * (size_t)node + k];
+        uint32_t c[2] = {b.left[node], b.right[node]};
+        nn += 2;
+        float tn[2];
+        bool hit[2];
+        for (int k = 0; k < 2; k++) {
+            int64_t idx = (c[k] & kLeaf) ? ni + (c[k] & ~kLeaf) : c[k];
             hit[k] = box_test(r, &b.lo[3 * idx], &b.hi[3 * idx], tmin, h.t, tn[k]);
         }
         bool done = false;
-        for (int k = 0; k < 4 && !done; k++) {   // any hit stops at the first occluder
-            if (hit[k] && (ref[k] & kLeaf)) {
+        for (int k = 0; k < 2 && !done; k++) {   // any hit stops at the first occluder
+            if (hit[k] && (c[k] & kLeaf)) {
                 hit[k] = false;
-                if (try_leaf(ref[k] & ~kLeaf) && ANY) done = true;
+                if (try_leaf(c[k] & ~kLeaf) && ANY) done = true;
             }
         }
         if (done) break;
-        int order[4], m = 0;
-        for (int k = 0; k < 4; k++)
-            if (hit[k]) order[m++] = k;
-        // stable by entry distance
-        for (int i = 1; i < m; i++)
-            for (int j = i; j > 0 && tn[order[j]] < tn[order[j - 1]]; j--) std::swap(order[j], order[j - 1]);
-        if (m == 0) {
+        if (hit[0] && hit[1]) {
+            int nearer = (tn[1] < tn[0]) ? 1 : 0;
+            stack[sp++] = c[1 - nearer];
+            node = c[nearer];
+        } else if (hit[0]) {
+            node = c[0];
+        } else if (hit[1]) {
+            node = c[1];
+        } else {
             if (sp == 0) break;
             node = stack[--sp];
-            continue;
         }
-        for (int i = m - 1; i >= 1; i--) stack[sp++] = ref[order[i]];
-        node = ref[order[0]];
     }
     if (cnt) { cnt[0] = nn; cnt[1] = nt; }
     return h;
@@ -1764,7 +1709,6 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
 
 void load_scene(Scene& S, const orc_scene* sc) {
     build_bvh(S.bvh, sc->n_tris, sc->positions);
-    build_wide(S.bvh, S.bvh.wide);
     S.positions = sc->positions;
     S.normals = sc->normals;
     S.mat_ids = sc->mat_ids;
@@ -1831,19 +1775,9 @@ int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t*
 void* orc_bvh_create(int64_t n, const float* positions) {
     Bvh* b = new Bvh;
     build_bvh(*b, n, positions);
-    build_wide(*b, b->wide);
     return b;
 }
 void orc_bvh_destroy(void* bvh) { delete static_cast<Bvh*>(bvh); }
-int64_t orc_bvh_wide(const void* bvh, int64_t cap, uint32_t* refs, int64_t* boxes) {
-    const Bvh& b = *static_cast<const Bvh*>(bvh);
-    const int64_t nodes = (int64_t)b.wide.ref.size() / 4;
-    if (refs && boxes && cap >= nodes) {
-        std::memcpy(refs, b.wide.ref.data(), 16 * (size_t)nodes);
-        std::memcpy(boxes, b.wide.box.data(), 32 * (size_t)nodes);
-    }
-    return nodes;
-}
 
 void orc_bvh_closest(const void* bvh, int64_t n_rays, const float* rays, float* hits, uint32_t* counts) {
     const Bvh& b = *static_cast<const Bvh*>(bvh);

@@ -122,11 +122,6 @@ int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t*
    counts (optional): 2 uint32 per ray (node records fetched, triangle tests). */
 void* orc_bvh_create(int64_t n, const float* positions);
 void orc_bvh_destroy(void* bvh);
-/* The 4-wide traversal hierarchy: returns the node count; with cap >= it,
-   fills refs (4 per node: 0x80000000 | leaf, wide node, or 0xffffffff) and
-   boxes (4 per node: index of the slot's box among the binary nodes,
-   internal 0..n-2 then leaves). */
-int64_t orc_bvh_wide(const void* bvh, int64_t cap, uint32_t* refs, int64_t* boxes);
 void orc_bvh_closest(const void* bvh, int64_t n_rays, const float* rays, float* hits, uint32_t* counts);
 void orc_bvh_any(const void* bvh, int64_t n_rays, const float* rays, uint8_t* occluded, uint32_t* counts);
 void orc_brute_closest(int64_t n, const float* positions, int64_t n_rays, const float* rays, float* hits);
@@ -158,7 +153,7 @@ void orc_tonemap(int64_t n, const float* rgba, uint32_t* out);
 int orc_render(const orc_scene* scene, const orc_camera* cam, const orc_render_params* p,
                float* color, float* albedo, float* normal, float* depth, float* path_radiance);
 /* counts of extension/shadow rays and traversal work, for measurement:
-   out[0] extension rays, out[1] shadow rays, out[2] wide nodes fetched,
+   out[0] extension rays, out[1] shadow rays, out[2] node records fetched,
    out[3] triangle tests, out[4] / out[5] the same two for extension rays. */
 int orc_render_stats(const orc_scene* scene, const orc_camera* cam, const orc_render_params* p,
                      uint64_t* out4);

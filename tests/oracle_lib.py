@@ -55,8 +55,6 @@ def lib():
         L.orc_bvh_create.restype = _P
         L.orc_bvh_create.argtypes = [C.c_int64, _P]
         L.orc_bvh_destroy.argtypes = [_P]
-        L.orc_bvh_wide.argtypes = [_P, C.c_int64, _P, _P]
-        L.orc_bvh_wide.restype = C.c_int64
         L.orc_bvh_closest.argtypes = [_P, C.c_int64, _P, _P, _P]
         L.orc_bvh_any.argtypes = [_P, C.c_int64, _P, _P, _P]
         L.orc_brute_closest.argtypes = [C.c_int64, _P, C.c_int64, _P, _P]
@@ -138,14 +136,6 @@ class Bvh:
         counts = np.zeros((rays.shape[0], 2), np.uint32)
         lib().orc_bvh_closest(self.h, rays.shape[0], p(rays), p(hits), p(counts))
         return hits, counts
-
-    def wide(self):
-        """(refs (m, 4) uint32, boxes (m, 4) int64) of the 4-wide hierarchy."""
-        m = lib().orc_bvh_wide(self.h, 0, None, None)
-        refs = np.zeros((m, 4), np.uint32)
-        boxes = np.zeros((m, 4), np.int64)
-        lib().orc_bvh_wide(self.h, m, p(refs), p(boxes))
-        return refs, boxes
 
     def any(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
