@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
 }
 
 template <bool COUNT, int S>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(WaveParams p, int x) {
+__global__ __launch_bounds__(kTraceBlock, 8) void k_extend(WaveParams p, int x) {
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
     __shared__ uint32_t lds[S * kTraceBlock];
@@ -558,12 +558,14 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         } else if (closest == kFltMax && medium_index == -1) {              // devicePrograms.cu:354-373
             if (p.env_w > 0) {
                 f2 uv = sphere_to_plane(dir);
-                light_pdf = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
                 float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
                 light_radiance = f3(e.x, e.y, e.z);
-                light_pdf *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
                 float mw = 1.0f;
                 if (bounce != 0) {
+                    // the environment pdf only enters the MIS weight, which is
+                    // 1 for camera rays: their cache lookup is skipped
+                    light_pdf = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
+                    light_pdf *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
                     pre_pdf *= mult_trans_pdf;
                     mw = power_heuristic(pre_pdf, light_pdf);
                 }

@@ -1373,7 +1373,9 @@ constexpr uint32_t kNoHit = 0xffffffffu;
 
 // Canonical ordered traversal: both children of a visited node are fetched
 // (2 records), leaves are tested at once in (left, right) order, then the
-// nearer internal child is descended and the farther pushed.
+// nearer internal child is descended and the farther pushed (any hit: the
+// farther descended first, which finds the occluder of a ray leaving a
+// surface sooner; the answer is the same).
 template <bool ANY>
 Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
     Hit h{tmax, kNoHit, 0.0f, 0.0f};
@@ -1417,6 +1419,7 @@ Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
         if (done) break;
         if (hit[0] && hit[1]) {
             int nearer = (tn[1] < tn[0]) ? 1 : 0;
+            if (ANY) nearer = 1 - nearer;  // any hit descends the farther child first
             stack[sp++] = c[1 - nearer];
             node = c[nearer];
         } else if (hit[0]) {
