@@ -140,6 +140,7 @@ struct WaveParams {
     //   [2] throughput, bounce (int bits)   [3] radiance, sample id (uint bits; kNoSample: none)
     float4* path;
     uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
+    uint32_t* fresh_g[2];  // sample ids of the new-path regions (kShards regions of `region`)
     float4* hits;      // per queue position (2*kShards regions)
     float4* sh_env;    // kShards regions of `region` records, 3 float4 each
     float4* sh_light;

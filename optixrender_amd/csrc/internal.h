@@ -51,6 +51,7 @@ struct WaveEngine {
     float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
     float* s_dep = nullptr;
     uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr, *cls_queue = nullptr;
+    uint32_t *g0 = nullptr, *g1 = nullptr;  // sample ids of the new-path regions
     int* ctr = nullptr;
     int* overflow = nullptr;
     unsigned long long* trav = nullptr;

@@ -211,8 +211,11 @@ OXR_D int find_req(const ReqDesc* r, int n, int64_t i) {
 // consecutive positions starts 64 neighbouring pixels of one frame.
 OXR_D int64_t sample_of(int k, int pos) { return (((int64_t)(pos >> 6) * kShards + k) << 6) + (pos & 63); }
 
-// Camera ray of sample g (devicePrograms.cu:214-236) into slot pid.
-OXR_D void start_path(const WaveParams& p, uint32_t pid, int64_t g) {
+// Camera ray of sample g (devicePrograms.cu:214-236) and the RNG state after
+// the two jitter draws.  A path at bounce 0 has no slot record: extend and
+// shade recompute this from the sample id, and the record is written only
+// once the path survives its first shade.
+OXR_D void camera_ray(const WaveParams& p, int64_t g, f3& org, f3& dir, uint32_t& rng_state) {
     const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, g)];
     const int64_t local = g - q.path_begin;
     const int npix = q.width * q.height;
@@ -223,11 +226,15 @@ OXR_D void start_path(const WaveParams& p, uint32_t pid, int64_t g) {
     f2 jit = filter_jitter(f2{j0, j1}, q.filter);                         // devicePrograms.cu:220-227
     float sx = ((float)ix + 0.5f + jit.x) / (float)q.width;
     float sy = ((float)iy + 0.5f + jit.y) / (float)q.height;
-    f3 dir = normalize(make3(q.cam_dir) + (sx - 0.5f) * make3(q.cam_h) + (sy - 0.5f) * make3(q.cam_v));
-    p.path[4 * (size_t)pid + 0] = make_float4(q.cam_pos[0], q.cam_pos[1], q.cam_pos[2], 0.0f);
-    p.path[4 * (size_t)pid + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng.s));
-    p.path[4 * (size_t)pid + 2] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
-    p.path[4 * (size_t)pid + 3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)g));
+    dir = normalize(make3(q.cam_dir) + (sx - 0.5f) * make3(q.cam_h) + (sy - 0.5f) * make3(q.cam_v));
+    org = make3(q.cam_pos);
+    rng_state = rng.s;
+}
+
+// The new-path region of queue x: positions at or past kShards * region.
+OXR_D bool is_fresh(const WaveParams& p, int64_t qp) { return qp >= (int64_t)kShards * p.region; }
+OXR_D uint32_t fresh_sample(const WaveParams& p, int x, int64_t qp) {
+    return p.fresh_g[x][qp - (int64_t)kShards * p.region];
 }
 
 // Draws one new sample of shard k for every lane that asks (consecutive
@@ -243,9 +250,11 @@ OXR_D bool restart(const WaveParams& p, int k, int y, bool want, uint32_t pid) {
     const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
     const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
     const bool start = want && g < total;
-    if (start) start_path(p, pid, g);
     const int o = wave_append(ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + kShards + k), start);
-    if (o >= 0) p.queue[y][(size_t)(kShards + k) * p.region + o] = pid;
+    if (o >= 0) {
+        p.queue[y][(size_t)(kShards + k) * p.region + o] = pid;
+        p.fresh_g[y][(size_t)k * p.region + o] = (uint32_t)g;
+    }
     return start;
 }
 
@@ -340,9 +349,16 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_extend(WaveParams p, int x) 
         const int64_t qp = sq.position(j, k, p.region);
         int cls = -1;
         if (qp >= 0) {
-            uint32_t pid = q[qp];
-            float4 o = p.path[4 * (size_t)pid + 0], d = p.path[4 * (size_t)pid + 1];
-            Hit h = traverse<false, COUNT>(p.bvh, xyz(o), xyz(d), kEps, kFltMax, st, nn, nt, overflow);
+            f3 org, dir;
+            if (is_fresh(p, qp)) {
+                uint32_t rs;
+                camera_ray(p, fresh_sample(p, x, qp), org, dir, rs);
+            } else {
+                const uint32_t pid = q[qp];
+                org = xyz(p.path[4 * (size_t)pid + 0]);
+                dir = xyz(p.path[4 * (size_t)pid + 1]);
+            }
+            Hit h = traverse<false, COUNT>(p.bvh, org, dir, kEps, kFltMax, st, nn, nt, overflow);
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
             cls = h.leaf == kNoHit ? 0 : 1 + p.mats[p.mat_ids[h.leaf]].type;
         }
@@ -437,20 +453,38 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 // MT: the material type of the hit surface, fixed by the shade class; < 0 for
 // rays that left the scene.  MEDIA: the scene has participating media (the
 // medium code is compiled only then).
+// fresh: the path is at bounce 0 with sample id g_fresh and no record yet.
+// The slot record is written only if the path continues; rad_out returns
+// (radiance, sample id) for the caller.
 template <int MT, bool MEDIA>
-OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, ShadowOut& sh_env,
-                      ShadowOut& sh_light, float4& rad_out) {
+OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, bool fresh, uint32_t g_fresh,
+                      ShadowOut& sh_env, ShadowOut& sh_light, float4& rad_out) {
     sh_env.emit = false;
     sh_light.emit = false;
-    float4 ro = p.path[4 * (size_t)pid + 0], rd = p.path[4 * (size_t)pid + 1], th = p.path[4 * (size_t)pid + 2];
-    f3 org = xyz(ro), dir = xyz(rd);
-    float pre_pdf = ro.w;
-    Rng rng{__float_as_uint(rd.w)};
-    f3 history = xyz(th);
-    const int bounce = __float_as_int(th.w);
-    float4 rad4 = p.path[4 * (size_t)pid + 3];
-    f3 radiance = xyz(rad4);
-    const uint32_t g = __float_as_uint(rad4.w);
+    f3 org, dir, history, radiance;
+    float pre_pdf;
+    Rng rng;
+    int bounce;
+    uint32_t g;
+    if (fresh) {
+        camera_ray(p, g_fresh, org, dir, rng.s);
+        pre_pdf = 0.0f;
+        history = f3(1.0f);
+        radiance = f3(0.0f);
+        bounce = 0;
+        g = g_fresh;
+    } else {
+        const float4 ro = p.path[4 * (size_t)pid + 0], rd = p.path[4 * (size_t)pid + 1];
+        const float4 th = p.path[4 * (size_t)pid + 2], rad4 = p.path[4 * (size_t)pid + 3];
+        org = xyz(ro);
+        dir = xyz(rd);
+        pre_pdf = ro.w;
+        rng.s = __float_as_uint(rd.w);
+        history = xyz(th);
+        bounce = __float_as_int(th.w);
+        radiance = xyz(rad4);
+        g = __float_as_uint(rad4.w);
+    }
     f3 V = -dir, L = dir;
     float mult_trans_pdf = 1.0f;
 
@@ -651,13 +685,13 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         }
         if (alive && any_nan(history)) alive = false;                       // devicePrograms.cu:487-489
     }
-    rad_out = make_float4(radiance.x, radiance.y, radiance.z, rad4.w);
-    p.path[4 * (size_t)pid + 3] = rad_out;
+    rad_out = make_float4(radiance.x, radiance.y, radiance.z, __uint_as_float(g));
     const bool cont = alive && bounce + 1 < p.max_depth;
     if (cont) {
         p.path[4 * (size_t)pid + 0] = make_float4(pos.x, pos.y, pos.z, pre_pdf);
         p.path[4 * (size_t)pid + 1] = make_float4(L.x, L.y, L.z, __uint_as_float(rng.s));
         p.path[4 * (size_t)pid + 2] = make_float4(history.x, history.y, history.z, __int_as_float(bounce + 1));
+        p.path[4 * (size_t)pid + 3] = rad_out;
     }
     return cont;
 }
@@ -707,14 +741,18 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         if (active) {
             const uint32_t qp = cq[slot];
             pid = q[qp];
-            cont = shade_path<CLS - 1, MEDIA>(p, lut, pid, p.hits[qp], sev, slv, rad);
+            const bool fresh = is_fresh(p, qp);
+            cont = shade_path<CLS - 1, MEDIA>(p, lut, pid, p.hits[qp], fresh, fresh ? fresh_sample(p, x, qp) : 0u,
+                                              sev, slv, rad);
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
-        // once.  One that still waits for a connect is finished by k_regen.
+        // once.  One that still waits for a connect keeps its radiance in the
+        // slot record for connect to add to; k_regen finishes it.
         const bool ended = active && !cont;
         const bool final = ended && !sev.emit && !slv.emit;
         if (final) p.s_rad[__float_as_uint(rad.w)] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        if (ended && !final) p.path[4 * (size_t)pid + 3] = rad;
         int o = wave_append(qo_count, cont);
         if (o >= 0) qo[o] = pid;
         o = wave_append(term_count, ended && !final);

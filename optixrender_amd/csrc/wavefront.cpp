@@ -40,8 +40,9 @@ int64_t wave_round_pool(int64_t paths) {
 }
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
-    // per slot: state 64, hits 2x16, queues 2x2x4, term 4, shadow records 2x48, shade classes 9x4
-    return (size_t)pool * (64 + 32 + 16 + 4 + 96 + 36) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
+    // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, term 4, shadow records
+    // 2x48, shade classes 9x4
+    return (size_t)pool * (64 + 32 + 16 + 8 + 4 + 96 + 36) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
            ctr_ints() * 4 + 4096;
 }
 
@@ -53,7 +54,7 @@ void wave_drop_graphs(WaveEngine& e) {
 
 void wave_release(WaveEngine& e) {
     wave_drop_graphs(e);
-    wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light); wfree(e.cls_queue);
+    wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
     wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.gstack);
     wfree(e.ctr); wfree(e.overflow); wfree(e.trav);
     if (e.host_counts) (void)hipHostFree(e.host_counts);
@@ -83,13 +84,14 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
     if (pool > e.pool) {
         wave_drop_graphs(e);
         wfree(e.path); wfree(e.hits); wfree(e.sh_env);
-        wfree(e.sh_light); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.cls_queue);
+        wfree(e.sh_light); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
         e.pool = 0;
         if ((r = walloc(&e.path, 4 * pool)) != hipSuccess ||
             (r = walloc(&e.hits, 2 * pool)) != hipSuccess || (r = walloc(&e.sh_env, 3 * pool)) != hipSuccess ||
             (r = walloc(&e.sh_light, 3 * pool)) != hipSuccess || (r = walloc(&e.q0, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.q1, 2 * pool)) != hipSuccess || (r = walloc(&e.term, pool)) != hipSuccess ||
-            (r = walloc(&e.cls_queue, (size_t)kShadeClasses * pool)) != hipSuccess)
+            (r = walloc(&e.cls_queue, (size_t)kShadeClasses * pool)) != hipSuccess ||
+            (r = walloc(&e.g0, pool)) != hipSuccess || (r = walloc(&e.g1, pool)) != hipSuccess)
             return r;
         e.pool = pool;
     }
@@ -117,6 +119,7 @@ void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p) {
     p.region = (int)(pool / kShards);
     p.path = e.path;
     p.queue[0] = e.q0; p.queue[1] = e.q1;
+    p.fresh_g[0] = e.g0; p.fresh_g[1] = e.g1;
     p.hits = e.hits; p.sh_env = e.sh_env; p.sh_light = e.sh_light; p.term = e.term; p.cls_queue = e.cls_queue;
     p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm; p.s_dep = e.s_dep;
     p.ctr = e.ctr; p.traversal = e.trav; p.gstack = e.gstack; p.overflow = e.overflow;
