@@ -3,12 +3,11 @@
 One step = one render call of the headline workload: the 983,040-triangle
 sphere field (12 icospheres at subdivision 6 under a 2048x1024 procedural HDR
 sky) at 1920x1080, 64 spp, max depth 16: 132.7 M paths through the
-regenerating wavefront loop (extend -> shade -> connect -> regenerate on a
-resident path pool, then accumulate and tone map).  Scene upload and the BVH
-build happen before the timed region (the build time is reported
-separately).
+regenerating wavefront loop (trace -> regenerate -> shade on a resident path
+pool, then accumulate and tone map).  Scene upload and the BVH build happen
+before the timed region (the build time is reported separately).
 
-Prints one JSON line with Mpaths/s, the traversal roofline of the extend
+Prints one JSON line with Mpaths/s, the traversal roofline of the trace
 kernel (algorithmic bytes from an untimed counting pass of the same frames,
 kernel time from HIP events on the library's stream during the timed steps)
 and the CPU oracle timed on a bounded sample on this host.
@@ -29,6 +28,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES = 32, 36, 48   # 32-B node record, 36-B triangle, 32-B ray in + 16-B hit out
+SHADOW_RAY_BYTES = 40   # 4-B slot id + 32-B ray in + 4-B flag cleared when occluded
 
 
 def parse():
@@ -129,7 +129,7 @@ def copy_calibration():
     del src, dst
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "pmc_k_extend_sphere_field_1080p_64spp.md"
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_k_trace_sphere_field_1080p_64spp.md"
 
 
 def tuning_key(t) -> str:
@@ -137,7 +137,7 @@ def tuning_key(t) -> str:
 
 
 def measured_traffic(workload: str, tuning: str):
-    """HBM bytes per k_extend launch from the committed counter summary, if
+    """HBM bytes per k_trace launch from the committed counter summary, if
     it was measured on this workload with this tuning (lines `workload: ...`,
     `tuning: ...`, `traffic_bytes_per_launch: N`), else None."""
     if not TRAFFIC_FILE.exists():
@@ -201,17 +201,16 @@ def main():
     for i in range(args.warmup):
         step()
         print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    ctx_ms, ext_ms, conn_ms, shade_ms, regen_ms = [], [], [], [], []
+    ctx_ms, trace_ms, shade_ms, regen_ms = [], [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         st = step()
         ctx_ms.append(st.render_ms)
-        ext_ms.append(st.extend_ms)
-        conn_ms.append(st.connect_ms)
+        trace_ms.append(st.trace_ms)
         shade_ms.append(st.shade_ms)
         regen_ms.append(st.generate_ms)
-        print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms (extend {st.extend_ms:.1f}, shade "
-              f"{st.shade_ms:.1f}, connect {st.connect_ms:.1f}, regenerate {st.generate_ms:.1f}, accumulate "
+        print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms (trace {st.trace_ms:.1f}, shade "
+              f"{st.shade_ms:.1f}, regenerate {st.generate_ms:.1f}, accumulate "
               f"{st.accumulate_ms:.1f}; {st.iterations} iterations, {st.kernel_launches} launches)",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -242,9 +241,8 @@ def main():
         "config": {"workload": f"sphere_field {scene.n_tris} tris, {W}x{H}, {spp} spp, depth {depth}, "
                                f"2048x1024 HDR sky (seed 1)",
                    "bvh_build_ms": round(build_ms, 3), "device_ms_per_step": round(float(np.mean(ctx_ms)), 3),
-                   "extend_ms_per_step": round(float(np.mean(ext_ms)), 3),
+                   "trace_ms_per_step": round(float(np.mean(trace_ms)), 3),
                    "shade_ms_per_step": round(float(np.mean(shade_ms)), 3),
-                   "connect_ms_per_step": round(float(np.mean(conn_ms)), 3),
                    "regenerate_ms_per_step": round(float(np.mean(regen_ms)), 3),
                    "iterations_per_step": int(st.iterations), "launches_per_step": int(st.kernel_launches),
                    "tuning": ctx.get_tuning().as_dict(),
@@ -256,21 +254,20 @@ def main():
     if counted is not None:
         ext_bytes = NODE_BYTES * counted.ext_node_fetches + TRI_BYTES * counted.ext_tri_tests + \
             RAY_BYTES * counted.extension_rays
-        conn_nodes = counted.node_fetches - counted.ext_node_fetches
-        conn_tris = counted.tri_tests - counted.ext_tri_tests
-        conn_bytes = NODE_BYTES * conn_nodes + TRI_BYTES * conn_tris + RAY_BYTES * counted.shadow_rays
-        ext_s = float(np.mean(ext_ms)) / 1e3
-        achieved = ext_bytes / ext_s / 1e9
-        both = (ext_bytes + conn_bytes) / ((float(np.mean(ext_ms)) + float(np.mean(conn_ms))) / 1e3) / 1e9
+        sh_nodes = counted.node_fetches - counted.ext_node_fetches
+        sh_tris = counted.tri_tests - counted.ext_tri_tests
+        sh_bytes = NODE_BYTES * sh_nodes + TRI_BYTES * sh_tris + SHADOW_RAY_BYTES * counted.shadow_rays
+        alg = ext_bytes + sh_bytes
+        achieved = alg / (float(np.mean(trace_ms)) / 1e3) / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(out["config"]["workload"], tuning_key(ctx.get_tuning())),
-            "kernel": "k_extend (all launches of one step)",
+            "kernel": "k_trace (all launches of one step: extension and shadow rays)",
             "launches_per_step": int(st.iterations),
-            "bytes_per_step": int(ext_bytes),
-            "per_ray_bytes": round(ext_bytes / max(counted.extension_rays, 1), 1),
-            "extend_plus_connect_GBs": round(both, 1),
+            "bytes_per_step": int(alg),
+            "per_ray_bytes": {"extension": round(ext_bytes / max(counted.extension_rays, 1), 1),
+                              "shadow": round(sh_bytes / max(counted.shadow_rays, 1), 1)},
         }
         out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
                               "rays_per_s": round((counted.extension_rays + counted.shadow_rays) /

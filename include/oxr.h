@@ -122,7 +122,9 @@ typedef struct oxr_render_params {
 
 typedef struct oxr_render_stats {
     double render_ms;        /* whole oxr_render wall time on the device stream */
-    double extend_ms, shade_ms, connect_ms;
+    double trace_ms;         /* ray traversal: each iteration traces its extension rays and the previous
+                                bounce's shadow rays in one launch */
+    double shade_ms;
     double generate_ms;      /* window start and path regeneration */
     double accumulate_ms, tonemap_ms;
     double denoise_ms;       /* KPCNN denoiser (denoise = 1 only) */

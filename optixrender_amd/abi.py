@@ -190,9 +190,8 @@ class RenderParams(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [
         ("render_ms", C.c_double),
-        ("extend_ms", C.c_double),
+        ("trace_ms", C.c_double),
         ("shade_ms", C.c_double),
-        ("connect_ms", C.c_double),
         ("generate_ms", C.c_double),
         ("accumulate_ms", C.c_double),
         ("tonemap_ms", C.c_double),

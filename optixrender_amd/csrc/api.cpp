@@ -722,9 +722,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
         c->denoised_valid = true;
     }
     st.generate_ms = stage[0];
-    st.extend_ms = stage[1];
+    st.trace_ms = stage[1];
     st.shade_ms = stage[2];
-    st.connect_ms = stage[3];
     st.accumulate_ms = stage[4];
     st.ext_node_fetches = tv[0];
     st.ext_tri_tests = tv[1];

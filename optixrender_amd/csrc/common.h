@@ -94,22 +94,24 @@ constexpr int kShards = 32;
 constexpr int kCtrStride = 64;  // ints between counters: one 256-byte line each
 
 // Counter ids (int index = id * kCtrStride).
+// Lists written by the stage at parity y and read by a later stage carry a
+// parity index; see wave_trace for who writes, reads and clears each.
 enum CtrId {
     kCtrQueue = 0,                       // 2 x 64 region counts: [parity][region]; region k survivors, 32+k new paths
-    kCtrShEnv = kCtrQueue + 2 * 2 * kShards,   // per shard: env shadow records
-    kCtrShLight = kCtrShEnv + kShards,   // per shard: light shadow records
-    kCtrTerm = kCtrShLight + kShards,    // per shard: slots whose path ended
-    kCtrFetchExtend = kCtrTerm + kShards,
-    kCtrFetchShade = kCtrFetchExtend + kShards,
-    kCtrFetchConnEnv = kCtrFetchShade + kShards,
-    kCtrFetchConnLight = kCtrFetchConnEnv + kShards,
-    kCtrFetchRegen = kCtrFetchConnLight + kShards,
+    kCtrTerm = kCtrQueue + 2 * 2 * kShards,    // [parity][shard]: slots whose path ended with shadow rays pending
+    kCtrShEnv = kCtrTerm + 2 * kShards,  // [parity][shard]: env shadow rays listed
+    kCtrShLight = kCtrShEnv + 2 * kShards,     // [parity][shard]: light shadow rays listed
+    kCtrFetchTrace = kCtrShLight + 2 * kShards,
+    kCtrFetchRegen = kCtrFetchTrace + kShards,
     kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
     kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
-    kCtrClass = kCtrWindowTotal + 1,           // [class][shard]: entries of the shade-class queues
-    kCtrFetchClass = kCtrClass + 9 * kShards,  // [class][shard]: chunks taken by the shade kernels
+    kCtrClass = kCtrWindowTotal + 1,           // [parity][class][shard]: entries of the shade-class queues
+    kCtrFetchClass = kCtrClass + 2 * 9 * kShards,  // [class][shard]: chunks taken by the shade kernels
     kCtrIds = kCtrFetchClass + 9 * kShards
 };
+// The host's completion poll copies counters [kCtrQueue, kPollIds): the
+// queue and pending-finish counts of both parities.
+constexpr int kPollIds = kCtrTerm + 2 * kShards;
 constexpr size_t ctr_ints() { return (size_t)kCtrIds * kCtrStride; }
 constexpr int kShadeClasses = 9;
 
@@ -142,9 +144,15 @@ struct WaveParams {
     uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
     uint32_t* fresh_g[2];  // sample ids of the new-path regions (kShards regions of `region`)
     float4* hits;      // per queue position (2*kShards regions)
-    float4* sh_env;    // kShards regions of `region` records, 3 float4 each
+    // shadow rays, one record of 3 float4 per slot and kind (env / light):
+    //   [0] origin, tmax   [1] direction   [2] contribution, pending flag
+    // The flag is 1 when the slot's last shade emitted the ray and the trace
+    // found it unoccluded, 0 otherwise; the slot's next shade (or k_regen)
+    // adds the contribution.
+    float4* sh_env;
     float4* sh_light;
-    uint32_t* term;    // kShards regions of `region` slot ids
+    uint32_t* sh_list[2];  // [parity]: slot ids with an env ray in kShards regions, then with a light ray
+    uint32_t* term[2];     // [parity]: kShards regions of `region` slot ids
     // shade classes: 0 = the ray left the scene, 1 + t = it hit a surface of
     // material type t.  cls_queue[class][shard] holds `region` queue positions.
     uint32_t* cls_queue;
@@ -157,7 +165,7 @@ struct WaveParams {
     int* ctr;          // device counters, ctr_ints() ints
     const ReqDesc* reqs;  // device table of the window's requests
     int nreq;
-    unsigned long long* traversal;  // [0,1] extend node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] connect node fetches / triangle tests
+    unsigned long long* traversal;  // [0,1] extension-ray node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] shadow-ray node fetches / triangle tests
     uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
     int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
     int* overflow;
@@ -167,18 +175,24 @@ constexpr uint32_t kNoSample = 0xffffffffu;
 // Window set-up: rewrites frame0 / path_frame_offset / frames of request 0
 // (single renders; the server writes its table from the host), zeroes every
 // counter, stores the window's sample count and lists every pool slot as
-// free.  Followed by one wave_regen to fill the pool.
+// pending finish at parity 0.  Followed by wave_regen(1) to fill the pool.
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s);
-// One iteration at queue parity x: extend -> shade -> connect(env) ->
-// connect(lights) -> regen.  Each stage is its own launch.
-hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStream_t s);
-// One launch per shade class present in p.class_mask.
-hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s);
-hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s);
-// Finishes the paths that ended in the last shade (writes their radiance to
-// s_rad) and starts new samples in their slots, appended to queue x.
-hipError_t wave_regen(const WaveParams& p, int x, int grid, hipStream_t s);
+// One iteration at parity y is trace(y) -> regen(y) -> shade(y):
+//   trace(y): extension rays of queue y and the shadow rays listed at parity
+//     y^1 (by the previous shade) in one launch; hits go to the shade-class
+//     lists of parity y, occluded shadow rays clear their flags.
+//   regen(y): finishes the paths listed at parity y^1 as ended with shadow
+//     rays pending and starts new samples in their slots (queue y^1).
+//   shade(y): one launch per class in p.class_mask; adds the slot's pending
+//     shadow contributions, then shades; survivors and new samples go to
+//     queue y^1, shadow rays and pending finishes are listed at parity y.
+// Counters of parity y are cleared by trace(y) for lists its iteration
+// refills (queue y^1 regions, pending finishes and shadow rays of parity y,
+// shade classes of parity y^1); fetch counters by the stage before their user.
+hipError_t wave_trace(const WaveParams& p, int y, int grid, bool count, hipStream_t s);
+hipError_t wave_shade(const WaveParams& p, int y, int grid, hipStream_t s);
+hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s);
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
 // ---- denoiser (denoise.hip) ----
 // KPCNN 10 -> 32 -> 32 -> 32 -> 32 -> 25, 3x3 convolutions.  lanes: BF16
@@ -196,8 +210,8 @@ size_t denoise_scratch_bytes(int w, int h);
 hipError_t denoise_run(const DenoiserDev& d, const float4* color, const float4* albedo, const float4* normal,
                        const float* depth, int w, int h, void* scratch, float4* out, hipStream_t s);
 
-// Largest resident workgroups per CU of the extend / connect instance for
-// stack_lds, and of shade / regen.
+// Largest resident workgroups per CU of the trace instance for stack_lds,
+// and of shade / regen.
 int wave_trace_occupancy(int stack_lds);
 int wave_shade_occupancy();
 

@@ -50,12 +50,14 @@ struct WaveEngine {
     float4 *hits = nullptr, *sh_env = nullptr, *sh_light = nullptr;
     float4 *s_rad = nullptr, *s_alb = nullptr, *s_nrm = nullptr;
     float* s_dep = nullptr;
-    uint32_t *q0 = nullptr, *q1 = nullptr, *term = nullptr, *gstack = nullptr, *cls_queue = nullptr;
+    uint32_t *q0 = nullptr, *q1 = nullptr, *gstack = nullptr, *cls_queue = nullptr;
+    uint32_t *t0 = nullptr, *t1 = nullptr;    // pending-finish lists per parity
+    uint32_t *sl0 = nullptr, *sl1 = nullptr;  // shadow-ray slot lists per parity
     uint32_t *g0 = nullptr, *g1 = nullptr;  // sample ids of the new-path regions
     int* ctr = nullptr;
     int* overflow = nullptr;
     unsigned long long* trav = nullptr;
-    int* host_counts = nullptr;  // pinned: two copies of one queue parity's region counts
+    int* host_counts = nullptr;  // pinned: two copies of the polled counters
     hipEvent_t poll[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> stage_events;
     struct Graph {

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(BvhView b, const flo
     __shared__ uint32_t lds[32 * kTraceBlock];
     int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-    Stack<32> st{lds + threadIdx.x, gstack + gt, gstride, 0};
+    Stack<32> st(lds, gstack, gstride);
     for (int64_t i = gt; i < n; i += gstride) {
         const float* r = rays + 8 * i;
         uint32_t nn = 0, nt = 0;
@@ -265,7 +265,7 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
         const int id = (int)(t / kCtrStride);
         int v = 0;
         if (t % kCtrStride == 0) {
-            if (id >= kCtrTerm && id < kCtrTerm + kShards) v = p.region;  // every slot is free
+            if (id >= kCtrTerm && id < kCtrTerm + kShards) v = p.region;  // every slot: pending finish, parity 0
             if (id == kCtrWindowTotal) v = total;
         }
         p.ctr[t] = v;
@@ -276,24 +276,47 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
         r0->frames = frames;
     }
     if (t < p.pool) {
-        p.term[t] = (uint32_t)t;
+        p.term[0][t] = (uint32_t)t;
         p.path[4 * t + 3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoSample));
     }
 }
 
-// Finish the paths that ended in the last shade, then start new samples in
-// their slots (queue y, new-path region of the shard).
+// Adds slot pid's pending shadow contributions to r in emission order (the
+// environment's, then the light's: devicePrograms.cu:376-471 add them to
+// radiance in that order as soon as each shadow ray is found unoccluded).
+OXR_D void add_pending(const WaveParams& p, uint32_t pid, float4& r) {
+    if (p.env_w > 0) {
+        const float4 c = p.sh_env[3 * (size_t)pid + 2];
+        if (c.w != 0.0f) {
+            r.x = r.x + c.x;
+            r.y = r.y + c.y;
+            r.z = r.z + c.z;
+        }
+    }
+    if (p.n_lights != 0) {
+        const float4 c = p.sh_light[3 * (size_t)pid + 2];
+        if (c.w != 0.0f) {
+            r.x = r.x + c.x;
+            r.y = r.y + c.y;
+            r.z = r.z + c.z;
+        }
+    }
+}
+
+// regen(y): finish the paths that the previous shade listed (parity y^1) as
+// ended with shadow rays pending -- their rays were traced by trace(y) --
+// then start new samples in their slots (new-path region of queue y^1).
 __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
-    if (blockIdx.x == 0) {  // counters of the next extend, idle now
-        if (threadIdx.x < kShards) *ctr_at(p.ctr, kCtrFetchExtend + threadIdx.x) = 0;
-        for (int i = threadIdx.x; i < kShadeClasses * kShards; i += blockDim.x) *ctr_at(p.ctr, kCtrClass + i) = 0;
+    if (blockIdx.x == 0) {  // fetch counters of the next trace and shade, idle now
+        if (threadIdx.x < kShards) *ctr_at(p.ctr, kCtrFetchTrace + threadIdx.x) = 0;
+        for (int i = threadIdx.x; i < kShadeClasses * kShards; i += blockDim.x) *ctr_at(p.ctr, kCtrFetchClass + i) = 0;
     }
     const int k = wave_shard();
-    const int n = *ctr_at(p.ctr, kCtrTerm + k);
+    const int n = *ctr_at(p.ctr, kCtrTerm + (y ^ 1) * kShards + k);
     const int chunks = (n + 63) >> 6;
     if (chunks == 0) return;
     int* fetch = ctr_at(p.ctr, kCtrFetchRegen + k);
-    const uint32_t* term = p.term + (size_t)k * p.region;
+    const uint32_t* term = p.term[y ^ 1] + (size_t)k * p.region;
     while (true) {
         const int j = wave_next(fetch);
         if (j >= chunks) break;
@@ -304,55 +327,84 @@ __global__ __launch_bounds__(256) void k_regen(WaveParams p, int y) {
             pid = term[slot];
             float4 r = p.path[4 * (size_t)pid + 3];
             uint32_t g = __float_as_uint(r.w);
-            if (g != kNoSample) p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
+            if (g != kNoSample) {
+                add_pending(p, pid, r);
+                p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
+            }
         }
-        restart(p, k, y, have, pid);
+        restart(p, k, y ^ 1, have, pid);
     }
 }
 
+// trace(y): the shadow rays the previous shade listed at parity y^1, then
+// the extension rays of queue y, in one launch.  Each shard's work is one
+// sequence of 64-ray chunks, shadow chunks first: a shadow ray leaving a
+// surface visits several times the nodes of an extension ray, so the long
+// rays start first and the short ones fill the waves that finish early.
+// Extension rays: hit record per queue position and the position listed
+// under its shade class (parity y).  Shadow rays: an occluded ray clears its
+// record's pending flag.
 template <bool COUNT, int S>
-__global__ __launch_bounds__(kTraceBlock, 8) void k_extend(WaveParams p, int x) {
+__global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
     uint32_t* gstack = p.gstack;
     int* overflow = p.overflow;
     __shared__ uint32_t lds[S * kTraceBlock];
     if (blockIdx.x == 0) {
-        // counters of the stages after this one, all idle now
+        // lists this iteration refills, whose readers have finished
         const int t = threadIdx.x;
-        if (t < 2 * kShards) *ctr_at(p.ctr, kCtrQueue + (x ^ 1) * 2 * kShards + t) = 0;
+        if (t < 2 * kShards) *ctr_at(p.ctr, kCtrQueue + (y ^ 1) * 2 * kShards + t) = 0;
         if (t < kShards) {
-            *ctr_at(p.ctr, kCtrShEnv + t) = 0;
-            *ctr_at(p.ctr, kCtrShLight + t) = 0;
-            *ctr_at(p.ctr, kCtrTerm + t) = 0;
-            *ctr_at(p.ctr, kCtrFetchShade + t) = 0;
-            *ctr_at(p.ctr, kCtrFetchConnEnv + t) = 0;
-            *ctr_at(p.ctr, kCtrFetchConnLight + t) = 0;
+            *ctr_at(p.ctr, kCtrTerm + y * kShards + t) = 0;
+            *ctr_at(p.ctr, kCtrShEnv + y * kShards + t) = 0;
+            *ctr_at(p.ctr, kCtrShLight + y * kShards + t) = 0;
             *ctr_at(p.ctr, kCtrFetchRegen + t) = 0;
         }
-        for (int i = t; i < kShadeClasses * kShards; i += blockDim.x) *ctr_at(p.ctr, kCtrFetchClass + i) = 0;
+        for (int i = t; i < kShadeClasses * kShards; i += blockDim.x)
+            *ctr_at(p.ctr, kCtrClass + ((y ^ 1) * kShadeClasses) * kShards + i) = 0;
         if (t == 0) {
-            unsigned long long rays = 0;
-            for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, kCtrQueue + x * 2 * kShards + r);
+            unsigned long long rays = 0, shadow = 0;
+            for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + r);
+            for (int r = 0; r < kShards; r++)
+                shadow += (unsigned)*ctr_at(p.ctr, kCtrShEnv + (y ^ 1) * kShards + r) +
+                          (unsigned)*ctr_at(p.ctr, kCtrShLight + (y ^ 1) * kShards + r);
             if (rays) atomicAdd(&p.traversal[2], rays);
+            if (shadow) atomicAdd(&p.traversal[3], shadow);
         }
     }
     const int k = wave_shard();
-    const ShardQueue sq(p.ctr, x, k);
-    if (sq.chunks == 0) return;
-    int* fetch = ctr_at(p.ctr, kCtrFetchExtend + k);
-    const uint32_t* q = p.queue[x];
-    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
-    uint32_t nn = 0, nt = 0;
+    const ShardQueue sq(p.ctr, y, k);
+    const int n_env = *ctr_at(p.ctr, kCtrShEnv + (y ^ 1) * kShards + k);
+    const int n_light = *ctr_at(p.ctr, kCtrShLight + (y ^ 1) * kShards + k);
+    const int c_env = (n_env + 63) >> 6, c_shadow = c_env + ((n_light + 63) >> 6);
+    const int chunks = c_shadow + sq.chunks;
+    if (chunks == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchTrace + k);
+    const uint32_t* q = p.queue[y];
+    const uint32_t* shl = p.sh_list[y ^ 1] + (size_t)k * p.region;
+    Stack<S> st(lds, gstack, (int64_t)gridDim.x * blockDim.x);
+    uint32_t nn = 0, nt = 0, sn = 0, stt = 0;
     while (true) {
         const int j = wave_next(fetch);
-        if (j >= sq.chunks) break;
-        const int64_t qp = sq.position(j, k, p.region);
+        if (j >= chunks) break;
+        if (j < c_shadow) {
+            const bool light = j >= c_env;
+            const int slot = ((light ? j - c_env : j) << 6) + lane_id();
+            if (slot < (light ? n_light : n_env)) {
+                const uint32_t pid = shl[(light ? (size_t)p.pool : 0) + slot];
+                float4* rec = (light ? p.sh_light : p.sh_env) + 3 * (size_t)pid;
+                const float4 a = rec[0], b = rec[1];
+                Hit h = traverse<true, COUNT>(p.bvh, xyz(a), xyz(b), kEps, a.w, st, sn, stt, overflow);
+                if (h.leaf != kNoHit) reinterpret_cast<float*>(rec + 2)[3] = 0.0f;
+            }
+            continue;
+        }
+        const int64_t qp = sq.position(j - c_shadow, k, p.region);
         int cls = -1;
         if (qp >= 0) {
             f3 org, dir;
             if (is_fresh(p, qp)) {
                 uint32_t rs;
-                camera_ray(p, fresh_sample(p, x, qp), org, dir, rs);
+                camera_ray(p, fresh_sample(p, y, qp), org, dir, rs);
             } else {
                 const uint32_t pid = q[qp];
                 org = xyz(p.path[4 * (size_t)pid + 0]);
@@ -368,7 +420,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_extend(WaveParams p, int x) 
         while (left) {
             const int c = __shfl(cls, (int)__builtin_ctzll(left));
             const bool mine = cls == c;
-            const int o = wave_append(ctr_at(p.ctr, kCtrClass + c * kShards + k), mine);
+            const int o = wave_append(ctr_at(p.ctr, kCtrClass + (y * kShadeClasses + c) * kShards + k), mine);
             if (mine) p.cls_queue[((size_t)c * kShards + k) * p.region + o] = (uint32_t)qp;
             left &= ~__ballot(mine);
         }
@@ -376,50 +428,8 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_extend(WaveParams p, int x) 
     if (COUNT) {
         wave_count_add(&p.traversal[0], nn);
         wave_count_add(&p.traversal[1], nt);
-    }
-}
-
-template <bool COUNT, int S>
-__global__ __launch_bounds__(kTraceBlock) void k_connect(WaveParams p, int which) {
-    uint32_t* gstack = p.gstack;
-    int* overflow = p.overflow;
-    __shared__ uint32_t lds[S * kTraceBlock];
-    const int cnt_id = which == 0 ? kCtrShEnv : kCtrShLight;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        unsigned long long rays = 0;
-        for (int r = 0; r < kShards; r++) rays += (unsigned)*ctr_at(p.ctr, cnt_id + r);
-        if (rays) atomicAdd(&p.traversal[3], rays);
-    }
-    const int k = wave_shard();
-    const int count = *ctr_at(p.ctr, cnt_id + k);
-    const int chunks = (count + 63) >> 6;
-    if (chunks == 0) return;
-    const float4* rec = (which == 0 ? p.sh_env : p.sh_light) + 3 * (size_t)k * p.region;
-    int* fetch = ctr_at(p.ctr, (which == 0 ? kCtrFetchConnEnv : kCtrFetchConnLight) + k);
-    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    Stack<S> st{lds + threadIdx.x, gstack + gt, (int64_t)gridDim.x * blockDim.x, 0};
-    uint32_t nn = 0, nt = 0;
-    while (true) {
-        const int j = wave_next(fetch);
-        if (j >= chunks) break;
-        const int slot = (j << 6) + lane_id();
-        if (slot < count) {
-            float4 a = rec[3 * (size_t)slot], b = rec[3 * (size_t)slot + 1];
-            Hit h = traverse<true, COUNT>(p.bvh, xyz(a), xyz(b), kEps, a.w, st, nn, nt, overflow);
-            if (h.leaf == kNoHit) {
-                float4 c = rec[3 * (size_t)slot + 2];
-                uint32_t pid = __float_as_uint(b.w);
-                float4 r = p.path[4 * (size_t)pid + 3];
-                r.x = r.x + c.x;
-                r.y = r.y + c.y;
-                r.z = r.z + c.z;
-                p.path[4 * (size_t)pid + 3] = r;
-            }
-        }
-    }
-    if (COUNT) {
-        wave_count_add(&p.traversal[4], nn);
-        wave_count_add(&p.traversal[5], nt);
+        wave_count_add(&p.traversal[4], sn);
+        wave_count_add(&p.traversal[5], stt);
     }
 }
 
@@ -448,8 +458,9 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 
 // One bounce of __raygen__renderFrame's loop body (devicePrograms.cu:270-496)
 // for one path, after its radiance ray was traced.  Returns whether the path
-// continues; NEE shadow rays are returned for the connect stage with their
-// contribution precomputed (it is added only if the ray is unoccluded).
+// continues; NEE shadow rays are returned for the next trace with their
+// contribution precomputed (it is added only if the ray is unoccluded, by the
+// slot's next shade or by k_regen).
 // MT: the material type of the hit surface, fixed by the shade class; < 0 for
 // rays that left the scene.  MEDIA: the scene has participating media (the
 // medium code is compiled only then).
@@ -475,7 +486,9 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         g = g_fresh;
     } else {
         const float4 ro = p.path[4 * (size_t)pid + 0], rd = p.path[4 * (size_t)pid + 1];
-        const float4 th = p.path[4 * (size_t)pid + 2], rad4 = p.path[4 * (size_t)pid + 3];
+        const float4 th = p.path[4 * (size_t)pid + 2];
+        float4 rad4 = p.path[4 * (size_t)pid + 3];
+        add_pending(p, pid, rad4);  // the last bounce's unoccluded shadow rays
         org = xyz(ro);
         dir = xyz(rd);
         pre_pdf = ro.w;
@@ -696,23 +709,31 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     return cont;
 }
 
-OXR_D void emit_shadow(float4* rec, int* ctr, const ShadowOut& s, uint32_t pid) {
-    int slot = wave_append(ctr, s.emit);
-    if (slot >= 0) {
-        rec[3 * (size_t)slot] = make_float4(s.o.x, s.o.y, s.o.z, s.tmax);
-        rec[3 * (size_t)slot + 1] = make_float4(s.d.x, s.d.y, s.d.z, __uint_as_float(pid));
-        rec[3 * (size_t)slot + 2] = make_float4(s.c.x, s.c.y, s.c.z, 0.0f);
+// Writes slot pid's shadow record of one kind and lists the slot for the
+// next trace.  `pending`: the path is not final, so its next shade or k_regen
+// reads the record's flag, which must then say whether a ray was emitted.
+// All lanes of the wave must call it.
+OXR_D void emit_shadow(float4* recs, uint32_t* list, int* ctr, const ShadowOut& s, uint32_t pid, bool pending) {
+    const int slot = wave_append(ctr, s.emit);
+    if (slot >= 0) list[slot] = pid;
+    float4* rec = recs + 3 * (size_t)pid;
+    if (s.emit) {
+        rec[0] = make_float4(s.o.x, s.o.y, s.o.z, s.tmax);
+        rec[1] = make_float4(s.d.x, s.d.y, s.d.z, 0.0f);
+        rec[2] = make_float4(s.c.x, s.c.y, s.c.z, 1.0f);
+    } else if (pending) {
+        reinterpret_cast<float*>(rec + 2)[3] = 0.0f;
     }
 }
 
 // One shade class: the paths of queue x whose extension ray left the scene
-// (CLS 0) or hit a surface of material type CLS - 1, listed by k_extend in
+// (CLS 0) or hit a surface of material type CLS - 1, listed by k_trace in
 // cls_queue.  Each class compiles only its own BSDF, so the common case (a
 // ray that left the scene) runs at full occupancy.
 template <int CLS, bool MEDIA>
 __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
     const int k = wave_shard();
-    const int n = *ctr_at(p.ctr, kCtrClass + CLS * kShards + k);
+    const int n = *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + CLS) * kShards + k);
     const int chunks = (n + 63) >> 6;
     if (chunks == 0) return;
     int* fetch = ctr_at(p.ctr, kCtrFetchClass + CLS * kShards + k);
@@ -721,12 +742,12 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
     const size_t rbase = (size_t)k * p.region;
     uint32_t* qo = p.queue[x ^ 1] + rbase;  // survivors region of shard k
     int* qo_count = ctr_at(p.ctr, kCtrQueue + (x ^ 1) * 2 * kShards + k);
-    uint32_t* term = p.term + rbase;
-    int* term_count = ctr_at(p.ctr, kCtrTerm + k);
-    float4* se = p.sh_env + 3 * rbase;
-    float4* sl = p.sh_light + 3 * rbase;
-    int* se_count = ctr_at(p.ctr, kCtrShEnv + k);
-    int* sl_count = ctr_at(p.ctr, kCtrShLight + k);
+    uint32_t* term = p.term[x] + rbase;
+    int* term_count = ctr_at(p.ctr, kCtrTerm + x * kShards + k);
+    uint32_t* le = p.sh_list[x] + rbase;
+    uint32_t* ll = p.sh_list[x] + p.pool + rbase;
+    int* se_count = ctr_at(p.ctr, kCtrShEnv + x * kShards + k);
+    int* sl_count = ctr_at(p.ctr, kCtrShLight + x * kShards + k);
     Lut lut{p.lut_bsdf, p.lut_albedo};
     while (true) {
         const int j = wave_next(fetch);
@@ -747,8 +768,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
-        // once.  One that still waits for a connect keeps its radiance in the
-        // slot record for connect to add to; k_regen finishes it.
+        // once.  One whose shadow rays are still to be traced keeps its
+        // radiance in the slot record; k_regen adds them and finishes it.
         const bool ended = active && !cont;
         const bool final = ended && !sev.emit && !slv.emit;
         if (final) p.s_rad[__float_as_uint(rad.w)] = make_float4(rad.x, rad.y, rad.z, 0.0f);
@@ -757,8 +778,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         if (o >= 0) qo[o] = pid;
         o = wave_append(term_count, ended && !final);
         if (o >= 0) term[o] = pid;
-        emit_shadow(se, se_count, sev, pid);
-        emit_shadow(sl, sl_count, slv, pid);
+        const bool pending = active && !final;
+        if (p.env_w > 0) emit_shadow(p.sh_env, le, se_count, sev, pid, pending);
+        if (p.n_lights != 0) emit_shadow(p.sh_light, ll, sl_count, slv, pid, pending);
         restart(p, k, x ^ 1, final, pid);
     }
 }
@@ -871,21 +893,15 @@ hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s) {
 }
 
 template <bool C>
-void launch_extend(const WaveParams& p, int x, int grid, hipStream_t s) {
-    if (p.stack_lds <= 8) k_extend<C, 8><<<grid, kTraceBlock, 0, s>>>(p, x);
-    else if (p.stack_lds <= 16) k_extend<C, 16><<<grid, kTraceBlock, 0, s>>>(p, x);
-    else k_extend<C, 32><<<grid, kTraceBlock, 0, s>>>(p, x);
-}
-template <bool C>
-void launch_connect(const WaveParams& p, int which, int grid, hipStream_t s) {
-    if (p.stack_lds <= 8) k_connect<C, 8><<<grid, kTraceBlock, 0, s>>>(p, which);
-    else if (p.stack_lds <= 16) k_connect<C, 16><<<grid, kTraceBlock, 0, s>>>(p, which);
-    else k_connect<C, 32><<<grid, kTraceBlock, 0, s>>>(p, which);
+void launch_trace(const WaveParams& p, int y, int grid, hipStream_t s) {
+    if (p.stack_lds <= 8) k_trace<C, 8><<<grid, kTraceBlock, 0, s>>>(p, y);
+    else if (p.stack_lds <= 16) k_trace<C, 16><<<grid, kTraceBlock, 0, s>>>(p, y);
+    else k_trace<C, 32><<<grid, kTraceBlock, 0, s>>>(p, y);
 }
 
-hipError_t wave_extend(const WaveParams& p, int x, int grid, bool count, hipStream_t s) {
-    if (count) launch_extend<true>(p, x, grid, s);
-    else launch_extend<false>(p, x, grid, s);
+hipError_t wave_trace(const WaveParams& p, int y, int grid, bool count, hipStream_t s) {
+    if (count) launch_trace<true>(p, y, grid, s);
+    else launch_trace<false>(p, y, grid, s);
     return hipGetLastError();
 }
 
@@ -909,12 +925,6 @@ hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t wave_connect(const WaveParams& p, int which, int grid, bool count, hipStream_t s) {
-    if (count) launch_connect<true>(p, which, grid, s);
-    else launch_connect<false>(p, which, grid, s);
-    return hipGetLastError();
-}
-
 template <class K>
 int occupancy(K kernel, int block) {
     int n = 0;
@@ -923,11 +933,9 @@ int occupancy(K kernel, int block) {
 }
 
 int wave_trace_occupancy(int stack_lds) {
-    if (stack_lds <= 8)
-        return std::min(occupancy(k_extend<false, 8>, kTraceBlock), occupancy(k_connect<false, 8>, kTraceBlock));
-    if (stack_lds <= 16)
-        return std::min(occupancy(k_extend<false, 16>, kTraceBlock), occupancy(k_connect<false, 16>, kTraceBlock));
-    return std::min(occupancy(k_extend<false, 32>, kTraceBlock), occupancy(k_connect<false, 32>, kTraceBlock));
+    if (stack_lds <= 8) return occupancy(k_trace<false, 8>, kTraceBlock);
+    if (stack_lds <= 16) return occupancy(k_trace<false, 16>, kTraceBlock);
+    return occupancy(k_trace<false, 32>, kTraceBlock);
 }
 
 int wave_shade_occupancy() {

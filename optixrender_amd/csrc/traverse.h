@@ -20,8 +20,11 @@ namespace oxr {
 
 struct RayPre {
     f3 o, inv;
-    int kx, ky, kz;
+    uint32_t axes;  // kx | ky << 2 | kz << 4: one register instead of three
     float Sx, Sy, Sz;
+    OXR_D int kx() const { return (int)(axes & 3u); }
+    OXR_D int ky() const { return (int)((axes >> 2) & 3u); }
+    OXR_D int kz() const { return (int)((axes >> 4) & 3u); }
 };
 
 OXR_D float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
@@ -39,7 +42,7 @@ OXR_D RayPre prep_ray(f3 o, f3 d) {
     int kx = kz + 1 == 3 ? 0 : kz + 1;
     int ky = kx + 1 == 3 ? 0 : kx + 1;
     if (comp(d, kz) < 0.0f) { int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.axes = (uint32_t)kx | (uint32_t)ky << 2 | (uint32_t)kz << 4;
     float dz = comp(d, kz);
     r.Sx = comp(d, kx) / dz;
     r.Sy = comp(d, ky) / dz;
@@ -49,10 +52,11 @@ OXR_D RayPre prep_ray(f3 o, f3 d) {
 
 OXR_D bool tri_test(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float& t, float& u, float& v) {
     f3 A = xyz(a) - r.o, B = xyz(b) - r.o, C = xyz(c) - r.o;
-    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-    float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
-    float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
-    float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+    const int kx = r.kx(), ky = r.ky(), kz = r.kz();
+    float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+    float Ax = comp(A, kx) - r.Sx * Akz, Ay = comp(A, ky) - r.Sy * Akz;
+    float Bx = comp(B, kx) - r.Sx * Bkz, By = comp(B, ky) - r.Sy * Bkz;
+    float Cx = comp(C, kx) - r.Sx * Ckz, Cy = comp(C, ky) - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;
@@ -81,23 +85,50 @@ struct Hit {
     float u, v;
 };
 
-// S entries per lane in LDS, the remaining kStackTotal - S in HBM.
+// A pointer into LDS.  Kept in its own address space so that the LDS and
+// HBM halves of the stack stay distinct loads: with two generic pointers the
+// compiler folds pop() into one flat load of a selected address, which
+// waits on every outstanding memory operation of the wave.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// This lane's index in its wave, computed afresh at every use: the
+// traversal loop runs at the 64-VGPR limit of 8 waves per SIMD, and a lane
+// index (or a per-lane stack pointer) kept live across it gets spilled and
+// reloaded from scratch around every stack access.  Two ALU instructions,
+// no memory access.
+OXR_D uint32_t lane_now() {
+    uint32_t l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
+// S entries per lane in LDS, the remaining kStackTotal - S in HBM.  Only
+// wave-uniform bases are kept (in scalar registers); the lane's addresses are
+// rebuilt from lane_now() at each access.
 template <int S>
 struct Stack {
-    uint32_t* lds;   // this lane's first entry; entries kTraceBlock apart
-    uint32_t* gbl;   // this lane's first spill entry; entries gstride apart
+    lds_u32* lds;       // the block's array: entry i of block lane t at i * kTraceBlock + t
+    uint32_t* gbase;    // spill area: entry i of grid lane g at g + i * gstride
+    uint32_t wave_lds;  // block lane of the wave's first lane
+    uint32_t wave_glb;  // grid lane of the wave's first lane
     int64_t gstride;
     int sp;
+    OXR_D Stack(uint32_t* block_lds, uint32_t* gbase_, int64_t gstride_)
+        : lds((lds_u32*)block_lds), gbase(gbase_), gstride(gstride_), sp(0) {
+        wave_lds = (uint32_t)__builtin_amdgcn_readfirstlane((int)threadIdx.x);
+        wave_glb = blockIdx.x * blockDim.x + wave_lds;
+    }
     OXR_D bool push(uint32_t x) {
-        if (sp < S) lds[sp * kTraceBlock] = x;
-        else if (sp < kStackTotal) gbl[(int64_t)(sp - S) * gstride] = x;
+        if (sp < S) lds[sp * kTraceBlock + wave_lds + lane_now()] = x;
+        else if (sp < kStackTotal) gbase[(wave_glb + lane_now()) + (int64_t)(sp - S) * gstride] = x;
         else return false;
         sp++;
         return true;
     }
     OXR_D uint32_t pop() {
         sp--;
-        return sp < S ? lds[sp * kTraceBlock] : gbl[(int64_t)(sp - S) * gstride];
+        if (sp < S) return lds[sp * kTraceBlock + wave_lds + lane_now()];
+        return gbase[(wave_glb + lane_now()) + (int64_t)(sp - S) * gstride];
     }
 };
 
@@ -131,7 +162,7 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
     uint32_t node = 0;
     // A tree traversal visits each internal node at most once; the cap only
     // guards against a malformed hierarchy.
-    int64_t budget = b.n + 64;
+    int budget = (int)b.n + 64;
     while (true) {
         if (--budget < 0 || node >= (uint32_t)(b.n - 1)) { *overflow = 2; break; }
         const float4* pr = b.pairs + 4 * (size_t)node;

@@ -5,8 +5,8 @@
 //
 // The number of iterations a window needs is data dependent (Russian
 // roulette, escapes), so the host replays a graph of `iters_per_poll`
-// iterations and, after each, copies the live-path counts of the queue that
-// the next iteration would read into pinned memory.  Polls run one graph
+// iterations and, after each, copies the live-path counts (the queue the
+// next iteration would trace, the paths it would finish) into pinned memory.  Polls run one graph
 // behind the device: the check of replay i overlaps replay i + 1, and the
 // window ends at the first replay that leaves no live path (a following
 // replay, already queued, finds empty queues and returns at once).
@@ -30,7 +30,7 @@ void wfree(T*& p) {
     p = nullptr;
 }
 
-constexpr size_t kPollInts = (size_t)2 * kShards * kCtrStride;  // region counts of one queue parity
+constexpr size_t kPollInts = (size_t)kPollIds * kCtrStride;
 
 }  // namespace
 
@@ -40,9 +40,9 @@ int64_t wave_round_pool(int64_t paths) {
 }
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
-    // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, term 4, shadow records
-    // 2x48, shade classes 9x4
-    return (size_t)pool * (64 + 32 + 16 + 8 + 4 + 96 + 36) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
+    // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, pending-finish lists 2x4,
+    // shadow records 2x48, shadow lists 2x2x4, shade classes 9x4
+    return (size_t)pool * (64 + 32 + 16 + 8 + 8 + 96 + 16 + 36) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
            ctr_ints() * 4 + 4096;
 }
 
@@ -55,7 +55,8 @@ void wave_drop_graphs(WaveEngine& e) {
 void wave_release(WaveEngine& e) {
     wave_drop_graphs(e);
     wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
-    wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.gstack);
+    wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.gstack);
+    wfree(e.t0); wfree(e.t1); wfree(e.sl0); wfree(e.sl1);
     wfree(e.ctr); wfree(e.overflow); wfree(e.trav);
     if (e.host_counts) (void)hipHostFree(e.host_counts);
     e.host_counts = nullptr;
@@ -83,13 +84,15 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
     }
     if (pool > e.pool) {
         wave_drop_graphs(e);
-        wfree(e.path); wfree(e.hits); wfree(e.sh_env);
-        wfree(e.sh_light); wfree(e.q0); wfree(e.q1); wfree(e.term); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
+        wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light); wfree(e.q0); wfree(e.q1);
+        wfree(e.t0); wfree(e.t1); wfree(e.sl0); wfree(e.sl1); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
         e.pool = 0;
         if ((r = walloc(&e.path, 4 * pool)) != hipSuccess ||
             (r = walloc(&e.hits, 2 * pool)) != hipSuccess || (r = walloc(&e.sh_env, 3 * pool)) != hipSuccess ||
             (r = walloc(&e.sh_light, 3 * pool)) != hipSuccess || (r = walloc(&e.q0, 2 * pool)) != hipSuccess ||
-            (r = walloc(&e.q1, 2 * pool)) != hipSuccess || (r = walloc(&e.term, pool)) != hipSuccess ||
+            (r = walloc(&e.q1, 2 * pool)) != hipSuccess || (r = walloc(&e.t0, pool)) != hipSuccess ||
+            (r = walloc(&e.t1, pool)) != hipSuccess || (r = walloc(&e.sl0, 2 * pool)) != hipSuccess ||
+            (r = walloc(&e.sl1, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.cls_queue, (size_t)kShadeClasses * pool)) != hipSuccess ||
             (r = walloc(&e.g0, pool)) != hipSuccess || (r = walloc(&e.g1, pool)) != hipSuccess)
             return r;
@@ -120,7 +123,9 @@ void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p) {
     p.path = e.path;
     p.queue[0] = e.q0; p.queue[1] = e.q1;
     p.fresh_g[0] = e.g0; p.fresh_g[1] = e.g1;
-    p.hits = e.hits; p.sh_env = e.sh_env; p.sh_light = e.sh_light; p.term = e.term; p.cls_queue = e.cls_queue;
+    p.hits = e.hits; p.sh_env = e.sh_env; p.sh_light = e.sh_light; p.cls_queue = e.cls_queue;
+    p.term[0] = e.t0; p.term[1] = e.t1;
+    p.sh_list[0] = e.sl0; p.sh_list[1] = e.sl1;
     p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm; p.s_dep = e.s_dep;
     p.ctr = e.ctr; p.traversal = e.trav; p.gstack = e.gstack; p.overflow = e.overflow;
 }
@@ -138,7 +143,8 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     const bool eager = r.eager || timing;
     r.launches = 0;
     r.iterations = 0;
-    // stage ids: 0 regen (window start included), 1 extend, 2 shade, 3 connect, 4 accumulate.
+    // stage ids: 0 regen (window start included), 1 trace (extension and shadow rays), 2 shade,
+    // 4 accumulate.
     // With stage timing an event follows every stage; all are read once the
     // window has finished, so timing does not stall the pipelined polls.
     std::vector<int> stages;
@@ -168,17 +174,14 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     auto iterations = [&]() -> hipError_t {
         hipError_t x;
         for (int it = 0; it < K; it++) {
-            const int q = it & 1;
-            if ((x = wave_extend(p, q, r.trace_grid, r.count, s)) != hipSuccess || (x = mark(1)) != hipSuccess) return x;
-            if ((x = wave_shade(p, q, r.shade_grid, s)) != hipSuccess || (x = mark(2)) != hipSuccess) return x;
-            if (p.env_w > 0 && (x = wave_connect(p, 0, r.trace_grid, r.count, s)) != hipSuccess) return x;
-            if (p.n_lights > 0 && (x = wave_connect(p, 1, r.trace_grid, r.count, s)) != hipSuccess) return x;
-            if ((x = mark(3)) != hipSuccess) return x;
-            if ((x = wave_regen(p, q ^ 1, r.shade_grid, s)) != hipSuccess || (x = mark(0)) != hipSuccess) return x;
+            const int y = it & 1;
+            if ((x = wave_trace(p, y, r.trace_grid, r.count, s)) != hipSuccess || (x = mark(1)) != hipSuccess) return x;
+            if ((x = wave_regen(p, y, r.shade_grid, s)) != hipSuccess || (x = mark(0)) != hipSuccess) return x;
+            if ((x = wave_shade(p, y, r.shade_grid, s)) != hipSuccess || (x = mark(2)) != hipSuccess) return x;
         }
         return hipSuccess;
     };
-    const int per_iter = 2 + __builtin_popcount(p.class_mask) + (p.env_w > 0) + (p.n_lights > 0);
+    const int per_iter = 2 + __builtin_popcount(p.class_mask);
 
     if (timing) {
         if (e.stage_events.empty()) {
@@ -190,7 +193,7 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     }
     if ((err = wave_window_init(p, r.set_req0, r.frame0, r.path_frame_offset, r.frames, r.total_samples, s)) !=
             hipSuccess ||
-        (err = wave_regen(p, 0, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
+        (err = wave_regen(p, 1, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
         return err;
     r.launches += 2;
 
@@ -218,10 +221,11 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
             e.graphs.push_back(std::move(g));
         }
     }
-    // every path ends within max_depth bounces and every iteration refills the
-    // slots freed by the one before, so this many replays always suffice
+    // every path ends within max_depth bounces plus the iteration that
+    // finishes it, and every iteration refills the slots freed by the one
+    // before, so this many replays always suffice
     const int64_t max_replays =
-        ((r.total_samples + p.pool - 1) / p.pool + 2) * (int64_t)(p.max_depth + 1) / K + 2;
+        ((r.total_samples + p.pool - 1) / p.pool + 2) * (int64_t)(p.max_depth + 2) / K + 2;
     auto replay = [&](int slot) -> hipError_t {
         hipError_t x = exec ? hipGraphLaunch(exec, s) : iterations();
         if (x != hipSuccess) return x;
@@ -233,10 +237,13 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
             return x;
         return hipEventRecord(e.poll[slot], s);
     };
+    // after an even number of iterations the next one is at parity 0: it
+    // traces queue 0 and finishes the paths listed at parity 1
     auto live = [&](int slot) {
         const int* src = e.host_counts + slot * kPollInts;
         int64_t n = 0;
-        for (int q = 0; q < 2 * kShards; q++) n += src[(size_t)q * kCtrStride];
+        for (int q = 0; q < 2 * kShards; q++) n += src[(size_t)(kCtrQueue + q) * kCtrStride];
+        for (int k = 0; k < kShards; k++) n += src[(size_t)(kCtrTerm + kShards + k) * kCtrStride];
         return n;
     };
     if ((err = replay(0)) != hipSuccess) return err;

@@ -97,7 +97,7 @@ def test_stage_timing_and_counts(ctx, cornell):
     # box test, so whole-render totals agree to 1e-3.
     for k, v in ref.items():
         assert abs(got[k] - v) <= 1e-3 * v + 2, k
-    assert st.extend_ms > 0 and st.shade_ms > 0
+    assert st.trace_ms > 0 and st.shade_ms > 0
 
 
 # (stack_lds, trace blocks/CU, shade blocks/CU, iterations per poll, pool slots, window samples).
