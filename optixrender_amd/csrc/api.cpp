@@ -390,7 +390,7 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
     const int64_t n = d->n_tris;
     s->n = n;
     s->n_mats = d->n_materials;
-    s->class_mask = 1u;  // rays that leave the scene
+    s->class_mask = 1u | 1u << kClassCameraMiss;  // rays that leave the scene
     for (int i = 0; i < d->n_materials; i++) s->class_mask |= 1u << (1 + d->materials[i].type);
     s->n_lights = d->n_lights;
     s->n_media = d->n_media;

@@ -106,14 +106,18 @@ enum CtrId {
     kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
     kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
     kCtrClass = kCtrWindowTotal + 1,           // [parity][class][shard]: entries of the shade-class queues
-    kCtrFetchClass = kCtrClass + 2 * 9 * kShards,  // [class][shard]: chunks taken by the shade kernels
-    kCtrIds = kCtrFetchClass + 9 * kShards
+    kCtrFetchClass = kCtrClass + 2 * 10 * kShards, // [class][shard]: chunks taken by the shade kernels
+    kCtrIds = kCtrFetchClass + 10 * kShards
 };
 // The host's completion poll copies counters [kCtrQueue, kPollIds): the
 // queue and pending-finish counts of both parities.
 constexpr int kPollIds = kCtrTerm + 2 * kShards;
 constexpr size_t ctr_ints() { return (size_t)kCtrIds * kCtrStride; }
-constexpr int kShadeClasses = 9;
+// Shade classes: 0 = a path past bounce 0 whose ray left the scene, 1 + t =
+// the ray hit a surface of material type t, 9 = a camera ray (bounce 0)
+// that left the scene.
+constexpr int kShadeClasses = 10;
+constexpr int kClassCameraMiss = 9;
 
 // Wavefront integrator state for one context.
 struct WaveParams {
@@ -153,8 +157,8 @@ struct WaveParams {
     float4* sh_light;
     uint32_t* sh_list[2];  // [parity]: slot ids with an env ray in kShards regions, then with a light ray
     uint32_t* term[2];     // [parity]: kShards regions of `region` slot ids
-    // shade classes: 0 = the ray left the scene, 1 + t = it hit a surface of
-    // material type t.  cls_queue[class][shard] holds `region` queue positions.
+    // shade classes (kShadeClasses): cls_queue[class][shard] holds `region`
+    // queue positions.
     uint32_t* cls_queue;
     uint32_t class_mask;  // classes the scene can produce (bit c): the shade launches
     // per-sample outputs of the window
@@ -174,10 +178,11 @@ constexpr uint32_t kNoSample = 0xffffffffu;
 
 // Window set-up: rewrites frame0 / path_frame_offset / frames of request 0
 // (single renders; the server writes its table from the host), zeroes every
-// counter, stores the window's sample count and lists every pool slot as
-// pending finish at parity 0.  Followed by wave_regen(1) to fill the pool.
+// counter and stores the window's sample count.  Followed by wave_fill, which
+// starts a sample in every slot (queue 0).  grid: any wave_grid size.
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s);
+hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s);
 // One iteration at parity y is trace(y) -> regen(y) -> shade(y):
 //   trace(y): extension rays of queue y and the shadow rays listed at parity
 //     y^1 (by the previous shade) in one launch; hits go to the shade-class

@@ -20,7 +20,7 @@ struct SceneDev {
     uint32_t* mat_sorted = nullptr;
     oxr_material* mats = nullptr;
     int n_mats = 0;
-    uint32_t class_mask = 1;  // shade classes: bit 0 rays that leave the scene, bit 1 + t material type t
+    uint32_t class_mask = 1;  // shade classes the scene can produce (bit c for class c, kShadeClasses)
     oxr_light* lights = nullptr;
     int n_lights = 0;
     oxr_medium* media = nullptr;

@@ -265,7 +265,8 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
         const int id = (int)(t / kCtrStride);
         int v = 0;
         if (t % kCtrStride == 0) {
-            if (id >= kCtrTerm && id < kCtrTerm + kShards) v = p.region;  // every slot: pending finish, parity 0
+            // k_fill draws positions 0 .. region-1 of every shard's sequence
+            if (id >= kCtrSamplePos && id < kCtrSamplePos + kShards) v = p.region;
             if (id == kCtrWindowTotal) v = total;
         }
         p.ctr[t] = v;
@@ -275,10 +276,35 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
         r0->path_frame_offset = offset;
         r0->frames = frames;
     }
-    if (t < p.pool) {
-        p.term[0][t] = (uint32_t)t;
-        p.path[4 * t + 3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoSample));
+}
+
+// Window start: slot s of shard k takes position s of the shard's draw
+// sequence and, if that sample exists, queue position s of the new-path
+// region of queue 0.  sample_of grows with the position, so the started slots
+// are a prefix of the shard and the queue stays dense; each wave adds its
+// count to the shard's queue counter once.  Waves take the shard's 64-slot
+// chunks in a fixed stride, with no fetch counter.
+__global__ __launch_bounds__(256) void k_fill(WaveParams p) {
+    const int k = wave_shard();
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
+    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
+    const int chunks = p.region >> 6;  // region is a multiple of 64
+    const size_t rbase = (size_t)k * p.region;
+    uint32_t* qn = p.queue[0] + (size_t)(kShards + k) * p.region;
+    uint32_t* gn = p.fresh_g[0] + rbase;
+    int started = 0;
+    for (int j = wave / kShards; j < chunks; j += stride) {
+        const int slot = (j << 6) + lane_id();
+        const int64_t g = sample_of(k, slot);
+        const bool st = g < total;
+        if (st) {
+            qn[slot] = (uint32_t)(rbase + slot);
+            gn[slot] = (uint32_t)g;
+        }
+        started += (int)__popcll(__ballot(st));
     }
+    if (lane_id() == 0 && started) atomicAdd(ctr_at(p.ctr, kCtrQueue + kShards + k), started);
 }
 
 // Adds slot pid's pending shadow contributions to r in emission order (the
@@ -402,7 +428,8 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         int cls = -1;
         if (qp >= 0) {
             f3 org, dir;
-            if (is_fresh(p, qp)) {
+            const bool fresh = is_fresh(p, qp);
+            if (fresh) {
                 uint32_t rs;
                 camera_ray(p, fresh_sample(p, y, qp), org, dir, rs);
             } else {
@@ -412,7 +439,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
             }
             Hit h = traverse<false, COUNT>(p.bvh, org, dir, kEps, kFltMax, st, nn, nt, overflow);
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
-            cls = h.leaf == kNoHit ? 0 : 1 + p.mats[p.mat_ids[h.leaf]].type;
+            cls = h.leaf != kNoHit ? 1 + p.mats[p.mat_ids[h.leaf]].type : fresh ? kClassCameraMiss : 0;
         }
         // list the queue position under its shade class: one append per
         // class present in the wave
@@ -727,11 +754,13 @@ OXR_D void emit_shadow(float4* recs, uint32_t* list, int* ctr, const ShadowOut& 
 }
 
 // One shade class: the paths of queue x whose extension ray left the scene
-// (CLS 0) or hit a surface of material type CLS - 1, listed by k_trace in
-// cls_queue.  Each class compiles only its own BSDF, so the common case (a
-// ray that left the scene) runs at full occupancy.
+// (CLS 0, past bounce 0; CLS kClassCameraMiss, camera rays) or hit a surface
+// of material type CLS - 1, listed by k_trace in cls_queue.  Each class
+// compiles only its own BSDF, and the two miss classes know whether the path
+// is at bounce 0, so the common cases run at high occupancy.
 template <int CLS, bool MEDIA>
 __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
+    constexpr int MT = (CLS == 0 || CLS == kClassCameraMiss) ? -1 : CLS - 1;
     const int k = wave_shard();
     const int n = *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + CLS) * kShards + k);
     const int chunks = (n + 63) >> 6;
@@ -762,9 +791,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         if (active) {
             const uint32_t qp = cq[slot];
             pid = q[qp];
-            const bool fresh = is_fresh(p, qp);
-            cont = shade_path<CLS - 1, MEDIA>(p, lut, pid, p.hits[qp], fresh, fresh ? fresh_sample(p, x, qp) : 0u,
-                                              sev, slv, rad);
+            const bool fresh = CLS == kClassCameraMiss ? true : CLS == 0 ? false : is_fresh(p, qp);
+            cont = shade_path<MT, MEDIA>(p, lut, pid, p.hits[qp], fresh, fresh ? fresh_sample(p, x, qp) : 0u, sev,
+                                         slv, rad);
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
@@ -881,7 +910,7 @@ hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const
 
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s) {
-    const int64_t n = std::max<int64_t>(p.pool, (int64_t)ctr_ints());
+    const int64_t n = (int64_t)ctr_ints();
     k_window_init<<<blocks(n, 256), 256, 0, s>>>(p, const_cast<ReqDesc*>(p.reqs), set_req0 ? 1 : 0, frame0,
                                                  path_frame_offset, frames, (int)total_samples);
     return hipGetLastError();
@@ -889,6 +918,11 @@ hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int 
 
 hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s) {
     k_regen<<<grid, 256, 0, s>>>(p, y);
+    return hipGetLastError();
+}
+
+hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s) {
+    k_fill<<<grid, 256, 0, s>>>(p);
     return hipGetLastError();
 }
 
@@ -917,6 +951,7 @@ void launch_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
     if (m & 64u) k_shade<6, MEDIA><<<grid, 256, 0, s>>>(p, x);
     if (m & 128u) k_shade<7, MEDIA><<<grid, 256, 0, s>>>(p, x);
     if (m & 256u) k_shade<8, MEDIA><<<grid, 256, 0, s>>>(p, x);
+    if (m & 512u) k_shade<kClassCameraMiss, MEDIA><<<grid, 256, 0, s>>>(p, x);
 }
 
 hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s) {

@@ -41,8 +41,8 @@ int64_t wave_round_pool(int64_t paths) {
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
     // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, pending-finish lists 2x4,
-    // shadow records 2x48, shadow lists 2x2x4, shade classes 9x4
-    return (size_t)pool * (64 + 32 + 16 + 8 + 8 + 96 + 16 + 36) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
+    // shadow records 2x48, shadow lists 2x2x4, shade classes 10x4
+    return (size_t)pool * (64 + 32 + 16 + 8 + 8 + 96 + 16 + 40) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
            ctr_ints() * 4 + 4096;
 }
 
@@ -193,7 +193,7 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     }
     if ((err = wave_window_init(p, r.set_req0, r.frame0, r.path_frame_offset, r.frames, r.total_samples, s)) !=
             hipSuccess ||
-        (err = wave_regen(p, 1, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
+        (err = wave_fill(p, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
         return err;
     r.launches += 2;
 
