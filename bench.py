@@ -252,8 +252,11 @@ def main():
         out["config"]["graph_replay_ms_per_step"] = round(graph_ms, 3)
         out["config"]["graph_replay_Mpaths_s"] = round(paths / (graph_ms / 1e3) / 1e6, 3)
     if counted is not None:
-        ext_bytes = NODE_BYTES * counted.ext_node_fetches + TRI_BYTES * counted.ext_tri_tests + \
-            RAY_BYTES * counted.extension_rays
+        # the counting pass traces every camera ray; the timed steps resolve the
+        # ones the cover mask rules out without a trace, so their bytes are not moved
+        ext_bytes = NODE_BYTES * (counted.ext_node_fetches - counted.resolved_node_fetches) + \
+            TRI_BYTES * (counted.ext_tri_tests - counted.resolved_tri_tests) + \
+            RAY_BYTES * (counted.extension_rays - counted.camera_rays_resolved)
         sh_nodes = counted.node_fetches - counted.ext_node_fetches
         sh_tris = counted.tri_tests - counted.ext_tri_tests
         sh_bytes = NODE_BYTES * sh_nodes + TRI_BYTES * sh_tris + SHADOW_RAY_BYTES * counted.shadow_rays
@@ -266,11 +269,13 @@ def main():
             "kernel": "k_trace (all launches of one step: extension and shadow rays)",
             "launches_per_step": int(st.iterations),
             "bytes_per_step": int(alg),
-            "per_ray_bytes": {"extension": round(ext_bytes / max(counted.extension_rays, 1), 1),
+            "per_ray_bytes": {"extension": round(ext_bytes / max(counted.extension_rays - counted.camera_rays_resolved, 1), 1),
                               "shadow": round(sh_bytes / max(counted.shadow_rays, 1), 1)},
         }
         out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
-                              "rays_per_s": round((counted.extension_rays + counted.shadow_rays) /
+                              "camera_rays_resolved_without_trace": int(counted.camera_rays_resolved),
+                              "traced_rays_per_s": round((counted.extension_rays - counted.camera_rays_resolved +
+                                                          counted.shadow_rays) /
                                                   (ms_per_step / 1e3), 0)})
     if not args.no_serving and not args.profile_steps:
         out["config"]["serving"] = serving(ctx, args)

@@ -106,6 +106,7 @@ typedef struct oxr_scene_desc {
 #define OXR_FLAG_COUNT_TRAVERSAL 2u     /* count node fetches / triangle tests (slower; measurement only) */
 #define OXR_FLAG_EAGER 4u               /* launch kernels one by one instead of replaying a captured hipGraph */
 #define OXR_FLAG_STAGE_TIMING 8u        /* per-stage device times (HIP events between stages; implies eager) */
+#define OXR_FLAG_NO_CAMERA_CULL 16u     /* trace every camera ray (results are identical either way) */
 
 /* One render call = spp progressive frames of one sample each (reference
    Render() with numPixelSamples = 1, Renderer.cpp:635-793). */
@@ -134,8 +135,16 @@ typedef struct oxr_render_stats {
     uint64_t paths;
     int32_t frame_id;                   /* next frameID */
     int32_t kernel_launches;
-    int32_t iterations;                 /* wavefront iterations (extend .. regenerate) over all windows */
+    int32_t iterations;                 /* wavefront iterations (trace .. shade) over all windows */
     int32_t windows;                    /* sample windows rendered */
+    /* Camera rays known to leave the scene without a trace: their pixel's
+       tile lies outside the image of every box of the BVH's top levels (the
+       cover set, a few hundred boxes).  extension_rays counts the rays traced.
+       With OXR_FLAG_COUNT_TRAVERSAL every ray is traced, and these are the
+       rays that would have been skipped and the node fetches / triangle tests
+       they took (included in ext_node_fetches / ext_tri_tests). */
+    uint64_t camera_rays_resolved;
+    uint64_t resolved_node_fetches, resolved_tri_tests;
 } oxr_render_stats;
 
 /* ---- context and memory (CUDABuffer-style API, common/CUDABuffer.h:26-81) ---- */

@@ -16,6 +16,7 @@ OXR_FLAG_KEEP_PATH_RADIANCE = 1
 OXR_FLAG_COUNT_TRAVERSAL = 2
 OXR_FLAG_EAGER = 4
 OXR_FLAG_STAGE_TIMING = 8
+OXR_FLAG_NO_CAMERA_CULL = 16
 
 # MaterialType (core/include/Scene.h:59-68)
 DIFFUSE, CONDUCTOR, DIELECTRIC, PLASTIC, METAL_WORKFLOW, THIN_DIELECTRIC, CLEARCOAT_CONDUCTOR, DIFFUSE_TRANSMITTER = range(8)
@@ -207,6 +208,9 @@ class RenderStats(C.Structure):
         ("kernel_launches", C.c_int32),
         ("iterations", C.c_int32),
         ("windows", C.c_int32),
+        ("camera_rays_resolved", C.c_uint64),
+        ("resolved_node_fetches", C.c_uint64),
+        ("resolved_tri_tests", C.c_uint64),
     ]
 
     def as_dict(self):

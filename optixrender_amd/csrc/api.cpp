@@ -153,6 +153,7 @@ void free_scene(SceneDev* s) {
     dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache);
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
     dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
+    dfree(s->cover_boxes); dfree(s->cover_count);
     delete s;
 }
 
@@ -260,6 +261,8 @@ void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_b
     p.env_h = s->env_h;
     p.lut_bsdf = lut_bsdf;
     p.lut_albedo = lut_albedo;
+    p.cover_boxes = s->cover_boxes;
+    p.cover_count = s->cover_count;
 }
 
 extern "C" {
@@ -313,7 +316,7 @@ int oxr_destroy(oxr_ctx* c) {
     wave_release(c->eng);
     dfree(c->lut_bsdf); dfree(c->lut_albedo); dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba);
     dfree(c->depth);
-    dfree(c->path_radiance); dfree(c->reqs);
+    dfree(c->path_radiance); dfree(c->reqs); dfree(c->cover);
     dfree(c->dn_lanes); dfree(c->dn_bias); dfree(c->denoised);
     if (c->dn_scratch) (void)hipFree(c->dn_scratch);
     for (hipEvent_t ev : {c->t0, c->t1, c->td0, c->td1})
@@ -443,6 +446,8 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         OXR_HIP(c, dalloc(&b.tris, 3 * n));
         OXR_HIP(c, dalloc(&s->normals_sorted, 9 * n));
         OXR_HIP(c, dalloc(&s->mat_sorted, n));
+        OXR_HIP(c, dalloc(&s->cover_boxes, 2 * (size_t)kCoverBoxes));
+        OXR_HIP(c, dalloc(&s->cover_count, 1));
     }
     void* scratch = nullptr;
     size_t sb = bvh_scratch_bytes(n);
@@ -457,6 +462,8 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
             e = gather_sorted_attributes(s->normals, s->mat_ids, s->bvh.order, n, s->normals_sorted, s->mat_sorted,
                                          c->stream);
     }
+    if (e == hipSuccess) e = build_cover_set(BvhView{s->bvh.pairs, s->bvh.tris, n}, s->cover_boxes, s->cover_count,
+                                             c->stream);
     OXR_HIP(c, hipEventRecord(e1, c->stream));
     hipError_t es = hipStreamSynchronize(c->stream);
     uint32_t bad = 0;
@@ -622,6 +629,14 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     OXR_HIP(c, wave_reserve(c->eng, pool, window, c->trace_grid));
     const bool keep = (prm->flags & OXR_FLAG_KEEP_PATH_RADIANCE) != 0;
     const bool count = (prm->flags & OXR_FLAG_COUNT_TRAVERSAL) != 0;
+    const bool cull = (prm->flags & OXR_FLAG_NO_CAMERA_CULL) == 0;
+    const size_t tiles = (size_t)cover_tiles(W) * cover_tiles(H);
+    if (cull && tiles > c->cover_tiles) {
+        dfree(c->cover);
+        c->cover_tiles = 0;
+        OXR_HIP(c, dalloc(&c->cover, tiles));
+        c->cover_tiles = tiles;
+    }
     if (keep) {
         size_t need = (size_t)3 * prm->spp * npix;
         if (need > c->path_radiance_floats) {
@@ -649,6 +664,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     rq.normal = c->normal;
     rq.depth = c->depth;
     rq.path_radiance = keep ? c->path_radiance : nullptr;
+    rq.cover = cull ? c->cover : nullptr;
+    rq.tiles_w = cover_tiles(W);
     OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
     fill_scene_params(p, s, c->lut_bsdf, c->lut_albedo);
@@ -659,10 +676,11 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.reqs = c->reqs;
     p.nreq = 1;
     p.stack_lds = c->tuning.stack_lds;
+    p.resolve = count ? 0 : 1;  // a counting render traces every ray and tallies the ones it could skip
 
     oxr_render_stats& st = c->stats;
     std::memset(&st, 0, sizeof(st));
-    OXR_HIP(c, hipMemsetAsync(c->eng.trav, 0, 8 * sizeof(unsigned long long), c->stream));
+    OXR_HIP(c, hipMemsetAsync(c->eng.trav, 0, kTravCounters * sizeof(unsigned long long), c->stream));
     OXR_HIP(c, hipMemsetAsync(c->eng.overflow, 0, sizeof(int), c->stream));
     OXR_HIP(c, hipEventRecord(c->t0, c->stream));
     WindowRun r;
@@ -708,7 +726,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     launches++;
     OXR_HIP(c, hipEventRecord(c->t1, c->stream));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
-    unsigned long long tv[8];
+    unsigned long long tv[kTravCounters];
     OXR_HIP(c, hipMemcpy(tv, c->eng.trav, sizeof(tv), hipMemcpyDeviceToHost));
     int ov = 0;
     OXR_HIP(c, hipMemcpy(&ov, c->eng.overflow, sizeof(int), hipMemcpyDeviceToHost));
@@ -731,6 +749,9 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.tri_tests = tv[1] + tv[5];
     st.extension_rays = tv[2];
     st.shadow_rays = tv[3];
+    st.camera_rays_resolved = tv[6];
+    st.resolved_node_fetches = tv[7];
+    st.resolved_tri_tests = tv[8];
     st.paths = (uint64_t)prm->spp * npix;
     st.kernel_launches = launches;
     st.iterations = iterations;

@@ -82,7 +82,21 @@ struct ReqDesc {
     float4* normal;
     float* depth;          // first-hit depth d / (1 + d), 1 for a miss
     float* path_radiance;  // optional, frame-major 3 floats per sample
+    // Camera-ray cover mask, one byte per kCoverTile x kCoverTile pixel tile
+    // (row-major, tiles_w per row): 0 = no camera ray of the tile's pixels can
+    // hit the scene, so its samples skip the first trace.  nullptr: none.
+    const uint8_t* cover;
+    int tiles_w;
 };
+
+// Camera-ray culling (build_cover_mask): the boxes of the BVH's top levels,
+// at most kCoverBoxes, projected per request onto tiles of kCoverTile pixels.
+// kCoverMargin pixels cover the largest pixel-filter offset (the Gaussian's
+// 0.5 + 0.375 sqrt(-2 ln FLT_MIN) < 5.5) plus 2 pixels for rounding.
+constexpr int kCoverBoxes = 512;
+constexpr int kCoverTile = 8;
+constexpr double kCoverMargin = 8.0;
+constexpr int cover_tiles(int n) { return (n + kCoverTile - 1) / kCoverTile; }
 
 // The path pool is split into kShards independent shards of pool/kShards
 // slots.  Wave w of any stage kernel serves shard w % kShards, so with the
@@ -92,13 +106,16 @@ struct ReqDesc {
 // MI355X), different addresses proceed in parallel.
 constexpr int kShards = 32;
 constexpr int kCtrStride = 64;  // ints between counters: one 256-byte line each
+constexpr int kQueueRegions = 3;  // per shard: survivors, new paths, new paths resolved without a trace
 
 // Counter ids (int index = id * kCtrStride).
 // Lists written by the stage at parity y and read by a later stage carry a
 // parity index; see wave_trace for who writes, reads and clears each.
 enum CtrId {
-    kCtrQueue = 0,                       // 2 x 64 region counts: [parity][region]; region k survivors, 32+k new paths
-    kCtrTerm = kCtrQueue + 2 * 2 * kShards,    // [parity][shard]: slots whose path ended with shadow rays pending
+    // [parity][region]: region k survivors, kShards + k new paths to trace,
+    // 2 kShards + k new paths whose camera ray is known to leave the scene
+    kCtrQueue = 0,
+    kCtrTerm = kCtrQueue + 2 * kQueueRegions * kShards,  // [parity][shard]: paths ended with shadow rays pending
     kCtrShEnv = kCtrTerm + 2 * kShards,  // [parity][shard]: env shadow rays listed
     kCtrShLight = kCtrShEnv + 2 * kShards,     // [parity][shard]: light shadow rays listed
     kCtrFetchTrace = kCtrShLight + 2 * kShards,
@@ -109,6 +126,8 @@ enum CtrId {
     kCtrFetchClass = kCtrClass + 2 * 10 * kShards, // [class][shard]: chunks taken by the shade kernels
     kCtrIds = kCtrFetchClass + 10 * kShards
 };
+// Counter of queue x's region r (0 survivors, 1 new, 2 new resolved) of shard k.
+constexpr int queue_ctr(int x, int r, int k) { return kCtrQueue + (x * kQueueRegions + r) * kShards + k; }
 // The host's completion poll copies counters [kCtrQueue, kPollIds): the
 // queue and pending-finish counts of both parities.
 constexpr int kPollIds = kCtrTerm + 2 * kShards;
@@ -145,9 +164,9 @@ struct WaveParams {
     //   [0] origin, pre_pdf   [1] direction, rng state bits
     //   [2] throughput, bounce (int bits)   [3] radiance, sample id (uint bits; kNoSample: none)
     float4* path;
-    uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
-    uint32_t* fresh_g[2];  // sample ids of the new-path regions (kShards regions of `region`)
-    float4* hits;      // per queue position (2*kShards regions)
+    uint32_t* queue[2];  // kQueueRegions*kShards regions of `region` slot ids each
+    uint32_t* fresh_g[2];  // sample ids of the new-path regions ((kQueueRegions-1)*kShards regions of `region`)
+    float4* hits;      // per traced queue position (2*kShards regions)
     // shadow rays, one record of 3 float4 per slot and kind (env / light):
     //   [0] origin, tmax   [1] direction   [2] contribution, pending flag
     // The flag is 1 when the slot's last shade emitted the ray and the trace
@@ -169,12 +188,21 @@ struct WaveParams {
     int* ctr;          // device counters, ctr_ints() ints
     const ReqDesc* reqs;  // device table of the window's requests
     int nreq;
-    unsigned long long* traversal;  // [0,1] extension-ray node fetches / triangle tests (when counting), [2] extension rays, [3] shadow rays, [4,5] shadow-ray node fetches / triangle tests
+    // [0,1] extension-ray node fetches / triangle tests (when counting),
+    // [2] extension rays, [3] shadow rays, [4,5] shadow-ray node fetches /
+    // triangle tests, [6] camera rays resolved by the cover mask (counting:
+    // that it would resolve), [7,8] node fetches / triangle tests of those
+    // (counting only; included in [0,1])
+    unsigned long long* traversal;
     uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
     int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
     int* overflow;
+    int resolve;  // 1: new samples in uncovered tiles skip the trace; 0: traced (and tallied when counting)
+    const float4* cover_boxes;  // the scene's cover set (build_cover_set)
+    const int* cover_count;
 };
 constexpr uint32_t kNoSample = 0xffffffffu;
+constexpr int kTravCounters = 16;  // entries of WaveParams::traversal
 
 // Window set-up: rewrites frame0 / path_frame_offset / frames of request 0
 // (single renders; the server writes its table from the host), zeroes every
@@ -183,6 +211,14 @@ constexpr uint32_t kNoSample = 0xffffffffu;
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s);
 hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s);
+// Scene cover set: the boxes of the BVH's top levels (breadth first, at most
+// kCoverBoxes; leaves met on the way included), every triangle inside one.
+// boxes: 2 float4 (lo, hi) each; count: one int.
+hipError_t build_cover_set(const BvhView& b, float4* boxes, int* count, hipStream_t s);
+// Cover masks of requests r0 .. r0 + nreq - 1 of the device table: a tile is
+// 1 when the camera ray of some sample of its pixels may enter a cover box.
+hipError_t build_cover_masks(const ReqDesc* reqs, int nreq, int max_tiles, const float4* boxes, const int* count,
+                             hipStream_t s);
 // One iteration at parity y is trace(y) -> regen(y) -> shade(y):
 //   trace(y): extension rays of queue y and the shadow rays listed at parity
 //     y^1 (by the previous shade) in one launch; hits go to the shade-class

@@ -30,6 +30,8 @@ struct SceneDev {
     int env_w = 0, env_h = 0;
     float env_lum_sum = 0.0f;  // sequential luminance sum of the environment (host, at upload)
     BvhBuffers bvh;
+    float4* cover_boxes = nullptr;  // cover set of the BVH (2 float4 per box, kCoverBoxes at most)
+    int* cover_count = nullptr;
     bool built = false;
     float build_ms = 0.0f;
 };
@@ -117,6 +119,8 @@ struct oxr_ctx {
     oxr_render_stats stats;
     hipEvent_t t0 = nullptr, t1 = nullptr, td0 = nullptr, td1 = nullptr;
     oxr::ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
+    uint8_t* cover = nullptr;      // camera-ray cover mask of single renders
+    size_t cover_tiles = 0;        // its capacity
     // denoiser: weights (oxr_denoiser_load), output and scratch of the last denoised render
     uint16_t* dn_lanes = nullptr;
     float* dn_bias = nullptr;

@@ -180,8 +180,8 @@ OXR_D int wave_next(int* ctr) {
 struct ShardQueue {
     int n_surv, n_new, c_surv, chunks;
     OXR_D ShardQueue(int* ctr, int x, int k) {
-        n_surv = *ctr_at(ctr, kCtrQueue + x * 2 * kShards + k);
-        n_new = *ctr_at(ctr, kCtrQueue + x * 2 * kShards + kShards + k);
+        n_surv = *ctr_at(ctr, queue_ctr(x, 0, k));
+        n_new = *ctr_at(ctr, queue_ctr(x, 1, k));
         c_surv = (n_surv + 63) >> 6;
         chunks = c_surv + ((n_new + 63) >> 6);
     }
@@ -237,10 +237,46 @@ OXR_D uint32_t fresh_sample(const WaveParams& p, int x, int64_t qp) {
     return p.fresh_g[x][qp - (int64_t)kShards * p.region];
 }
 
+// Whether the camera ray of sample g surely leaves the scene: its pixel's
+// tile is clear in the request's cover mask.
+OXR_D bool camera_ray_misses(const WaveParams& p, int64_t g) {
+    const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, g)];
+    if (!q.cover) return false;
+    const uint32_t local = (uint32_t)(g - q.path_begin);  // a window holds fewer than 2^31 samples
+    const uint32_t w = (uint32_t)q.width;
+    const uint32_t pix = local % (w * (uint32_t)q.height);
+    const uint32_t ix = pix % w, iy = pix / w;
+    return q.cover[(iy / kCoverTile) * (uint32_t)q.tiles_w + ix / kCoverTile] == 0;
+}
+
+// Puts new sample g in slot pid into queue y: `trace` lanes go to the
+// new-path region (the next trace casts their camera ray), `skip` lanes to
+// the resolved region and straight onto the camera-miss shade list of parity
+// y (their camera ray leaves the scene).  o_trace / o_skip: the lane's
+// position in its region, taken by the caller.
+OXR_D void place_sample(const WaveParams& p, int k, int y, bool trace, bool skip, int o_trace, int o_skip,
+                        uint32_t pid, uint32_t g) {
+    if (trace) {
+        p.queue[y][(size_t)(kShards + k) * p.region + o_trace] = pid;
+        p.fresh_g[y][(size_t)k * p.region + o_trace] = g;
+    }
+    if (skip) {
+        const size_t qp = (size_t)(2 * kShards + k) * p.region + o_skip;
+        p.queue[y][qp] = pid;
+        p.fresh_g[y][(size_t)(kShards + k) * p.region + o_skip] = g;
+    }
+}
+OXR_D void list_skipped(const WaveParams& p, int k, int y, bool skip, int o_skip) {
+    const int o = wave_append(ctr_at(p.ctr, kCtrClass + (y * kShadeClasses + kClassCameraMiss) * kShards + k), skip);
+    if (o >= 0)
+        p.cls_queue[((size_t)kClassCameraMiss * kShards + k) * p.region + o] =
+            (uint32_t)((size_t)(2 * kShards + k) * p.region + o_skip);
+}
+
 // Draws one new sample of shard k for every lane that asks (consecutive
 // positions of the shard's sequence, one atomic per wave), starts it in the
-// lane's slot and appends the slot to the new-path region of queue y.  All
-// lanes of the wave must call it.  Returns whether this lane started a path.
+// lane's slot and appends the slot to queue y.  All lanes of the wave must
+// call it.  Returns whether this lane started a path.
 OXR_D bool restart(const WaveParams& p, int k, int y, bool want, uint32_t pid) {
     const uint64_t m = __ballot(want);
     if (m == 0) return false;
@@ -250,11 +286,12 @@ OXR_D bool restart(const WaveParams& p, int k, int y, bool want, uint32_t pid) {
     const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
     const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
     const bool start = want && g < total;
-    const int o = wave_append(ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + kShards + k), start);
-    if (o >= 0) {
-        p.queue[y][(size_t)(kShards + k) * p.region + o] = pid;
-        p.fresh_g[y][(size_t)k * p.region + o] = (uint32_t)g;
-    }
+    const bool skip = start && p.resolve && camera_ray_misses(p, g);
+    const bool trace = start && !skip;
+    const int o_trace = wave_append(ctr_at(p.ctr, queue_ctr(y, 1, k)), trace);
+    const int o_skip = wave_append(ctr_at(p.ctr, queue_ctr(y, 2, k)), skip);
+    place_sample(p, k, y, trace, skip, o_trace, o_skip, pid, (uint32_t)g);
+    list_skipped(p, k, y, skip, o_skip);
     return start;
 }
 
@@ -279,32 +316,58 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
 }
 
 // Window start: slot s of shard k takes position s of the shard's draw
-// sequence and, if that sample exists, queue position s of the new-path
-// region of queue 0.  sample_of grows with the position, so the started slots
-// are a prefix of the shard and the queue stays dense; each wave adds its
-// count to the shard's queue counter once.  Waves take the shard's 64-slot
-// chunks in a fixed stride, with no fetch counter.
+// sequence and, if that sample exists, starts it (queue 0).  Waves take the
+// shard's 64-slot chunks in a fixed stride, with no fetch counter, and
+// reserve their queue positions with one atomic per region: a first pass
+// counts, a second writes.
 __global__ __launch_bounds__(256) void k_fill(WaveParams p) {
     const int k = wave_shard();
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
     const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
     const int chunks = p.region >> 6;  // region is a multiple of 64
-    const size_t rbase = (size_t)k * p.region;
-    uint32_t* qn = p.queue[0] + (size_t)(kShards + k) * p.region;
-    uint32_t* gn = p.fresh_g[0] + rbase;
-    int started = 0;
-    for (int j = wave / kShards; j < chunks; j += stride) {
+    const int first = wave / kShards;
+    auto classify = [&](int j, bool& trace, bool& skip) -> int64_t {
         const int slot = (j << 6) + lane_id();
         const int64_t g = sample_of(k, slot);
-        const bool st = g < total;
-        if (st) {
-            qn[slot] = (uint32_t)(rbase + slot);
-            gn[slot] = (uint32_t)g;
-        }
-        started += (int)__popcll(__ballot(st));
+        const bool start = g < total;
+        skip = start && p.resolve && camera_ray_misses(p, g);
+        trace = start && !skip;
+        return g;
+    };
+    int n_trace = 0, n_skip = 0;
+    for (int j = first; j < chunks; j += stride) {
+        bool trace, skip;
+        classify(j, trace, skip);
+        n_trace += (int)__popcll(__ballot(trace));
+        n_skip += (int)__popcll(__ballot(skip));
     }
-    if (lane_id() == 0 && started) atomicAdd(ctr_at(p.ctr, kCtrQueue + kShards + k), started);
+    int b_trace = 0, b_skip = 0, b_cls = 0;
+    if (lane_id() == 0) {
+        if (n_trace) b_trace = atomicAdd(ctr_at(p.ctr, queue_ctr(0, 1, k)), n_trace);
+        if (n_skip) {
+            b_skip = atomicAdd(ctr_at(p.ctr, queue_ctr(0, 2, k)), n_skip);
+            b_cls = atomicAdd(ctr_at(p.ctr, kCtrClass + kClassCameraMiss * kShards + k), n_skip);
+        }
+    }
+    b_trace = __shfl(b_trace, 0);
+    b_skip = __shfl(b_skip, 0);
+    b_cls = __shfl(b_cls, 0);
+    for (int j = first; j < chunks; j += stride) {
+        bool trace, skip;
+        const int64_t g = classify(j, trace, skip);
+        const uint64_t mt = __ballot(trace), ms = __ballot(skip);
+        const int o_trace = b_trace + (int)__popcll(mt & lanes_below());
+        const int o_skip = b_skip + (int)__popcll(ms & lanes_below());
+        const uint32_t pid = (uint32_t)((int64_t)k * p.region + (j << 6) + lane_id());
+        place_sample(p, k, 0, trace, skip, o_trace, o_skip, pid, (uint32_t)g);
+        if (skip)
+            p.cls_queue[((size_t)kClassCameraMiss * kShards + k) * p.region + b_cls + (int)__popcll(ms & lanes_below())] =
+                (uint32_t)((size_t)(2 * kShards + k) * p.region + o_skip);
+        b_trace += (int)__popcll(mt);
+        b_skip += (int)__popcll(ms);
+        b_cls += (int)__popcll(ms);
+    }
 }
 
 // Adds slot pid's pending shadow contributions to r in emission order (the
@@ -378,7 +441,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
     if (blockIdx.x == 0) {
         // lists this iteration refills, whose readers have finished
         const int t = threadIdx.x;
-        if (t < 2 * kShards) *ctr_at(p.ctr, kCtrQueue + (y ^ 1) * 2 * kShards + t) = 0;
+        if (t < kQueueRegions * kShards) *ctr_at(p.ctr, queue_ctr(y ^ 1, 0, 0) + t) = 0;
         if (t < kShards) {
             *ctr_at(p.ctr, kCtrTerm + y * kShards + t) = 0;
             *ctr_at(p.ctr, kCtrShEnv + y * kShards + t) = 0;
@@ -388,8 +451,10 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         for (int i = t; i < kShadeClasses * kShards; i += blockDim.x)
             *ctr_at(p.ctr, kCtrClass + ((y ^ 1) * kShadeClasses) * kShards + i) = 0;
         if (t == 0) {
-            unsigned long long rays = 0, shadow = 0;
-            for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, kCtrQueue + y * 2 * kShards + r);
+            unsigned long long rays = 0, shadow = 0, resolved = 0;
+            for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, queue_ctr(y, 0, 0) + r);
+            for (int r = 0; r < kShards; r++) resolved += (unsigned)*ctr_at(p.ctr, queue_ctr(y, 2, r));
+            if (resolved) atomicAdd(&p.traversal[6], resolved);
             for (int r = 0; r < kShards; r++)
                 shadow += (unsigned)*ctr_at(p.ctr, kCtrShEnv + (y ^ 1) * kShards + r) +
                           (unsigned)*ctr_at(p.ctr, kCtrShLight + (y ^ 1) * kShards + r);
@@ -409,6 +474,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
     const uint32_t* shl = p.sh_list[y ^ 1] + (size_t)k * p.region;
     Stack<S> st(lds, gstack, (int64_t)gridDim.x * blockDim.x);
     uint32_t nn = 0, nt = 0, sn = 0, stt = 0;
+    uint32_t cn = 0, cnn = 0, cnt = 0;  // counting: rays the cover mask would resolve, their fetches / tests
     while (true) {
         const int j = wave_next(fetch);
         if (j >= chunks) break;
@@ -429,15 +495,24 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         if (qp >= 0) {
             f3 org, dir;
             const bool fresh = is_fresh(p, qp);
+            bool tally = false;
             if (fresh) {
                 uint32_t rs;
-                camera_ray(p, fresh_sample(p, y, qp), org, dir, rs);
+                const uint32_t g = fresh_sample(p, y, qp);
+                camera_ray(p, g, org, dir, rs);
+                if (COUNT) tally = camera_ray_misses(p, g);
             } else {
                 const uint32_t pid = q[qp];
                 org = xyz(p.path[4 * (size_t)pid + 0]);
                 dir = xyz(p.path[4 * (size_t)pid + 1]);
             }
+            const uint32_t nn0 = nn, nt0 = nt;
             Hit h = traverse<false, COUNT>(p.bvh, org, dir, kEps, kFltMax, st, nn, nt, overflow);
+            if (COUNT && tally) {
+                cn++;
+                cnn += nn - nn0;
+                cnt += nt - nt0;
+            }
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
             cls = h.leaf != kNoHit ? 1 + p.mats[p.mat_ids[h.leaf]].type : fresh ? kClassCameraMiss : 0;
         }
@@ -457,6 +532,9 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         wave_count_add(&p.traversal[1], nt);
         wave_count_add(&p.traversal[4], sn);
         wave_count_add(&p.traversal[5], stt);
+        wave_count_add(&p.traversal[6], cn);
+        wave_count_add(&p.traversal[7], cnn);
+        wave_count_add(&p.traversal[8], cnt);
     }
 }
 
@@ -770,7 +848,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
     const uint32_t* q = p.queue[x];
     const size_t rbase = (size_t)k * p.region;
     uint32_t* qo = p.queue[x ^ 1] + rbase;  // survivors region of shard k
-    int* qo_count = ctr_at(p.ctr, kCtrQueue + (x ^ 1) * 2 * kShards + k);
+    int* qo_count = ctr_at(p.ctr, queue_ctr(x ^ 1, 0, k));
     uint32_t* term = p.term[x] + rbase;
     int* term_count = ctr_at(p.ctr, kCtrTerm + x * kShards + k);
     uint32_t* le = p.sh_list[x] + rbase;
@@ -792,8 +870,11 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
             const uint32_t qp = cq[slot];
             pid = q[qp];
             const bool fresh = CLS == kClassCameraMiss ? true : CLS == 0 ? false : is_fresh(p, qp);
-            cont = shade_path<MT, MEDIA>(p, lut, pid, p.hits[qp], fresh, fresh ? fresh_sample(p, x, qp) : 0u, sev,
-                                         slv, rad);
+            // a camera miss may sit in the resolved region, which has no hit
+            // record; the miss classes do not read it
+            const float4 hit = MT < 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : p.hits[qp];
+            cont = shade_path<MT, MEDIA>(p, lut, pid, hit, fresh, fresh ? fresh_sample(p, x, qp) : 0u, sev, slv,
+                                         rad);
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
@@ -859,9 +940,161 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     q.depth[pix] = dz;
 }
 
+// The scene's cover set (one thread): starting from the root's two child
+// records, replace every internal node by its children's records, level by
+// level, while the list stays within kCoverBoxes.  Expanding from the back
+// writes each entry at or after the one being read.
+__global__ void k_cover_set(BvhView b, float4* boxes, int* count) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (b.n <= 0) {
+        *count = 0;
+        return;
+    }
+    if (b.n == 1) {
+        float4 v0 = b.tris[0], v1 = b.tris[1], v2 = b.tris[2];
+        boxes[0] = make_float4(fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y),
+                               fminf(fminf(v0.z, v1.z), v2.z), __uint_as_float(kLeafBit));
+        boxes[1] = make_float4(fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y),
+                               fmaxf(fmaxf(v0.z, v1.z), v2.z), 0.0f);
+        *count = 1;
+        return;
+    }
+    for (int i = 0; i < 4; i++) boxes[i] = b.pairs[i];
+    int n = 2;
+    while (true) {
+        int internal = 0;
+        for (int i = 0; i < n; i++) internal += (__float_as_uint(boxes[2 * i].w) & kLeafBit) ? 0 : 1;
+        if (internal == 0 || n + internal > kCoverBoxes) break;
+        int m = n + internal;
+        for (int i = n - 1; i >= 0; i--) {
+            const float4 lo = boxes[2 * i], hi = boxes[2 * i + 1];
+            const uint32_t ref = __float_as_uint(lo.w);
+            if (ref & kLeafBit) {
+                m--;
+                boxes[2 * m] = lo;
+                boxes[2 * m + 1] = hi;
+            } else {
+                const float4* pr = b.pairs + 4 * (size_t)ref;
+                m -= 2;
+                boxes[2 * m] = pr[0];
+                boxes[2 * m + 1] = pr[1];
+                boxes[2 * m + 2] = pr[2];
+                boxes[2 * m + 3] = pr[3];
+            }
+        }
+        n += internal;
+    }
+    *count = n;
+}
+
+// Cover mask of one request (one block).  A camera ray of pixel (ix, iy)
+// passes through the image-plane point X = ix + 0.5 + jx, Y = iy + 0.5 + jy
+// (pixel units, |j| below the largest filter offset): its direction is
+// cam_dir + (X / W - 0.5) cam_h + (Y / H - 0.5) cam_v.  Solving P - cam_pos =
+// a cam_dir + b cam_h + c cam_v for a box corner P (in double) gives its
+// image point X = (b / a + 0.5) W, Y = (c / a + 0.5) H when a > 0; every
+// point a camera ray reaches has a > 0, and the image of a box in front of
+// the camera lies in the rectangle of its corners' images.  A box wholly
+// behind the camera is skipped; one that straddles the camera plane marks
+// every tile.
+__global__ __launch_bounds__(256) void k_cover_mask(const ReqDesc* reqs, const float4* boxes, const int* count) {
+    const ReqDesc& q = reqs[blockIdx.x];
+    if (!q.cover) return;
+    uint8_t* cover = const_cast<uint8_t*>(q.cover);
+    const int tw = q.tiles_w, th = cover_tiles(q.height);
+    const int ntiles = tw * th;
+    __shared__ int all;
+    if (threadIdx.x == 0) all = 0;
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cover[i] = 0;
+    __syncthreads();
+    // inverse of the camera basis [cam_dir cam_h cam_v] (columns)
+    double m[3][3];
+    for (int r = 0; r < 3; r++) {
+        m[r][0] = q.cam_dir[r];
+        m[r][1] = q.cam_h[r];
+        m[r][2] = q.cam_v[r];
+    }
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    const double scale = fabs(m[0][0]) + fabs(m[1][0]) + fabs(m[2][0]) + fabs(m[0][1]) + fabs(m[1][1]) +
+                         fabs(m[2][1]) + fabs(m[0][2]) + fabs(m[1][2]) + fabs(m[2][2]);
+    const bool singular = !(fabs(det) > 1e-12 * scale * scale * scale);
+    double inv[3][3];
+    if (!singular) {
+        inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+        inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+        inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+        inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+        inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+        inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+        inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+        inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+        inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+    }
+    const int nb = count ? *count : 0;
+    if (!boxes || !count) all = 1;  // no cover set: every tile traced
+    for (int bi = threadIdx.x; bi < nb; bi += blockDim.x) {
+        if (singular) {
+            all = 1;
+            break;
+        }
+        const float4 lo = boxes[2 * bi], hi = boxes[2 * bi + 1];
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        int front = 0, behind = 0;
+        for (int c = 0; c < 8; c++) {
+            const double d[3] = {(double)((c & 1) ? hi.x : lo.x) - q.cam_pos[0],
+                                 (double)((c & 2) ? hi.y : lo.y) - q.cam_pos[1],
+                                 (double)((c & 4) ? hi.z : lo.z) - q.cam_pos[2]};
+            const double a = inv[0][0] * d[0] + inv[0][1] * d[1] + inv[0][2] * d[2];
+            const double bb = inv[1][0] * d[0] + inv[1][1] * d[1] + inv[1][2] * d[2];
+            const double cc = inv[2][0] * d[0] + inv[2][1] * d[1] + inv[2][2] * d[2];
+            if (a > 1e-9) {
+                front++;
+                const double X = (bb / a + 0.5) * q.width, Y = (cc / a + 0.5) * q.height;
+                x0 = fmin(x0, X);
+                x1 = fmax(x1, X);
+                y0 = fmin(y0, Y);
+                y1 = fmax(y1, Y);
+            } else if (a <= 0.0) {
+                behind++;
+            }
+        }
+        if (front == 0 && behind == 8) continue;  // no camera ray reaches it
+        if (front < 8) {
+            all = 1;
+            continue;
+        }
+        // pixels ix whose samples may fall in [x0, x1]: ix + 0.5 + j in it, |j| < margin
+        const double fx0 = floor((x0 - kCoverMargin - 0.5) / kCoverTile), fx1 = floor((x1 + kCoverMargin - 0.5) / kCoverTile);
+        const double fy0 = floor((y0 - kCoverMargin - 0.5) / kCoverTile), fy1 = floor((y1 + kCoverMargin - 0.5) / kCoverTile);
+        if (fx1 < 0.0 || fy1 < 0.0 || fx0 > tw - 1 || fy0 > th - 1) continue;
+        const int tx0 = (int)fmax(fx0, 0.0), tx1 = (int)fmin(fx1, (double)(tw - 1));
+        const int ty0 = (int)fmax(fy0, 0.0), ty1 = (int)fmin(fy1, (double)(th - 1));
+        for (int ty = ty0; ty <= ty1; ty++)
+            for (int tx = tx0; tx <= tx1; tx++) cover[ty * tw + tx] = 1;
+    }
+    __syncthreads();
+    if (all)
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cover[i] = 1;
+}
+
 inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
+
+hipError_t build_cover_set(const BvhView& b, float4* boxes, int* count, hipStream_t s) {
+    k_cover_set<<<1, 64, 0, s>>>(b, boxes, count);
+    return hipGetLastError();
+}
+
+hipError_t build_cover_masks(const ReqDesc* reqs, int nreq, int max_tiles, const float4* boxes, const int* count,
+                             hipStream_t s) {
+    (void)max_tiles;
+    if (nreq <= 0) return hipSuccess;
+    k_cover_mask<<<nreq, 256, 0, s>>>(reqs, boxes, count);
+    return hipGetLastError();
+}
 
 hipError_t trace_rays(const BvhView& b, const float* rays, int64_t n, int any, float4* hits, uint8_t* occ,
                       uint32_t* counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s) {

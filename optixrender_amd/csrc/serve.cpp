@@ -45,13 +45,16 @@ struct Request {
     float4 *color = nullptr, *albedo = nullptr, *normal = nullptr, *denoised = nullptr;
     float* depth = nullptr;
     uint32_t* rgba = nullptr;
+    uint8_t* cover = nullptr;  // camera-ray cover mask (one byte per 8x8-pixel tile)
     Clock::time_point submitted, finished;
     std::string error;
 };
 
-// colour, albedo, normal (float4), depth (float), RGBA8, and the denoised colour
+// colour, albedo, normal (float4), depth (float), RGBA8, the denoised colour,
+// and the cover mask
 size_t request_bytes(const oxr_request& r) {
-    return (size_t)r.width * r.height * (3 * 16 + 4 + 4 + (r.denoise ? 16 : 0));
+    return (size_t)r.width * r.height * (3 * 16 + 4 + 4 + (r.denoise ? 16 : 0)) +
+           (size_t)cover_tiles(r.width) * cover_tiles(r.height);
 }
 
 }  // namespace
@@ -100,6 +103,7 @@ void sfree(T*& p) {
 
 void free_request_buffers(oxr_server* s, Request* q) {
     sfree(q->color); sfree(q->albedo); sfree(q->normal); sfree(q->rgba); sfree(q->depth); sfree(q->denoised);
+    sfree(q->cover);
     if (q->bytes) s->in_use -= q->bytes;
     q->bytes = 0;
 }
@@ -121,6 +125,7 @@ void admit(oxr_server* s) {
         if ((e = salloc(&q->color, np)) != hipSuccess || (e = salloc(&q->albedo, np)) != hipSuccess ||
             (e = salloc(&q->normal, np)) != hipSuccess || (e = salloc(&q->rgba, np)) != hipSuccess ||
             (e = salloc(&q->depth, np)) != hipSuccess ||
+            (e = salloc(&q->cover, (size_t)cover_tiles(q->r.width) * cover_tiles(q->r.height))) != hipSuccess ||
             (q->r.denoise && (e = salloc(&q->denoised, np)) != hipSuccess) ||
             (e = hipMemsetAsync(q->depth, 0, 4 * np, s->stream)) != hipSuccess ||
             (e = hipMemsetAsync(q->color, 0, 16 * np, s->stream)) != hipSuccess ||
@@ -166,6 +171,8 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
         d.albedo = q->albedo;
         d.normal = q->normal;
         d.depth = q->depth;
+        d.cover = q->cover;
+        d.tiles_w = cover_tiles(q->r.width);
         const int64_t np = (int64_t)q->r.width * q->r.height;
         path += np * frames[i];
         pix += np;
@@ -182,6 +189,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     p.reqs = s->reqs_dev;
     p.nreq = (int)group.size();
     p.stack_lds = s->stack_lds;
+    p.resolve = 1;
     WindowRun run;
     run.trace_grid = s->trace_grid;
     run.shade_grid = s->shade_grid;

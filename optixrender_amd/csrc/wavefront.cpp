@@ -40,9 +40,9 @@ int64_t wave_round_pool(int64_t paths) {
 }
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
-    // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, pending-finish lists 2x4,
+    // per slot: state 64, hits 2x16, queues 2x3x4 + new-path sample ids 2x2x4, pending-finish lists 2x4,
     // shadow records 2x48, shadow lists 2x2x4, shade classes 10x4
-    return (size_t)pool * (64 + 32 + 16 + 8 + 8 + 96 + 16 + 40) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
+    return (size_t)pool * (64 + 32 + 24 + 16 + 8 + 96 + 16 + 40) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
            ctr_ints() * 4 + 4096;
 }
 
@@ -72,11 +72,11 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
     hipError_t r;
     if (!e.ctr) {
         if ((r = walloc(&e.ctr, ctr_ints())) != hipSuccess || (r = walloc(&e.overflow, 4)) != hipSuccess ||
-            (r = walloc(&e.trav, 8)) != hipSuccess)
+            (r = walloc(&e.trav, kTravCounters)) != hipSuccess)
             return r;
         if ((r = hipMemset(e.ctr, 0, ctr_ints() * sizeof(int))) != hipSuccess ||
             (r = hipMemset(e.overflow, 0, 4 * sizeof(int))) != hipSuccess ||
-            (r = hipMemset(e.trav, 0, 8 * sizeof(unsigned long long))) != hipSuccess)
+            (r = hipMemset(e.trav, 0, kTravCounters * sizeof(unsigned long long))) != hipSuccess)
             return r;
         if ((r = hipHostMalloc((void**)&e.host_counts, 2 * kPollInts * sizeof(int), 0)) != hipSuccess) return r;
         for (auto& ev : e.poll)
@@ -89,12 +89,14 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
         e.pool = 0;
         if ((r = walloc(&e.path, 4 * pool)) != hipSuccess ||
             (r = walloc(&e.hits, 2 * pool)) != hipSuccess || (r = walloc(&e.sh_env, 3 * pool)) != hipSuccess ||
-            (r = walloc(&e.sh_light, 3 * pool)) != hipSuccess || (r = walloc(&e.q0, 2 * pool)) != hipSuccess ||
-            (r = walloc(&e.q1, 2 * pool)) != hipSuccess || (r = walloc(&e.t0, pool)) != hipSuccess ||
+            (r = walloc(&e.sh_light, 3 * pool)) != hipSuccess ||
+            (r = walloc(&e.q0, (size_t)kQueueRegions * pool)) != hipSuccess ||
+            (r = walloc(&e.q1, (size_t)kQueueRegions * pool)) != hipSuccess || (r = walloc(&e.t0, pool)) != hipSuccess ||
             (r = walloc(&e.t1, pool)) != hipSuccess || (r = walloc(&e.sl0, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.sl1, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.cls_queue, (size_t)kShadeClasses * pool)) != hipSuccess ||
-            (r = walloc(&e.g0, pool)) != hipSuccess || (r = walloc(&e.g1, pool)) != hipSuccess)
+            (r = walloc(&e.g0, (size_t)(kQueueRegions - 1) * pool)) != hipSuccess ||
+            (r = walloc(&e.g1, (size_t)(kQueueRegions - 1) * pool)) != hipSuccess)
             return r;
         e.pool = pool;
     }
@@ -193,9 +195,10 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     }
     if ((err = wave_window_init(p, r.set_req0, r.frame0, r.path_frame_offset, r.frames, r.total_samples, s)) !=
             hipSuccess ||
+        (err = build_cover_masks(p.reqs, p.nreq, 0, p.cover_boxes, p.cover_count, s)) != hipSuccess ||
         (err = wave_fill(p, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
         return err;
-    r.launches += 2;
+    r.launches += 3;
 
     hipGraphExec_t exec = nullptr;
     if (!eager) {
@@ -242,7 +245,7 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     auto live = [&](int slot) {
         const int* src = e.host_counts + slot * kPollInts;
         int64_t n = 0;
-        for (int q = 0; q < 2 * kShards; q++) n += src[(size_t)(kCtrQueue + q) * kCtrStride];
+        for (int q = 0; q < kQueueRegions * kShards; q++) n += src[(size_t)(queue_ctr(0, 0, 0) + q) * kCtrStride];
         for (int k = 0; k < kShards; k++) n += src[(size_t)(kCtrTerm + kShards + k) * kCtrStride];
         return n;
     };

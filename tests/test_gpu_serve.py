@@ -61,7 +61,7 @@ def test_budget_admission_serialises(ctx, cornell):
     probe = Server(ctx, budget_bytes=1 << 34, max_active=8, tick_paths=tick, max_frame_pixels=fp)
     wave = probe.stats().bytes_in_use          # idle server: the shared wavefront buffers only
     probe.close()
-    one = 64 * 48 * 56                           # colour + albedo + normal float4, depth float, RGBA8
+    one = 64 * 48 * 56 + 8 * 6                   # colour + albedo + normal float4, depth float, RGBA8; 8x8 cover tiles
     srv = Server(ctx, budget_bytes=wave + one + 100, max_active=8, tick_paths=tick, max_frame_pixels=fp)
     try:
         ids = [srv.submit(sid, 64, 48, 2, abi.camera((0, 1, 3), (0, 1, 0)), max_depth=8) for _ in range(3)]
