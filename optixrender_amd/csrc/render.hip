@@ -574,7 +574,7 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 // (radiance, sample id) for the caller.
 template <int MT, bool MEDIA>
 OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 hit, bool fresh, uint32_t g_fresh,
-                      ShadowOut& sh_env, ShadowOut& sh_light, float4& rad_out) {
+                      ShadowOut& sh_env, ShadowOut& sh_light, float4& rad_out, bool& aov_default) {
     sh_env.emit = false;
     sh_light.emit = false;
     f3 org, dir, history, radiance;
@@ -635,10 +635,6 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     } else {
         distance = kFltMax;
         mi_in = mi_out = p.cam_medium;
-        if (bounce == 0) {
-            p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-            p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-        }
     }
     float closest = distance;
     f3 light_radiance(0.0f);
@@ -646,9 +642,17 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     int mi_light = 0;
     bool hitLight = light_trace(p, org, dir, mi_light, light_radiance, light_pdf, closest);
     if (hitLight) mi_in = mi_out = mi_light;
+    // A camera ray that met neither a surface nor a light has the default
+    // AOVs (albedo and normal 0, depth 1); the caller writes them only if the
+    // sample does not end now, else it flags them in the radiance record.
+    aov_default = bounce == 0 && leaf == kNoHit && closest == kFltMax;
+    if (bounce == 0 && leaf == kNoHit && !aov_default) {
+        p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    }
     // depth AOV (no reference counterpart; a denoiser input): first surface
     // or light along the camera ray as d / (1 + d), 1 for a miss
-    if (bounce == 0) p.s_dep[g] = closest < kFltMax ? closest / (1.0f + closest) : 1.0f;
+    if (bounce == 0 && !aov_default) p.s_dep[g] = closest < kFltMax ? closest / (1.0f + closest) : 1.0f;
 
     bool scattered = false;
     int medium_index = -1;
@@ -866,6 +870,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         ShadowOut sev, slv;
         sev.emit = slv.emit = false;
         float4 rad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool aov_default = false;
         if (active) {
             const uint32_t qp = cq[slot];
             pid = q[qp];
@@ -874,7 +879,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
             // record; the miss classes do not read it
             const float4 hit = MT < 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : p.hits[qp];
             cont = shade_path<MT, MEDIA>(p, lut, pid, hit, fresh, fresh ? fresh_sample(p, x, qp) : 0u, sev, slv,
-                                         rad);
+                                         rad, aov_default);
         }
         // A path that ended with no shadow ray in flight is final now: its
         // radiance goes to its sample and the slot starts the next sample at
@@ -882,7 +887,15 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         // radiance in the slot record; k_regen adds them and finishes it.
         const bool ended = active && !cont;
         const bool final = ended && !sev.emit && !slv.emit;
-        if (final) p.s_rad[__float_as_uint(rad.w)] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        const uint32_t g = __float_as_uint(rad.w);
+        // radiance record: w = 1 says the sample's AOVs are the defaults and
+        // were not written
+        if (final) p.s_rad[g] = make_float4(rad.x, rad.y, rad.z, aov_default ? 1.0f : 0.0f);
+        if (aov_default && !final) {
+            p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_dep[g] = 1.0f;
+        }
         if (ended && !final) p.path[4 * (size_t)pid + 3] = rad;
         int o = wave_append(qo_count, cont);
         if (o >= 0) qo[o] = pid;
@@ -908,10 +921,15 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     for (int s = 0; s < q.frames; s++) {
         size_t pid = (size_t)(q.path_begin + s * npix + pix);
         int frame = q.frame0 + s;
-        float4 r = p.s_rad[pid];
-        float4 al = p.s_alb[pid], no = p.s_nrm[pid];
+        const float4 r = p.s_rad[pid];
+        float4 al = make_float4(0.0f, 0.0f, 0.0f, 1.0f), no = al;
+        float z = 1.0f;
+        if (r.w == 0.0f) {  // AOVs written by the shade, else the defaults
+            al = p.s_alb[pid];
+            no = p.s_nrm[pid];
+            z = p.s_dep[pid];
+        }
         float4 x = make_float4(r.x, r.y, r.z, 1.0f);
-        float z = p.s_dep[pid];
         if (isnan(r.x) || isnan(r.y) || isnan(r.z)) {
             x = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
             al = x;
