@@ -676,7 +676,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.reqs = c->reqs;
     p.nreq = 1;
     p.stack_lds = c->tuning.stack_lds;
-    p.resolve = count ? 0 : 1;  // a counting render traces every ray and tallies the ones it could skip
+    p.resolve = wave_resolve_mode(p, count);
 
     oxr_render_stats& st = c->stats;
     std::memset(&st, 0, sizeof(st));

@@ -106,14 +106,13 @@ constexpr int cover_tiles(int n) { return (n + kCoverTile - 1) / kCoverTile; }
 // MI355X), different addresses proceed in parallel.
 constexpr int kShards = 32;
 constexpr int kCtrStride = 64;  // ints between counters: one 256-byte line each
-constexpr int kQueueRegions = 3;  // per shard: survivors, new paths, new paths resolved without a trace
+constexpr int kQueueRegions = 2;  // per shard: survivors, new paths
 
 // Counter ids (int index = id * kCtrStride).
 // Lists written by the stage at parity y and read by a later stage carry a
 // parity index; see wave_trace for who writes, reads and clears each.
 enum CtrId {
-    // [parity][region]: region k survivors, kShards + k new paths to trace,
-    // 2 kShards + k new paths whose camera ray is known to leave the scene
+    // [parity][region]: region k survivors, kShards + k new paths
     kCtrQueue = 0,
     kCtrTerm = kCtrQueue + 2 * kQueueRegions * kShards,  // [parity][shard]: paths ended with shadow rays pending
     kCtrShEnv = kCtrTerm + 2 * kShards,  // [parity][shard]: env shadow rays listed
@@ -122,11 +121,12 @@ enum CtrId {
     kCtrFetchRegen = kCtrFetchTrace + kShards,
     kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
     kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
-    kCtrClass = kCtrWindowTotal + 1,           // [parity][class][shard]: entries of the shade-class queues
+    kCtrShardSamples = kCtrWindowTotal + 1,    // per shard: samples listed for tracing by the sky pass
+    kCtrClass = kCtrShardSamples + kShards,    // [parity][class][shard]: entries of the shade-class queues
     kCtrFetchClass = kCtrClass + 2 * 10 * kShards, // [class][shard]: chunks taken by the shade kernels
     kCtrIds = kCtrFetchClass + 10 * kShards
 };
-// Counter of queue x's region r (0 survivors, 1 new, 2 new resolved) of shard k.
+// Counter of queue x's region r (0 survivors, 1 new paths) of shard k.
 constexpr int queue_ctr(int x, int r, int k) { return kCtrQueue + (x * kQueueRegions + r) * kShards + k; }
 // The host's completion poll copies counters [kCtrQueue, kPollIds): the
 // queue and pending-finish counts of both parities.
@@ -164,9 +164,13 @@ struct WaveParams {
     //   [0] origin, pre_pdf   [1] direction, rng state bits
     //   [2] throughput, bounce (int bits)   [3] radiance, sample id (uint bits; kNoSample: none)
     float4* path;
-    uint32_t* queue[2];  // kQueueRegions*kShards regions of `region` slot ids each
-    uint32_t* fresh_g[2];  // sample ids of the new-path regions ((kQueueRegions-1)*kShards regions of `region`)
-    float4* hits;      // per traced queue position (2*kShards regions)
+    uint32_t* queue[2];  // 2*kShards regions of `region` slot ids each
+    uint32_t* fresh_g[2];  // sample ids of the new-path regions (kShards regions of `region`)
+    float4* hits;      // per queue position (2*kShards regions)
+    // the window's samples whose camera ray is traced, per shard in draw
+    // order (shard k's at samples + k * shard_cap), listed by the sky pass
+    uint32_t* samples;
+    int64_t shard_cap;
     // shadow rays, one record of 3 float4 per slot and kind (env / light):
     //   [0] origin, tmax   [1] direction   [2] contribution, pending flag
     // The flag is 1 when the slot's last shade emitted the ray and the trace
@@ -190,14 +194,17 @@ struct WaveParams {
     int nreq;
     // [0,1] extension-ray node fetches / triangle tests (when counting),
     // [2] extension rays, [3] shadow rays, [4,5] shadow-ray node fetches /
-    // triangle tests, [6] camera rays resolved by the cover mask (counting:
-    // that it would resolve), [7,8] node fetches / triangle tests of those
+    // triangle tests, [6] camera rays the sky pass resolved (counting: that
+    // it would resolve), [7,8] node fetches / triangle tests of those
     // (counting only; included in [0,1])
     unsigned long long* traversal;
     uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
     int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
     int* overflow;
-    int resolve;  // 1: new samples in uncovered tiles skip the trace; 0: traced (and tallied when counting)
+    // 1: the sky pass shades the samples in uncovered tiles; 2: all traced,
+    // and a counting trace tallies the ones it would shade; 0: all traced
+    // (wave_resolve_mode)
+    int resolve;
     const float4* cover_boxes;  // the scene's cover set (build_cover_set)
     const int* cover_count;
 };
@@ -206,10 +213,23 @@ constexpr int kTravCounters = 16;  // entries of WaveParams::traversal
 
 // Window set-up: rewrites frame0 / path_frame_offset / frames of request 0
 // (single renders; the server writes its table from the host), zeroes every
-// counter and stores the window's sample count.  Followed by wave_fill, which
-// starts a sample in every slot (queue 0).  grid: any wave_grid size.
+// counter and stores the window's sample count.  Then build_cover_masks,
+// wave_sky and wave_fill, which starts a listed sample in every slot (queue 0).
+// grid: any wave_grid size.
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s);
+// Sky pass: every sample of the window whose pixel tile is clear in its
+// request's cover mask is shaded here -- its camera ray leaves the scene, so
+// it meets at most an analytic light or the environment -- and the others
+// are listed per shard for tracing (WaveParams::samples).
+hipError_t wave_sky(const WaveParams& p, int grid, hipStream_t s);
+// The resolve mode for a render: the sky pass needs no medium around the
+// camera, nor lights and media together (a light inside a medium would need
+// the medium sampled); a counting render traces everything.
+inline int wave_resolve_mode(const WaveParams& p, bool counting) {
+    if (p.cam_medium != -1 || (p.n_lights > 0 && p.n_media > 0)) return 0;
+    return counting ? 2 : 1;
+}
 hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s);
 // Scene cover set: the boxes of the BVH's top levels (breadth first, at most
 // kCoverBoxes; leaves met on the way included), every triangle inside one.

@@ -56,6 +56,7 @@ struct WaveEngine {
     uint32_t *t0 = nullptr, *t1 = nullptr;    // pending-finish lists per parity
     uint32_t *sl0 = nullptr, *sl1 = nullptr;  // shadow-ray slot lists per parity
     uint32_t *g0 = nullptr, *g1 = nullptr;  // sample ids of the new-path regions
+    uint32_t* samples = nullptr;            // per-shard lists of the window's samples to trace
     int* ctr = nullptr;
     int* overflow = nullptr;
     unsigned long long* trav = nullptr;
@@ -82,6 +83,7 @@ struct WindowRun {
 };
 
 int64_t wave_round_pool(int64_t paths);   // to a positive multiple of kShards * 64
+int64_t wave_shard_cap(int64_t window);   // list capacity of one shard for a window
 int wave_grid(int blocks);                // to a multiple of the blocks that give every shard a wave
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid);
 hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_grid);

@@ -249,49 +249,25 @@ OXR_D bool camera_ray_misses(const WaveParams& p, int64_t g) {
     return q.cover[(iy / kCoverTile) * (uint32_t)q.tiles_w + ix / kCoverTile] == 0;
 }
 
-// Puts new sample g in slot pid into queue y: `trace` lanes go to the
-// new-path region (the next trace casts their camera ray), `skip` lanes to
-// the resolved region and straight onto the camera-miss shade list of parity
-// y (their camera ray leaves the scene).  o_trace / o_skip: the lane's
-// position in its region, taken by the caller.
-OXR_D void place_sample(const WaveParams& p, int k, int y, bool trace, bool skip, int o_trace, int o_skip,
-                        uint32_t pid, uint32_t g) {
-    if (trace) {
-        p.queue[y][(size_t)(kShards + k) * p.region + o_trace] = pid;
-        p.fresh_g[y][(size_t)k * p.region + o_trace] = g;
-    }
-    if (skip) {
-        const size_t qp = (size_t)(2 * kShards + k) * p.region + o_skip;
-        p.queue[y][qp] = pid;
-        p.fresh_g[y][(size_t)(kShards + k) * p.region + o_skip] = g;
-    }
-}
-OXR_D void list_skipped(const WaveParams& p, int k, int y, bool skip, int o_skip) {
-    const int o = wave_append(ctr_at(p.ctr, kCtrClass + (y * kShadeClasses + kClassCameraMiss) * kShards + k), skip);
-    if (o >= 0)
-        p.cls_queue[((size_t)kClassCameraMiss * kShards + k) * p.region + o] =
-            (uint32_t)((size_t)(2 * kShards + k) * p.region + o_skip);
-}
-
-// Draws one new sample of shard k for every lane that asks (consecutive
-// positions of the shard's sequence, one atomic per wave), starts it in the
-// lane's slot and appends the slot to queue y.  All lanes of the wave must
-// call it.  Returns whether this lane started a path.
+// Draws the next listed sample of shard k for every lane that asks
+// (consecutive positions of the shard's list, one atomic per wave), starts it
+// in the lane's slot and appends the slot to the new-path region of queue y.
+// All lanes of the wave must call it.  Returns whether this lane started a
+// path.
 OXR_D bool restart(const WaveParams& p, int k, int y, bool want, uint32_t pid) {
     const uint64_t m = __ballot(want);
     if (m == 0) return false;
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(ctr_at(p.ctr, kCtrSamplePos + k), (int)__popcll(m));
     base = __shfl(base, 0);
-    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
-    const int64_t g = sample_of(k, base + (int)__popcll(m & lanes_below()));
-    const bool start = want && g < total;
-    const bool skip = start && p.resolve && camera_ray_misses(p, g);
-    const bool trace = start && !skip;
-    const int o_trace = wave_append(ctr_at(p.ctr, queue_ctr(y, 1, k)), trace);
-    const int o_skip = wave_append(ctr_at(p.ctr, queue_ctr(y, 2, k)), skip);
-    place_sample(p, k, y, trace, skip, o_trace, o_skip, pid, (uint32_t)g);
-    list_skipped(p, k, y, skip, o_skip);
+    const int listed = *ctr_at(p.ctr, kCtrShardSamples + k);
+    const int pos = base + (int)__popcll(m & lanes_below());
+    const bool start = want && pos < listed;
+    const int o = wave_append(ctr_at(p.ctr, queue_ctr(y, 1, k)), start);
+    if (o >= 0) {
+        p.queue[y][(size_t)(kShards + k) * p.region + o] = pid;
+        p.fresh_g[y][(size_t)k * p.region + o] = p.samples[(size_t)k * p.shard_cap + pos];
+    }
     return start;
 }
 
@@ -302,7 +278,7 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
         const int id = (int)(t / kCtrStride);
         int v = 0;
         if (t % kCtrStride == 0) {
-            // k_fill draws positions 0 .. region-1 of every shard's sequence
+            // k_fill draws positions 0 .. region-1 of every shard's list
             if (id >= kCtrSamplePos && id < kCtrSamplePos + kShards) v = p.region;
             if (id == kCtrWindowTotal) v = total;
         }
@@ -315,59 +291,100 @@ __global__ void k_window_init(WaveParams p, ReqDesc* r0, int set_req0, int frame
     }
 }
 
-// Window start: slot s of shard k takes position s of the shard's draw
-// sequence and, if that sample exists, starts it (queue 0).  Waves take the
-// shard's 64-slot chunks in a fixed stride, with no fetch counter, and
-// reserve their queue positions with one atomic per region: a first pass
-// counts, a second writes.
-__global__ __launch_bounds__(256) void k_fill(WaveParams p) {
+// lightTrace (defined with the shading below).
+OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& lpdf, float& closest);
+
+// Sky pass.  Shard k owns the window's 64-sample blocks k, k + kShards, ...
+// (the order in which it draws them, so that a wave starts 64 neighbouring
+// pixels of one frame).  A wave takes the shard's blocks in a fixed stride;
+// a first pass counts the samples to trace and reserves their list positions
+// with one atomic, a second lists them and shades the others: their camera
+// ray leaves the scene, so __miss__radiance / the light hit of
+// devicePrograms.cu:342-373 at bounce 0 decide the sample, exactly as the
+// camera-miss shade would.  Needs no medium around the camera, nor one
+// around a light the ray could meet (the caller's resolve flag).
+__global__ __launch_bounds__(256) void k_sky(WaveParams p) {
     const int k = wave_shard();
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
     const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
-    const int chunks = p.region >> 6;  // region is a multiple of 64
+    const int64_t nb = (total + 63) >> 6;
+    const int blocks_k = nb > k ? (int)((nb - k + kShards - 1) / kShards) : 0;
     const int first = wave / kShards;
-    auto classify = [&](int j, bool& trace, bool& skip) -> int64_t {
-        const int slot = (j << 6) + lane_id();
-        const int64_t g = sample_of(k, slot);
-        const bool start = g < total;
-        skip = start && p.resolve && camera_ray_misses(p, g);
-        trace = start && !skip;
-        return g;
+    auto classify = [&](int j, int64_t& g) -> int {  // 0: none, 1: trace, 2: resolve here
+        g = (((int64_t)j * kShards + k) << 6) + lane_id();
+        if (g >= total) return 0;
+        return (p.resolve == 1 && camera_ray_misses(p, g)) ? 2 : 1;
     };
-    int n_trace = 0, n_skip = 0;
-    for (int j = first; j < chunks; j += stride) {
-        bool trace, skip;
-        classify(j, trace, skip);
-        n_trace += (int)__popcll(__ballot(trace));
-        n_skip += (int)__popcll(__ballot(skip));
+    int n_trace = 0;
+    for (int j = first; j < blocks_k; j += stride) {
+        int64_t g;
+        n_trace += (int)__popcll(__ballot(classify(j, g) == 1));
     }
-    int b_trace = 0, b_skip = 0, b_cls = 0;
-    if (lane_id() == 0) {
-        if (n_trace) b_trace = atomicAdd(ctr_at(p.ctr, queue_ctr(0, 1, k)), n_trace);
-        if (n_skip) {
-            b_skip = atomicAdd(ctr_at(p.ctr, queue_ctr(0, 2, k)), n_skip);
-            b_cls = atomicAdd(ctr_at(p.ctr, kCtrClass + kClassCameraMiss * kShards + k), n_skip);
+    int base = 0;
+    if (lane_id() == 0 && n_trace) base = atomicAdd(ctr_at(p.ctr, kCtrShardSamples + k), n_trace);
+    base = __shfl(base, 0);
+    uint32_t* list = p.samples + (size_t)k * p.shard_cap;
+    uint32_t resolved = 0;
+    for (int j = first; j < blocks_k; j += stride) {
+        int64_t g;
+        const int c = classify(j, g);
+        const uint64_t mt = __ballot(c == 1);
+        if (c == 1) list[base + (int)__popcll(mt & lanes_below())] = (uint32_t)g;
+        base += (int)__popcll(mt);
+        if (c != 2) continue;
+        resolved++;
+        f3 org, dir;
+        uint32_t rs;
+        camera_ray(p, g, org, dir, rs);
+        f3 radiance(0.0f), history(1.0f);
+        float closest = kFltMax, light_pdf = 0.0f;
+        f3 light_radiance(0.0f);
+        int mi_light = 0;
+        if (light_trace(p, org, dir, mi_light, light_radiance, light_pdf, closest)) {
+            const float mw = 1.0f;                                          // devicePrograms.cu:342-351
+            radiance = radiance + mw * history * light_radiance;
+            p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            p.s_dep[g] = closest / (1.0f + closest);
+            p.s_rad[g] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        } else {
+            if (p.env_w > 0) {                                              // devicePrograms.cu:354-373
+                f2 uv = sphere_to_plane(dir);
+                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                const f3 lr(e.x, e.y, e.z);
+                const float mw = 1.0f;
+                radiance = radiance + mw * history * lr;
+            }
+            // default AOVs (albedo and normal 0, depth 1), flagged in w
+            p.s_rad[g] = make_float4(radiance.x, radiance.y, radiance.z, 1.0f);
         }
     }
-    b_trace = __shfl(b_trace, 0);
-    b_skip = __shfl(b_skip, 0);
-    b_cls = __shfl(b_cls, 0);
-    for (int j = first; j < chunks; j += stride) {
-        bool trace, skip;
-        const int64_t g = classify(j, trace, skip);
-        const uint64_t mt = __ballot(trace), ms = __ballot(skip);
-        const int o_trace = b_trace + (int)__popcll(mt & lanes_below());
-        const int o_skip = b_skip + (int)__popcll(ms & lanes_below());
-        const uint32_t pid = (uint32_t)((int64_t)k * p.region + (j << 6) + lane_id());
-        place_sample(p, k, 0, trace, skip, o_trace, o_skip, pid, (uint32_t)g);
-        if (skip)
-            p.cls_queue[((size_t)kClassCameraMiss * kShards + k) * p.region + b_cls + (int)__popcll(ms & lanes_below())] =
-                (uint32_t)((size_t)(2 * kShards + k) * p.region + o_skip);
-        b_trace += (int)__popcll(mt);
-        b_skip += (int)__popcll(ms);
-        b_cls += (int)__popcll(ms);
+    wave_count_add(&p.traversal[6], resolved);
+}
+
+// Window start: slot s of shard k takes position s of the shard's list, if
+// listed.  The started slots are a prefix of the shard, so queue position s
+// of the new-path region of queue 0 is slot s and the region's count is set
+// once.  Waves take the shard's 64-slot chunks in a fixed stride.
+__global__ __launch_bounds__(256) void k_fill(WaveParams p) {
+    const int k = wave_shard();
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
+    const int listed = *ctr_at(p.ctr, kCtrShardSamples + k);
+    const int chunks = p.region >> 6;  // region is a multiple of 64
+    const size_t rbase = (size_t)k * p.region;
+    uint32_t* qn = p.queue[0] + (size_t)(kShards + k) * p.region;
+    uint32_t* gn = p.fresh_g[0] + rbase;
+    const uint32_t* list = p.samples + (size_t)k * p.shard_cap;
+    for (int j = wave / kShards; j < chunks; j += stride) {
+        const int slot = (j << 6) + lane_id();
+        if (slot < listed) {
+            qn[slot] = (uint32_t)(rbase + slot);
+            gn[slot] = list[slot];
+        }
     }
+    if (wave / kShards == 0 && lane_id() == 0) *ctr_at(p.ctr, queue_ctr(0, 1, k)) = min(listed, p.region);
 }
 
 // Adds slot pid's pending shadow contributions to r in emission order (the
@@ -451,10 +468,8 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         for (int i = t; i < kShadeClasses * kShards; i += blockDim.x)
             *ctr_at(p.ctr, kCtrClass + ((y ^ 1) * kShadeClasses) * kShards + i) = 0;
         if (t == 0) {
-            unsigned long long rays = 0, shadow = 0, resolved = 0;
+            unsigned long long rays = 0, shadow = 0;
             for (int r = 0; r < 2 * kShards; r++) rays += (unsigned)*ctr_at(p.ctr, queue_ctr(y, 0, 0) + r);
-            for (int r = 0; r < kShards; r++) resolved += (unsigned)*ctr_at(p.ctr, queue_ctr(y, 2, r));
-            if (resolved) atomicAdd(&p.traversal[6], resolved);
             for (int r = 0; r < kShards; r++)
                 shadow += (unsigned)*ctr_at(p.ctr, kCtrShEnv + (y ^ 1) * kShards + r) +
                           (unsigned)*ctr_at(p.ctr, kCtrShLight + (y ^ 1) * kShards + r);
@@ -500,7 +515,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
                 uint32_t rs;
                 const uint32_t g = fresh_sample(p, y, qp);
                 camera_ray(p, g, org, dir, rs);
-                if (COUNT) tally = camera_ray_misses(p, g);
+                if (COUNT) tally = p.resolve == 2 && camera_ray_misses(p, g);
             } else {
                 const uint32_t pid = q[qp];
                 org = xyz(p.path[4 * (size_t)pid + 0]);
@@ -875,9 +890,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
             const uint32_t qp = cq[slot];
             pid = q[qp];
             const bool fresh = CLS == kClassCameraMiss ? true : CLS == 0 ? false : is_fresh(p, qp);
-            // a camera miss may sit in the resolved region, which has no hit
-            // record; the miss classes do not read it
-            const float4 hit = MT < 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : p.hits[qp];
+            const float4 hit = MT < 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : p.hits[qp];  // unused by misses
             cont = shade_path<MT, MEDIA>(p, lut, pid, hit, fresh, fresh ? fresh_sample(p, x, qp) : 0u, sev, slv,
                                          rad, aov_default);
         }
@@ -1174,6 +1187,11 @@ hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s) {
 
 hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s) {
     k_fill<<<grid, 256, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t wave_sky(const WaveParams& p, int grid, hipStream_t s) {
+    k_sky<<<grid, 256, 0, s>>>(p);
     return hipGetLastError();
 }
 

@@ -189,7 +189,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     p.reqs = s->reqs_dev;
     p.nreq = (int)group.size();
     p.stack_lds = s->stack_lds;
-    p.resolve = 1;
+    p.resolve = wave_resolve_mode(p, false);
     WindowRun run;
     run.trace_grid = s->trace_grid;
     run.shade_grid = s->shade_grid;

@@ -34,16 +34,20 @@ constexpr size_t kPollInts = (size_t)kPollIds * kCtrStride;
 
 }  // namespace
 
+// List capacity of one shard: shard k draws the window's 64-sample blocks
+// k, k + kShards, ..., at most window / kShards + 64 samples.
+int64_t wave_shard_cap(int64_t window) { return window / kShards + 64; }
+
 int64_t wave_round_pool(int64_t paths) {
     const int64_t unit = (int64_t)kShards * 64;
     return std::max<int64_t>(unit, (paths + unit - 1) / unit * unit);
 }
 
 size_t wave_bytes(int64_t pool, int64_t window, int trace_grid) {
-    // per slot: state 64, hits 2x16, queues 2x3x4 + new-path sample ids 2x2x4, pending-finish lists 2x4,
-    // shadow records 2x48, shadow lists 2x2x4, shade classes 10x4
-    return (size_t)pool * (64 + 32 + 24 + 16 + 8 + 96 + 16 + 40) + (size_t)window * 52 + gstack_words(trace_grid) * 4 +
-           ctr_ints() * 4 + 4096;
+    // per slot: state 64, hits 2x16, queues 2x2x4 + new-path sample ids 2x4, pending-finish lists 2x4,
+    // shadow records 2x48, shadow lists 2x2x4, shade classes 10x4; per sample: outputs 52, sample list 4
+    return (size_t)pool * (64 + 32 + 16 + 8 + 8 + 96 + 16 + 40) + (size_t)window * 56 + (size_t)kShards * 64 * 4 +
+           gstack_words(trace_grid) * 4 + ctr_ints() * 4 + 4096;
 }
 
 void wave_drop_graphs(WaveEngine& e) {
@@ -56,6 +60,7 @@ void wave_release(WaveEngine& e) {
     wave_drop_graphs(e);
     wfree(e.path); wfree(e.hits); wfree(e.sh_env); wfree(e.sh_light); wfree(e.cls_queue); wfree(e.g0); wfree(e.g1);
     wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.q0); wfree(e.q1); wfree(e.gstack);
+    wfree(e.samples);
     wfree(e.t0); wfree(e.t1); wfree(e.sl0); wfree(e.sl1);
     wfree(e.ctr); wfree(e.overflow); wfree(e.trav);
     if (e.host_counts) (void)hipHostFree(e.host_counts);
@@ -95,17 +100,17 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
             (r = walloc(&e.t1, pool)) != hipSuccess || (r = walloc(&e.sl0, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.sl1, 2 * pool)) != hipSuccess ||
             (r = walloc(&e.cls_queue, (size_t)kShadeClasses * pool)) != hipSuccess ||
-            (r = walloc(&e.g0, (size_t)(kQueueRegions - 1) * pool)) != hipSuccess ||
-            (r = walloc(&e.g1, (size_t)(kQueueRegions - 1) * pool)) != hipSuccess)
+            (r = walloc(&e.g0, pool)) != hipSuccess || (r = walloc(&e.g1, pool)) != hipSuccess)
             return r;
         e.pool = pool;
     }
     if (window > e.window) {
         wave_drop_graphs(e);
-        wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep);
+        wfree(e.s_rad); wfree(e.s_alb); wfree(e.s_nrm); wfree(e.s_dep); wfree(e.samples);
         e.window = 0;
         if ((r = walloc(&e.s_rad, window)) != hipSuccess || (r = walloc(&e.s_alb, window)) != hipSuccess ||
-            (r = walloc(&e.s_nrm, window)) != hipSuccess || (r = walloc(&e.s_dep, window)) != hipSuccess)
+            (r = walloc(&e.s_nrm, window)) != hipSuccess || (r = walloc(&e.s_dep, window)) != hipSuccess ||
+            (r = walloc(&e.samples, (size_t)kShards * wave_shard_cap(window))) != hipSuccess)
             return r;
         e.window = window;
     }
@@ -129,6 +134,8 @@ void wave_bind(const WaveEngine& e, int64_t pool, WaveParams& p) {
     p.term[0] = e.t0; p.term[1] = e.t1;
     p.sh_list[0] = e.sl0; p.sh_list[1] = e.sl1;
     p.s_rad = e.s_rad; p.s_alb = e.s_alb; p.s_nrm = e.s_nrm; p.s_dep = e.s_dep;
+    p.samples = e.samples;
+    p.shard_cap = wave_shard_cap(e.window);
     p.ctr = e.ctr; p.traversal = e.trav; p.gstack = e.gstack; p.overflow = e.overflow;
 }
 
@@ -196,9 +203,10 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     if ((err = wave_window_init(p, r.set_req0, r.frame0, r.path_frame_offset, r.frames, r.total_samples, s)) !=
             hipSuccess ||
         (err = build_cover_masks(p.reqs, p.nreq, 0, p.cover_boxes, p.cover_count, s)) != hipSuccess ||
+        (err = wave_sky(p, r.shade_grid, s)) != hipSuccess ||
         (err = wave_fill(p, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
         return err;
-    r.launches += 3;
+    r.launches += 4;
 
     hipGraphExec_t exec = nullptr;
     if (!eager) {

@@ -1,6 +1,7 @@
-"""Camera-ray culling: a new sample whose pixel tile lies outside the image of
-every box of the BVH's top levels skips its first trace and goes straight to
-the camera-miss shading.  The render must not change by a single bit.
+"""Camera-ray culling: a sample whose pixel tile lies outside the image of
+every box of the BVH's top levels is shaded by the window's sky pass without
+a trace (its camera ray can meet only the environment or an analytic light).
+The render must not change by a single bit.
 
 Cases: the sphere field from its bench camera (most of the image is sky),
 the same scene seen from among the spheres (boxes straddle the camera plane)
