@@ -260,7 +260,11 @@ def main():
         sh_nodes = counted.node_fetches - counted.ext_node_fetches
         sh_tris = counted.tri_tests - counted.ext_tri_tests
         sh_bytes = NODE_BYTES * sh_nodes + TRI_BYTES * sh_tris + SHADOW_RAY_BYTES * counted.shadow_rays
-        alg = ext_bytes + sh_bytes
+        # the tail kernel traces the last bounces of the window's longest paths
+        # in its own launch: its share of the counted work is not k_trace's
+        tail_bytes = NODE_BYTES * counted.tail_node_fetches + TRI_BYTES * counted.tail_tri_tests + \
+            RAY_BYTES * counted.tail_extension_rays + SHADOW_RAY_BYTES * counted.tail_shadow_rays
+        alg = ext_bytes + sh_bytes - tail_bytes
         achieved = alg / (float(np.mean(trace_ms)) / 1e3) / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -274,6 +278,7 @@ def main():
         }
         out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
                               "camera_rays_resolved_without_trace": int(counted.camera_rays_resolved),
+                              "tail_kernel_rays": int(counted.tail_extension_rays + counted.tail_shadow_rays),
                               "traced_rays_per_s": round((counted.extension_rays - counted.camera_rays_resolved +
                                                           counted.shadow_rays) /
                                                   (ms_per_step / 1e3), 0)})

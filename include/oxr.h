@@ -145,6 +145,11 @@ typedef struct oxr_render_stats {
        they took (included in ext_node_fetches / ext_tri_tests). */
     uint64_t camera_rays_resolved;
     uint64_t resolved_node_fetches, resolved_tri_tests;
+    /* Rays traced by the tail kernel (oxr_tuning.tail_paths), included in
+       extension_rays / shadow_rays, and (when counting) their node fetches /
+       triangle tests, included in node_fetches / tri_tests. */
+    uint64_t tail_extension_rays, tail_shadow_rays;
+    uint64_t tail_node_fetches, tail_tri_tests;
 } oxr_render_stats;
 
 /* ---- context and memory (CUDABuffer-style API, common/CUDABuffer.h:26-81) ---- */
@@ -253,6 +258,8 @@ typedef struct oxr_tuning {
     int32_t iters_per_poll;       /* wavefront iterations per graph replay between completion polls (even, 2..64) */
     int64_t pool_paths;           /* resident path slots (rounded up to a multiple of 2048) */
     int64_t window_samples;       /* samples whose radiance and AOVs are kept at once (48 B each) */
+    int64_t tail_paths;           /* an iteration with at most this many paths carries each to its end in one
+                                     launch (tail mode); 0 = never */
 } oxr_tuning;
 int oxr_get_tuning(oxr_ctx* ctx, oxr_tuning* out);
 int oxr_set_tuning(oxr_ctx* ctx, const oxr_tuning* t);

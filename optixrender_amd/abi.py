@@ -211,6 +211,10 @@ class RenderStats(C.Structure):
         ("camera_rays_resolved", C.c_uint64),
         ("resolved_node_fetches", C.c_uint64),
         ("resolved_tri_tests", C.c_uint64),
+        ("tail_extension_rays", C.c_uint64),
+        ("tail_shadow_rays", C.c_uint64),
+        ("tail_node_fetches", C.c_uint64),
+        ("tail_tri_tests", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -263,6 +267,7 @@ class Tuning(C.Structure):
         ("iters_per_poll", C.c_int32),
         ("pool_paths", C.c_int64),
         ("window_samples", C.c_int64),
+        ("tail_paths", C.c_int64),
     ]
 
     def as_dict(self):

@@ -298,6 +298,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.iters_per_poll = 4;
     c->tuning.pool_paths = 1 << 24;
     c->tuning.window_samples = 1 << 27;
+    c->tuning.tail_paths = 1 << 17;
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
     if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
@@ -677,6 +678,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     p.nreq = 1;
     p.stack_lds = c->tuning.stack_lds;
     p.resolve = wave_resolve_mode(p, count);
+    p.tail_paths = c->tuning.tail_paths;
 
     oxr_render_stats& st = c->stats;
     std::memset(&st, 0, sizeof(st));
@@ -752,6 +754,10 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.camera_rays_resolved = tv[6];
     st.resolved_node_fetches = tv[7];
     st.resolved_tri_tests = tv[8];
+    st.tail_node_fetches = tv[9];
+    st.tail_tri_tests = tv[10];
+    st.tail_extension_rays = tv[11];
+    st.tail_shadow_rays = tv[12];
     st.paths = (uint64_t)prm->spp * npix;
     st.kernel_launches = launches;
     st.iterations = iterations;
@@ -774,7 +780,8 @@ int oxr_set_tuning(oxr_ctx* c, const oxr_tuning* t) {
     if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > 16 || t->shade_blocks_per_cu < 1 ||
         t->shade_blocks_per_cu > 16 || !(t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32) ||
         t->iters_per_poll < 2 || t->iters_per_poll > 64 || (t->iters_per_poll & 1) || t->pool_paths < 1 ||
-        t->pool_paths > ((int64_t)1 << 28) || t->window_samples < 1 || t->window_samples > ((int64_t)1 << 30))
+        t->pool_paths > ((int64_t)1 << 28) || t->window_samples < 1 || t->window_samples > ((int64_t)1 << 30) ||
+        t->tail_paths < 0)
         return fail(c, OXR_ERR_INVALID, "tuning out of range");
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     wave_drop_graphs(c->eng);
@@ -806,14 +813,17 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
     if (rc != OXR_OK) return rc;
     const std::vector<int> stacks = {8, 16, 32}, tracebpc = {4, 8, 12, 16}, shadebpc = {2, 4, 8, 16};
     const std::vector<int64_t> pools = {1 << 21, 1 << 22, 1 << 23, 1 << 24};
-    for (int knob = 0; knob < 4; knob++) {
-        size_t nc = knob == 0 ? stacks.size() : knob == 1 ? tracebpc.size() : knob == 2 ? shadebpc.size() : pools.size();
+    const std::vector<int64_t> tails = {0, 1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 19};
+    const size_t counts[5] = {stacks.size(), tracebpc.size(), shadebpc.size(), pools.size(), tails.size()};
+    for (int knob = 0; knob < 5; knob++) {
+        const size_t nc = counts[knob];
         for (size_t i = 0; i < nc; i++) {
             oxr_tuning t = cur;
             if (knob == 0) t.stack_lds = stacks[i];
             if (knob == 1) t.trace_blocks_per_cu = tracebpc[i];
             if (knob == 2) t.shade_blocks_per_cu = shadebpc[i];
             if (knob == 3) t.pool_paths = pools[i];
+            if (knob == 4) t.tail_paths = tails[i];
             if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0.0;
             rc = time_it(t, ms);

@@ -122,7 +122,8 @@ enum CtrId {
     kCtrSamplePos = kCtrFetchRegen + kShards,  // per shard: samples drawn so far
     kCtrWindowTotal = kCtrSamplePos + kShards, // samples in the current window
     kCtrShardSamples = kCtrWindowTotal + 1,    // per shard: samples listed for tracing by the sky pass
-    kCtrClass = kCtrShardSamples + kShards,    // [parity][class][shard]: entries of the shade-class queues
+    kCtrTailMode = kCtrShardSamples + kShards, // 1 while the current iteration runs in tail mode (k_tail)
+    kCtrClass = kCtrTailMode + 1,              // [parity][class][shard]: entries of the shade-class queues
     kCtrFetchClass = kCtrClass + 2 * 10 * kShards, // [class][shard]: chunks taken by the shade kernels
     kCtrIds = kCtrFetchClass + 10 * kShards
 };
@@ -196,7 +197,9 @@ struct WaveParams {
     // [2] extension rays, [3] shadow rays, [4,5] shadow-ray node fetches /
     // triangle tests, [6] camera rays the sky pass resolved (counting: that
     // it would resolve), [7,8] node fetches / triangle tests of those
-    // (counting only; included in [0,1])
+    // (counting only; included in [0,1]), [9,10] node fetches / triangle
+    // tests of the tail kernel's rays (counting only; included in [0,1,4,5]),
+    // [11,12] extension / shadow rays the tail kernel traced (in [2,3])
     unsigned long long* traversal;
     uint32_t* gstack;               // traversal stack spill, (kStackTotal - kStackLdsMin) * grid * kTraceBlock
     int stack_lds;                  // 8, 16 or 32 stack entries in LDS (selects the kernel instance)
@@ -205,6 +208,10 @@ struct WaveParams {
     // and a counting trace tallies the ones it would shade; 0: all traced
     // (wave_resolve_mode)
     int resolve;
+    // tail mode: an iteration whose queue holds at most tail_paths paths
+    // (and at least one) shades them in k_tail, each carried to its end in
+    // that launch, instead of one launch per bounce; 0 = never
+    int64_t tail_paths;
     const float4* cover_boxes;  // the scene's cover set (build_cover_set)
     const int* cover_count;
 };
@@ -252,7 +259,10 @@ hipError_t build_cover_masks(const ReqDesc* reqs, int nreq, int max_tiles, const
 // refills (queue y^1 regions, pending finishes and shadow rays of parity y,
 // shade classes of parity y^1); fetch counters by the stage before their user.
 hipError_t wave_trace(const WaveParams& p, int y, int grid, bool count, hipStream_t s);
-hipError_t wave_shade(const WaveParams& p, int y, int grid, hipStream_t s);
+// shade(y) as the shade-class launches and the tail kernel: the class
+// kernels return at once in tail mode, the tail kernel at once otherwise.
+// trace_grid: the tail kernel traverses, so it uses the trace grid's stacks.
+hipError_t wave_shade(const WaveParams& p, int y, int grid, int trace_grid, bool count, hipStream_t s);
 hipError_t wave_regen(const WaveParams& p, int y, int grid, hipStream_t s);
 hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
 // ---- denoiser (denoise.hip) ----

@@ -217,11 +217,13 @@ OXR_D int64_t sample_of(int k, int pos) { return (((int64_t)(pos >> 6) * kShards
 // once the path survives its first shade.
 OXR_D void camera_ray(const WaveParams& p, int64_t g, f3& org, f3& dir, uint32_t& rng_state) {
     const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, g)];
-    const int64_t local = g - q.path_begin;
-    const int npix = q.width * q.height;
-    int s = (int)(local / npix), pix = (int)(local % npix);
-    int ix = pix % q.width, iy = pix / q.width;
-    Rng rng{lcg_seed((uint32_t)(ix + q.width * iy), (uint32_t)(q.frame0 + s))};
+    // a window holds fewer than 2^31 samples: 32-bit division is exact
+    const uint32_t local = (uint32_t)(g - q.path_begin);
+    const uint32_t npix = (uint32_t)(q.width * q.height), w = (uint32_t)q.width;
+    const uint32_t s = local / npix, pix = local - s * npix;
+    const uint32_t iyu = pix / w;
+    const int ix = (int)(pix - iyu * w), iy = (int)iyu;
+    Rng rng{lcg_seed((uint32_t)(ix + q.width * iy), (uint32_t)q.frame0 + s)};
     float j0 = rng(), j1 = rng();
     f2 jit = filter_jitter(f2{j0, j1}, q.filter);                         // devicePrograms.cu:220-227
     float sx = ((float)ix + 0.5f + jit.x) / (float)q.width;
@@ -245,7 +247,7 @@ OXR_D bool camera_ray_misses(const WaveParams& p, int64_t g) {
     const uint32_t local = (uint32_t)(g - q.path_begin);  // a window holds fewer than 2^31 samples
     const uint32_t w = (uint32_t)q.width;
     const uint32_t pix = local % (w * (uint32_t)q.height);
-    const uint32_t ix = pix % w, iy = pix / w;
+    const uint32_t iy = pix / w, ix = pix - iy * w;
     return q.cover[(iy / kCoverTile) * (uint32_t)q.tiles_w + ix / kCoverTile] == 0;
 }
 
@@ -475,6 +477,7 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
                           (unsigned)*ctr_at(p.ctr, kCtrShLight + (y ^ 1) * kShards + r);
             if (rays) atomicAdd(&p.traversal[2], rays);
             if (shadow) atomicAdd(&p.traversal[3], shadow);
+            *ctr_at(p.ctr, kCtrTailMode) = (rays > 0 && (int64_t)rays <= p.tail_paths) ? 1 : 0;
         }
     }
     const int k = wave_shard();
@@ -858,6 +861,7 @@ OXR_D void emit_shadow(float4* recs, uint32_t* list, int* ctr, const ShadowOut& 
 template <int CLS, bool MEDIA>
 __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
     constexpr int MT = (CLS == 0 || CLS == kClassCameraMiss) ? -1 : CLS - 1;
+    if (*ctr_at(p.ctr, kCtrTailMode)) return;  // k_tail shades this iteration
     const int k = wave_shard();
     const int n = *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + CLS) * kShards + k);
     const int chunks = (n + 63) >> 6;
@@ -918,6 +922,110 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams p, int x) {
         if (p.env_w > 0) emit_shadow(p.sh_env, le, se_count, sev, pid, pending);
         if (p.n_lights != 0) emit_shadow(p.sh_light, ll, sl_count, slv, pid, pending);
         restart(p, k, x ^ 1, final, pid);
+    }
+}
+
+// A shadow ray in the tail kernel: traced at once and recorded as the slot's
+// next shade (or the finish) reads it in add_pending.
+template <bool COUNT, class StackT>
+OXR_D void tail_shadow(const WaveParams& p, float4* recs, const ShadowOut& s, uint32_t pid, bool pending,
+                       StackT& st, uint32_t& sn, uint32_t& stt, uint32_t& rays) {
+    float4* rec = recs + 3 * (size_t)pid;
+    if (s.emit) {
+        rays++;
+        const Hit h = traverse<true, COUNT>(p.bvh, s.o, s.d, kEps, s.tmax, st, sn, stt, p.overflow);
+        rec[2] = make_float4(s.c.x, s.c.y, s.c.z, h.leaf == kNoHit ? 1.0f : 0.0f);
+    } else if (pending) {
+        reinterpret_cast<float*>(rec + 2)[3] = 0.0f;
+    }
+}
+
+// Tail mode (k_trace sets the flag when queue x holds few paths): the paths
+// listed under every shade class of parity x are carried to their end here,
+// each lane one path: shade (run-time material dispatch), trace its shadow
+// rays at once, finish it or trace its next ray, and so on; then the slot
+// starts a new sample if any is left (queue x^1).  The operations and their
+// order per path are those of the wavefront, so the result is the same.
+// Launched on the trace grid: the traversal stacks are that grid's.
+template <bool COUNT, bool MEDIA>
+__global__ __launch_bounds__(kTraceBlock) void k_tail(WaveParams p, int x) {
+    constexpr int S = 16;
+    if (!*ctr_at(p.ctr, kCtrTailMode)) return;
+    __shared__ uint32_t lds[S * kTraceBlock];
+    const int k = wave_shard();
+    int first[kShadeClasses + 1];
+    first[0] = 0;
+    for (int c = 0; c < kShadeClasses; c++)
+        first[c + 1] = first[c] + ((*ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + c) * kShards + k) + 63) >> 6);
+    const int chunks = first[kShadeClasses];
+    if (chunks == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchClass + k);  // class 0's counter: the class kernels are idle
+    const uint32_t* q = p.queue[x];
+    Stack<S> st(lds, p.gstack, (int64_t)gridDim.x * blockDim.x);
+    Lut lut{p.lut_bsdf, p.lut_albedo};
+    uint32_t nn = 0, nt = 0, sn = 0, stt = 0, ext_rays = 0, sh_rays = 0;
+    while (true) {
+        const int j = wave_next(fetch);
+        if (j >= chunks) break;
+        int c = 0;
+        while (j >= first[c + 1]) c++;
+        const int n = *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + c) * kShards + k);
+        const int slot = ((j - first[c]) << 6) + lane_id();
+        const bool active = slot < n;
+        uint32_t pid = 0;
+        if (active) {
+            const uint32_t qp = p.cls_queue[((size_t)c * kShards + k) * p.region + slot];
+            pid = q[qp];
+            bool fresh = is_fresh(p, qp);
+            uint32_t g_fresh = fresh ? fresh_sample(p, x, qp) : 0u;
+            float4 hit = (c == 0 || c == kClassCameraMiss) ? make_float4(kFltMax, __uint_as_float(kNoHit), 0.0f, 0.0f)
+                                                           : p.hits[qp];
+            while (true) {
+                ShadowOut sev, slv;
+                sev.emit = slv.emit = false;
+                float4 rad;
+                bool aov_default = false;
+                const bool cont = shade_path<kMatAny, MEDIA>(p, lut, pid, hit, fresh, g_fresh, sev, slv, rad,
+                                                             aov_default);
+                fresh = false;
+                const bool final = !cont && !sev.emit && !slv.emit;
+                if (p.env_w > 0) tail_shadow<COUNT>(p, p.sh_env, sev, pid, !final, st, sn, stt, sh_rays);
+                if (p.n_lights != 0) tail_shadow<COUNT>(p, p.sh_light, slv, pid, !final, st, sn, stt, sh_rays);
+                if (!cont) {
+                    const uint32_t g = __float_as_uint(rad.w);
+                    if (final) {
+                        p.s_rad[g] = make_float4(rad.x, rad.y, rad.z, aov_default ? 1.0f : 0.0f);
+                    } else {
+                        float4 r = rad;
+                        add_pending(p, pid, r);  // as k_regen finishes a path
+                        p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
+                    }
+                    if (aov_default && !final) {
+                        p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                        p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                        p.s_dep[g] = 1.0f;
+                    }
+                    break;
+                }
+                ext_rays++;
+                const f3 org = xyz(p.path[4 * (size_t)pid + 0]), dir = xyz(p.path[4 * (size_t)pid + 1]);
+                const Hit h = traverse<false, COUNT>(p.bvh, org, dir, kEps, kFltMax, st, nn, nt, p.overflow);
+                hit = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
+            }
+        }
+        restart(p, k, x ^ 1, active, pid);
+    }
+    wave_count_add(&p.traversal[2], ext_rays);
+    wave_count_add(&p.traversal[3], sh_rays);
+    wave_count_add(&p.traversal[11], ext_rays);
+    wave_count_add(&p.traversal[12], sh_rays);
+    if (COUNT) {
+        wave_count_add(&p.traversal[0], nn);
+        wave_count_add(&p.traversal[1], nt);
+        wave_count_add(&p.traversal[4], sn);
+        wave_count_add(&p.traversal[5], stt);
+        wave_count_add(&p.traversal[9], nn + sn);
+        wave_count_add(&p.traversal[10], nt + stt);
     }
 }
 
@@ -1223,9 +1331,18 @@ void launch_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
     if (m & 512u) k_shade<kClassCameraMiss, MEDIA><<<grid, 256, 0, s>>>(p, x);
 }
 
-hipError_t wave_shade(const WaveParams& p, int x, int grid, hipStream_t s) {
+hipError_t wave_shade(const WaveParams& p, int x, int grid, int trace_grid, bool count, hipStream_t s) {
     if (p.n_media != 0) launch_shade<true>(p, x, grid, s);
     else launch_shade<false>(p, x, grid, s);
+    if (p.tail_paths > 0) {
+        if (p.n_media != 0) {
+            if (count) k_tail<true, true><<<trace_grid, kTraceBlock, 0, s>>>(p, x);
+            else k_tail<false, true><<<trace_grid, kTraceBlock, 0, s>>>(p, x);
+        } else {
+            if (count) k_tail<true, false><<<trace_grid, kTraceBlock, 0, s>>>(p, x);
+            else k_tail<false, false><<<trace_grid, kTraceBlock, 0, s>>>(p, x);
+        }
+    }
     return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ struct oxr_server {
     int max_active = 0;
     int64_t tick_paths = 0, cap = 0, pool = 0;
     int trace_grid = 0, shade_grid = 0, stack_lds = 32, iters_per_poll = 4;
+    int64_t tail_paths = 0;
     WaveEngine eng;  // shared path pool; a tick is one sample window
     void* dn_scratch = nullptr;  // denoiser activations for one frame of max_frame_pixels
     ReqDesc* reqs_dev = nullptr;
@@ -190,6 +191,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     p.nreq = (int)group.size();
     p.stack_lds = s->stack_lds;
     p.resolve = wave_resolve_mode(p, false);
+    p.tail_paths = s->tail_paths;
     WindowRun run;
     run.trace_grid = s->trace_grid;
     run.shade_grid = s->shade_grid;
@@ -319,6 +321,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->trace_grid = c->trace_grid;
     s->shade_grid = c->shade_grid;
     s->stack_lds = c->tuning.stack_lds;
+    s->tail_paths = c->tuning.tail_paths;
     s->iters_per_poll = c->tuning.iters_per_poll;
     s->pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, s->cap));
     s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid) + denoise_scratch_bytes((int)cfg->max_frame_pixels, 1);

@@ -537,7 +537,9 @@ OXR_D f3 sample_material(const Surf& s, const Lut& lut, Rng& rng, f3 V, f3& L, f
 
 // The same dispatch fixed at compile time, for a shade kernel that only sees
 // material type MT.  MT < 0 is the class of rays that left the scene, which
-// never evaluate a BSDF: the calls compile to nothing.
+// never evaluate a BSDF: the calls compile to nothing.  MT == kMatAny keeps
+// the run-time dispatch (a kernel that shades every class).
+constexpr int kMatAny = 8;
 template <int MT>
 OXR_D f3 eval_material_t(const Surf& s, const Lut& lut, f3 V, f3 L, float& pdf) {
     pdf = 0.0f;
@@ -549,6 +551,7 @@ OXR_D f3 eval_material_t(const Surf& s, const Lut& lut, f3 V, f3 L, float& pdf) 
     else if constexpr (MT == 5) return eval_thin(s, V, L, pdf);
     else if constexpr (MT == 6) return eval_clearcoat(s, lut, V, L, pdf);
     else if constexpr (MT == 7) return eval_transmitter(s, V, L, pdf);
+    else if constexpr (MT == kMatAny) return eval_material(s, lut, V, L, pdf);
     else return f3(0.0f);
 }
 template <int MT>
@@ -562,6 +565,7 @@ OXR_D f3 sample_material_t(const Surf& s, const Lut& lut, Rng& rng, f3 V, f3& L,
     else if constexpr (MT == 5) return sample_thin(s, rng, V, L, pdf);
     else if constexpr (MT == 6) return sample_clearcoat(s, lut, rng, V, L, pdf);
     else if constexpr (MT == 7) return sample_transmitter(s, rng, V, L, pdf);
+    else if constexpr (MT == kMatAny) return sample_material(s, lut, rng, V, L, pdf);
     else return f3(0.0f);
 }
 

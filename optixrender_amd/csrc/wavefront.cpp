@@ -186,11 +186,12 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
             const int y = it & 1;
             if ((x = wave_trace(p, y, r.trace_grid, r.count, s)) != hipSuccess || (x = mark(1)) != hipSuccess) return x;
             if ((x = wave_regen(p, y, r.shade_grid, s)) != hipSuccess || (x = mark(0)) != hipSuccess) return x;
-            if ((x = wave_shade(p, y, r.shade_grid, s)) != hipSuccess || (x = mark(2)) != hipSuccess) return x;
+            if ((x = wave_shade(p, y, r.shade_grid, r.trace_grid, r.count, s)) != hipSuccess || (x = mark(2)) != hipSuccess)
+                return x;
         }
         return hipSuccess;
     };
-    const int per_iter = 2 + __builtin_popcount(p.class_mask);
+    const int per_iter = 2 + __builtin_popcount(p.class_mask) + (p.tail_paths > 0 ? 1 : 0);
 
     if (timing) {
         if (e.stage_events.empty()) {

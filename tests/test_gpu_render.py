@@ -100,21 +100,23 @@ def test_stage_timing_and_counts(ctx, cornell):
     assert st.trace_ms > 0 and st.shade_ms > 0
 
 
-# (stack_lds, trace blocks/CU, shade blocks/CU, iterations per poll, pool slots, window samples).
-# A pool of 2048 slots (64 per shard) for a 48x40x4 render restarts every
-# slot several times, and a window of one frame splits the render in four.
-@pytest.mark.parametrize("stack,tbpc,sbpc,iters,pool,window", [
-    (8, 4, 2, 2, 2048, 1 << 12),
-    (16, 16, 16, 6, 4096, W * H),
-    (32, 8, 8, 4, 1 << 23, 1 << 27),
-    (32, 1, 1, 64, 2048, 3 * W * H),
+# (stack_lds, trace blocks/CU, shade blocks/CU, iterations per poll, pool slots, window samples,
+# tail paths).  A pool of 2048 slots (64 per shard) for a 48x40x4 render
+# restarts every slot several times, and a window of one frame splits the
+# render in four; tail mode from the first iteration on (1 << 20), from a
+# few hundred paths on, and never (0).
+@pytest.mark.parametrize("stack,tbpc,sbpc,iters,pool,window,tail", [
+    (8, 4, 2, 2, 2048, 1 << 12, 0),
+    (16, 16, 16, 6, 4096, W * H, 300),
+    (32, 8, 8, 4, 1 << 23, 1 << 27, 1 << 20),
+    (32, 1, 1, 64, 2048, 3 * W * H, 1000),
 ])
-def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, iters, pool, window):
+def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, iters, pool, window, tail):
     sc, sid = cornell
     base = ctx.get_tuning()
     try:
         ref, _, _ = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
-        t = abi.Tuning(tbpc, sbpc, stack, iters, pool, window)
+        t = abi.Tuning(tbpc, sbpc, stack, iters, pool, window, tail)
         print(f"[render] tuning {t.as_dict()}", flush=True)
         ctx.set_tuning(t)
         got, _, st = render(ctx, sc, sid, spp=4, max_depth=8, rr_start=3)
@@ -126,9 +128,9 @@ def test_tuning_does_not_change_result(ctx, cornell, stack, tbpc, sbpc, iters, p
 
 def test_invalid_tuning_is_rejected(ctx):
     base = ctx.get_tuning()
-    for bad in [abi.Tuning(0, 8, 32, 4, 1 << 20, 1 << 20), abi.Tuning(8, 8, 24, 4, 1 << 20, 1 << 20),
-                abi.Tuning(8, 8, 32, 3, 1 << 20, 1 << 20), abi.Tuning(8, 8, 32, 4, 0, 1 << 20),
-                abi.Tuning(8, 8, 32, 4, 1 << 20, 0)]:
+    for bad in [abi.Tuning(0, 8, 32, 4, 1 << 20, 1 << 20, 0), abi.Tuning(8, 8, 24, 4, 1 << 20, 1 << 20, 0),
+                abi.Tuning(8, 8, 32, 3, 1 << 20, 1 << 20, 0), abi.Tuning(8, 8, 32, 4, 0, 1 << 20, 0),
+                abi.Tuning(8, 8, 32, 4, 1 << 20, 0, 0), abi.Tuning(8, 8, 32, 4, 1 << 20, 1 << 20, -1)]:
         with pytest.raises(abi.OxrError):
             ctx.set_tuning(bad)
     assert ctx.get_tuning().as_dict() == base.as_dict()
