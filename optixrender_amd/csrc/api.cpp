@@ -298,7 +298,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.iters_per_poll = 4;
     c->tuning.pool_paths = 1 << 24;
     c->tuning.window_samples = 1 << 27;
-    c->tuning.tail_paths = 1 << 17;
+    c->tuning.tail_paths = 1 << 18;
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
     if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
