@@ -222,6 +222,7 @@ void apply_tuning(oxr_ctx* c) {
     const int sb = std::min(c->tuning.shade_blocks_per_cu, wave_shade_occupancy());
     c->trace_grid = wave_grid(c->num_cu * tb);
     c->shade_grid = wave_grid(c->num_cu * sb);
+    c->sky_grid = c->shade_grid;
 }
 
 SceneDev* selected(oxr_ctx* c) {
@@ -688,6 +689,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     WindowRun r;
     r.trace_grid = c->trace_grid;
     r.shade_grid = c->shade_grid;
+    r.sky_grid = c->sky_grid;
     r.iters_per_poll = c->tuning.iters_per_poll;
     r.count = count;
     r.stage_timing = (prm->flags & OXR_FLAG_STAGE_TIMING) != 0;

@@ -226,10 +226,13 @@ constexpr int kTravCounters = 16;  // entries of WaveParams::traversal
 hipError_t wave_window_init(const WaveParams& p, bool set_req0, int frame0, int path_frame_offset, int frames,
                             int64_t total_samples, hipStream_t s);
 // Sky pass: every sample of the window whose pixel tile is clear in its
-// request's cover mask is shaded here -- its camera ray leaves the scene, so
-// it meets at most an analytic light or the environment -- and the others
-// are listed per shard for tracing (WaveParams::samples).
-hipError_t wave_sky(const WaveParams& p, int grid, hipStream_t s);
+// request's cover mask is shaded by wave_sky_shade -- its camera ray leaves
+// the scene, so it meets at most an analytic light or the environment --
+// and wave_sky_list lists the others per shard for tracing
+// (WaveParams::samples).  The shading is read only by the accumulation, so
+// it may run on another stream beside the path loop.  grid: a wave_grid size.
+hipError_t wave_sky_list(const WaveParams& p, int grid, hipStream_t s);
+hipError_t wave_sky_shade(const WaveParams& p, int grid, hipStream_t s);
 // The resolve mode for a render: the sky pass needs no medium around the
 // camera, nor lights and media together (a light inside a medium would need
 // the medium sampled); a counting render traces everything.

@@ -62,6 +62,9 @@ struct WaveEngine {
     unsigned long long* trav = nullptr;
     int* host_counts = nullptr;  // pinned: two copies of the polled counters
     hipEvent_t poll[2] = {nullptr, nullptr};
+    // the sky shading runs on `side` beside the path loop (default priority)
+    hipStream_t side = nullptr;
+    hipEvent_t side_start = nullptr, side_done = nullptr;
     std::vector<hipEvent_t> stage_events;
     struct Graph {
         std::vector<uint8_t> key;
@@ -73,12 +76,13 @@ struct WaveEngine {
 // One window: launch shape, options, and what it measured.
 struct WindowRun {
     int trace_grid = 0, shade_grid = 0;
+    int sky_grid = 0;  // the sky shading beside the path loop
     int iters_per_poll = 4;
     bool count = false, eager = false, stage_timing = false;
     bool set_req0 = false;  // write frame0/offset/frames into request 0 on the device
     int frame0 = 0, path_frame_offset = 0, frames = 0;
     int64_t total_samples = 0, npix_total = 0;
-    double stage_ms[5] = {0, 0, 0, 0, 0};  // regen (with window start), extend, shade, connect, accumulate
+    double stage_ms[5] = {0, 0, 0, 0, 0};  // regen (with window start), trace, shade (with tail), -, accumulate
     int launches = 0, iterations = 0;
 };
 
@@ -113,7 +117,7 @@ struct oxr_ctx {
     oxr_camera cam;
     int frame_id = 0;
     oxr::WaveEngine eng;
-    int trace_grid = 0, shade_grid = 0;
+    int trace_grid = 0, shade_grid = 0, sky_grid = 0;
     oxr_tuning tuning;
     float* path_radiance = nullptr;
     size_t path_radiance_floats = 0;

@@ -298,43 +298,62 @@ OXR_D bool light_trace(const WaveParams& p, f3 o, f3 d, int& mi, f3& Lr, float& 
 
 // Sky pass.  Shard k owns the window's 64-sample blocks k, k + kShards, ...
 // (the order in which it draws them, so that a wave starts 64 neighbouring
-// pixels of one frame).  A wave takes the shard's blocks in a fixed stride;
-// a first pass counts the samples to trace and reserves their list positions
-// with one atomic, a second lists them and shades the others: their camera
-// ray leaves the scene, so __miss__radiance / the light hit of
-// devicePrograms.cu:342-373 at bounce 0 decide the sample, exactly as the
-// camera-miss shade would.  Needs no medium around the camera, nor one
-// around a light the ray could meet (the caller's resolve flag).
-__global__ __launch_bounds__(256) void k_sky(WaveParams p) {
+// pixels of one frame).  Two kernels share the classification:
+//   k_sky_list: lists the samples to trace.  A wave takes the shard's blocks
+//     in a fixed stride; a first pass counts them and reserves their list
+//     positions with one atomic, a second writes them.
+//   k_sky_shade: shades the others.  Their camera ray leaves the scene, so
+//     __miss__radiance / the light hit of devicePrograms.cu:342-373 at bounce
+//     0 decide the sample, exactly as the camera-miss shade would.  Nothing
+//     reads these samples before k_accumulate, so it runs beside the path
+//     loop on a second stream.
+// Needs no medium around the camera, nor one around a light the ray could
+// meet (the caller's resolve mode).
+OXR_D int sky_class(const WaveParams& p, int k, int j, int64_t total, int64_t& g) {  // 0 none, 1 trace, 2 here
+    g = (((int64_t)j * kShards + k) << 6) + lane_id();
+    if (g >= total) return 0;
+    return (p.resolve == 1 && camera_ray_misses(p, g)) ? 2 : 1;
+}
+OXR_D int sky_blocks(int k, int64_t total) {
+    const int64_t nb = (total + 63) >> 6;
+    return nb > k ? (int)((nb - k + kShards - 1) / kShards) : 0;
+}
+
+__global__ __launch_bounds__(256) void k_sky_list(WaveParams p) {
     const int k = wave_shard();
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
     const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
-    const int64_t nb = (total + 63) >> 6;
-    const int blocks_k = nb > k ? (int)((nb - k + kShards - 1) / kShards) : 0;
+    const int blocks_k = sky_blocks(k, total);
     const int first = wave / kShards;
-    auto classify = [&](int j, int64_t& g) -> int {  // 0: none, 1: trace, 2: resolve here
-        g = (((int64_t)j * kShards + k) << 6) + lane_id();
-        if (g >= total) return 0;
-        return (p.resolve == 1 && camera_ray_misses(p, g)) ? 2 : 1;
-    };
     int n_trace = 0;
     for (int j = first; j < blocks_k; j += stride) {
         int64_t g;
-        n_trace += (int)__popcll(__ballot(classify(j, g) == 1));
+        n_trace += (int)__popcll(__ballot(sky_class(p, k, j, total, g) == 1));
     }
     int base = 0;
     if (lane_id() == 0 && n_trace) base = atomicAdd(ctr_at(p.ctr, kCtrShardSamples + k), n_trace);
     base = __shfl(base, 0);
     uint32_t* list = p.samples + (size_t)k * p.shard_cap;
-    uint32_t resolved = 0;
     for (int j = first; j < blocks_k; j += stride) {
         int64_t g;
-        const int c = classify(j, g);
-        const uint64_t mt = __ballot(c == 1);
-        if (c == 1) list[base + (int)__popcll(mt & lanes_below())] = (uint32_t)g;
+        const bool tr = sky_class(p, k, j, total, g) == 1;
+        const uint64_t mt = __ballot(tr);
+        if (tr) list[base + (int)__popcll(mt & lanes_below())] = (uint32_t)g;
         base += (int)__popcll(mt);
-        if (c != 2) continue;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sky_shade(WaveParams p) {
+    const int k = wave_shard();
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
+    const int64_t total = *ctr_at(p.ctr, kCtrWindowTotal);
+    const int blocks_k = sky_blocks(k, total);
+    uint32_t resolved = 0;
+    for (int j = wave / kShards; j < blocks_k; j += stride) {
+        int64_t g;
+        if (sky_class(p, k, j, total, g) != 2) continue;
         resolved++;
         f3 org, dir;
         uint32_t rs;
@@ -1298,8 +1317,13 @@ hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t wave_sky(const WaveParams& p, int grid, hipStream_t s) {
-    k_sky<<<grid, 256, 0, s>>>(p);
+hipError_t wave_sky_list(const WaveParams& p, int grid, hipStream_t s) {
+    k_sky_list<<<grid, 256, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t wave_sky_shade(const WaveParams& p, int grid, hipStream_t s) {
+    k_sky_shade<<<grid, 256, 0, s>>>(p);
     return hipGetLastError();
 }
 

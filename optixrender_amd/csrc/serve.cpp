@@ -65,7 +65,7 @@ struct oxr_server {
     size_t budget = 0, wave_bytes = 0, in_use = 0;
     int max_active = 0;
     int64_t tick_paths = 0, cap = 0, pool = 0;
-    int trace_grid = 0, shade_grid = 0, stack_lds = 32, iters_per_poll = 4;
+    int trace_grid = 0, shade_grid = 0, sky_grid = 0, stack_lds = 32, iters_per_poll = 4;
     int64_t tail_paths = 0;
     WaveEngine eng;  // shared path pool; a tick is one sample window
     void* dn_scratch = nullptr;  // denoiser activations for one frame of max_frame_pixels
@@ -195,6 +195,7 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     WindowRun run;
     run.trace_grid = s->trace_grid;
     run.shade_grid = s->shade_grid;
+    run.sky_grid = s->sky_grid;
     run.iters_per_poll = s->iters_per_poll;
     run.total_samples = path;
     run.npix_total = pix;
@@ -320,6 +321,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->cap = cfg->tick_paths + (int64_t)cfg->max_frame_pixels;  // one frame of the largest request may overshoot
     s->trace_grid = c->trace_grid;
     s->shade_grid = c->shade_grid;
+    s->sky_grid = c->sky_grid;
     s->stack_lds = c->tuning.stack_lds;
     s->tail_paths = c->tuning.tail_paths;
     s->iters_per_poll = c->tuning.iters_per_poll;

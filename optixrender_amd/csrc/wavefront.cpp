@@ -67,6 +67,9 @@ void wave_release(WaveEngine& e) {
     e.host_counts = nullptr;
     for (auto& ev : e.poll)
         if (ev) (void)hipEventDestroy(ev), ev = nullptr;
+    if (e.side_start) (void)hipEventDestroy(e.side_start), e.side_start = nullptr;
+    if (e.side_done) (void)hipEventDestroy(e.side_done), e.side_done = nullptr;
+    if (e.side) (void)hipStreamDestroy(e.side), e.side = nullptr;
     for (auto ev : e.stage_events) (void)hipEventDestroy(ev);
     e.stage_events.clear();
     e.pool = e.window = 0;
@@ -86,6 +89,10 @@ hipError_t wave_reserve(WaveEngine& e, int64_t pool, int64_t window, int trace_g
         if ((r = hipHostMalloc((void**)&e.host_counts, 2 * kPollInts * sizeof(int), 0)) != hipSuccess) return r;
         for (auto& ev : e.poll)
             if ((r = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return r;
+        if ((r = hipStreamCreateWithFlags(&e.side, hipStreamNonBlocking)) != hipSuccess ||
+            (r = hipEventCreateWithFlags(&e.side_start, hipEventDisableTiming)) != hipSuccess ||
+            (r = hipEventCreateWithFlags(&e.side_done, hipEventDisableTiming)) != hipSuccess)
+            return r;
     }
     if (pool > e.pool) {
         wave_drop_graphs(e);
@@ -204,10 +211,10 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
     if ((err = wave_window_init(p, r.set_req0, r.frame0, r.path_frame_offset, r.frames, r.total_samples, s)) !=
             hipSuccess ||
         (err = build_cover_masks(p.reqs, p.nreq, 0, p.cover_boxes, p.cover_count, s)) != hipSuccess ||
-        (err = wave_sky(p, r.shade_grid, s)) != hipSuccess ||
+        (err = wave_sky_list(p, r.shade_grid, s)) != hipSuccess ||
         (err = wave_fill(p, r.shade_grid, s)) != hipSuccess || (err = mark(0)) != hipSuccess)
         return err;
-    r.launches += 4;
+    r.launches += 5;
 
     hipGraphExec_t exec = nullptr;
     if (!eager) {
@@ -259,13 +266,23 @@ hipError_t wave_run_window(WaveEngine& e, const WaveParams& p, WindowRun& r, hip
         return n;
     };
     if ((err = replay(0)) != hipSuccess) return err;
+    // The sky shading is read only by the accumulation: it runs on the side
+    // stream from the end of the first replay on, beside the window's later
+    // iterations -- the tail kernel's few waves leave most of the ALUs idle.
+    if ((err = hipEventRecord(e.side_start, s)) != hipSuccess ||
+        (err = hipStreamWaitEvent(e.side, e.side_start, 0)) != hipSuccess ||
+        (err = wave_sky_shade(p, r.sky_grid, e.side)) != hipSuccess ||
+        (err = hipEventRecord(e.side_done, e.side)) != hipSuccess)
+        return err;
     for (int64_t i = 0;; i++) {
         if (i + 1 >= max_replays) return hipErrorLaunchTimeOut;  // cannot happen for a valid window
         if ((err = replay((int)((i + 1) & 1))) != hipSuccess) return err;
         if ((err = hipEventSynchronize(e.poll[i & 1])) != hipSuccess) return err;
         if (live((int)(i & 1)) == 0) break;
     }
-    if ((err = wave_accumulate(p, r.npix_total, s)) != hipSuccess || (err = mark(4)) != hipSuccess) return err;
+    if ((err = hipStreamWaitEvent(s, e.side_done, 0)) != hipSuccess ||
+        (err = wave_accumulate(p, r.npix_total, s)) != hipSuccess || (err = mark(4)) != hipSuccess)
+        return err;
     r.launches += 1;
     return collect();
 }
