@@ -80,12 +80,23 @@ typedef struct oxr_camera {
     float cos_fovy;
 } oxr_camera;
 
+/* RGBA8 material texture (reference: Texture, Scene.cpp:181-195: stbi RGBA
+   load flipped vertically, so row 0 is at v ~ 0).  Sampled as the reference's
+   tex2D<float4> with linear filtering, wrap addressing and normalised-float
+   reads (Renderer.cpp:95-144), in FP32 software bilinear. */
+typedef struct oxr_texture {
+    int32_t width, height;
+    const uint8_t* rgba8;  /* width*height*4 bytes, row-major */
+} oxr_texture;
+
 /* Scene upload (replaces Scene + BuildAccel's per-mesh buffers,
    Renderer.cpp:146-285).  Triangles are de-indexed as TriangleMesh::LoadMesh
    leaves them (Scene.cpp:167-174): 9 floats of positions and 9 of normals per
    triangle, one material slot per triangle.  uvs (6 floats per triangle) may
-   be NULL.  env_rgba: env_w*env_h RGBA32F, row 0 at v ~ 0 (stbi flipped load,
-   Scene.cpp:197-206).  Host pointers are only read during the call. */
+   be NULL: the texture coordinates are then (0,0), (1,0), (0,1) per triangle
+   (Scene.cpp:160-165).  env_rgba: env_w*env_h RGBA32F, row 0 at v ~ 0 (stbi
+   flipped load, Scene.cpp:197-206).  textures: the images a material's tex[]
+   ids index.  Host pointers are only read during the call. */
 typedef struct oxr_scene_desc {
     int64_t n_tris;
     const float* positions;
@@ -100,6 +111,8 @@ typedef struct oxr_scene_desc {
     const oxr_medium* media;
     int32_t env_w, env_h;
     const float* env_rgba;
+    int32_t n_textures;
+    const oxr_texture* textures;
 } oxr_scene_desc;
 
 #define OXR_FLAG_KEEP_PATH_RADIANCE 1u  /* keep per-(frame, pixel) radiance for oxr_download_path_radiance */

@@ -156,6 +156,10 @@ def camera(frm, at, up=(0.0, 1.0, 0.0), medium=-1, cos_fovy=0.66) -> Camera:
     return c
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba8", C.c_void_p)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("n_tris", C.c_int64),
@@ -172,6 +176,8 @@ class SceneDesc(C.Structure):
         ("env_w", C.c_int32),
         ("env_h", C.c_int32),
         ("env_rgba", C.c_void_p),
+        ("n_textures", C.c_int32),
+        ("textures", C.c_void_p),
     ]
 
 

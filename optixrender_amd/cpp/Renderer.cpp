@@ -60,7 +60,7 @@ oxr_material to_abi(const Material& m, int in_med, int out_med) {
 
 Renderer::Renderer(const Scene* s) : scene(s) {
     oxr_ctx* ctx = oxr_cpp::default_context();
-    std::vector<float> pos, nrm;
+    std::vector<float> pos, nrm, uvs;
     std::vector<uint32_t> mid;
     std::vector<oxr_material> mats;
     for (size_t m = 0; m < scene->meshes.size(); m++) {
@@ -70,8 +70,10 @@ Renderer::Renderer(const Scene* s) : scene(s) {
             for (int k : {t.x, t.y, t.z}) {
                 const vec3f& p = mesh->vertex[k];
                 const vec3f& n = mesh->normal[k];
+                const vec2f& t = mesh->texcoord[k];
                 pos.insert(pos.end(), {p.x, p.y, p.z});
                 nrm.insert(nrm.end(), {n.x, n.y, n.z});
+                uvs.insert(uvs.end(), {t.x, t.y});
             }
             mid.push_back((uint32_t)m);
         }
@@ -102,11 +104,22 @@ Renderer::Renderer(const Scene* s) : scene(s) {
         r.scale = m.scale;
         media.push_back(r);
     }
+    std::vector<oxr_texture> textures;  // CreateTextures (Renderer.cpp:95-144)
+    for (const Texture* t : scene->textures) {
+        oxr_texture r;
+        r.width = t->resolution.x;
+        r.height = t->resolution.y;
+        r.rgba8 = t->pixel.data();
+        textures.push_back(r);
+    }
     oxr_scene_desc d;
     std::memset(&d, 0, sizeof(d));
     d.n_tris = (int64_t)mid.size();
     d.positions = pos.data();
     d.normals = nrm.data();
+    d.uvs = uvs.data();
+    d.n_textures = (int32_t)textures.size();
+    d.textures = textures.empty() ? nullptr : textures.data();
     d.mat_ids = mid.data();
     d.n_materials = (int32_t)mats.size();
     d.materials = mats.data();

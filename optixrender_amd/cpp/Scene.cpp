@@ -270,7 +270,8 @@ void Scene::AddTriangles(const std::vector<float>& positions, const std::vector<
     for (size_t i = 0; i < positions.size(); i += 3) {
         mesh->vertex.push_back(vec3f(positions[i], positions[i + 1], positions[i + 2]));
         mesh->normal.push_back(vec3f(normals[i], normals[i + 1], normals[i + 2]));
-        mesh->texcoord.push_back(vec2f(0.0f));
+        const int corner = (int)(i / 3) % 3;  // (0,0), (1,0), (0,1) as LoadMesh leaves untextured faces
+        mesh->texcoord.push_back(vec2f(corner == 1 ? 1.0f : 0.0f, corner == 2 ? 1.0f : 0.0f));
     }
     for (int t = 0; t < (int)positions.size() / 9; t++) mesh->index.push_back(vec3i(3 * t, 3 * t + 1, 3 * t + 2));
     meshes.push_back(mesh);

@@ -6,7 +6,10 @@
 //   headless [settings.json] [out.ppm] [frames] [obj...]
 // Without OBJ arguments the scene is a procedural Cornell box.  With
 // OXR_HEADLESS_DENOISE=1 in the environment the frames are denoised
-// (denoiserOn, the viewer's 'D' key).
+// (denoiserOn, the viewer's 'D' key).  OBJ meshes take their textures from
+// OXR_HEADLESS_ALBEDO / _ROUGHNESS / _METALLIC / _NORMAL (PPM files), and
+// OXR_HEADLESS_ENV names an environment map (.hdr or .pfm), as main.cpp
+// gives its meshes texture files and its scene an HDR (main.cpp:165-213).
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -58,10 +61,19 @@ int main(int ac, char** av) {
         int frames = ac > 3 ? std::atoi(av[3]) : 16;
         Scene scene;
         Camera camera;
+        auto env_str = [](const char* k) { const char* v = std::getenv(k); return std::string(v ? v : ""); };
         if (ac > 4) {
             Material m;
             TextureFile t;
-            for (int i = 4; i < ac; i++) scene.AddMesh(av[i], m, t, -1, -1);
+            t.albedoFile = env_str("OXR_HEADLESS_ALBEDO");
+            t.roughnessFile = env_str("OXR_HEADLESS_ROUGHNESS");
+            t.metallicFile = env_str("OXR_HEADLESS_METALLIC");
+            t.normalFile = env_str("OXR_HEADLESS_NORMAL");
+            for (int i = 4; i < ac; i++) {
+                Material mi = m;  // AddMesh writes the texture ids into its material
+                scene.AddMesh(av[i], mi, t, -1, -1);
+            }
+            if (!env_str("OXR_HEADLESS_ENV").empty()) scene.AddEnv(env_str("OXR_HEADLESS_ENV"));
             camera = Camera{vec3f(0.2f, 0.2f, 0.2f), scene.bounds.center(), vec3f(0.f, 1.f, 0.f)};
         } else {
             cornell(scene);

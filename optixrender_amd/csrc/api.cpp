@@ -154,6 +154,7 @@ void free_scene(SceneDev* s) {
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
     dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
     dfree(s->cover_boxes); dfree(s->cover_count);
+    dfree(s->texels); dfree(s->tex_info); dfree(s->uvs); dfree(s->uvs_sorted);
     delete s;
 }
 
@@ -264,6 +265,10 @@ void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_b
     p.lut_albedo = lut_albedo;
     p.cover_boxes = s->cover_boxes;
     p.cover_count = s->cover_count;
+    p.texels = s->texels;
+    p.tex_info = s->tex_info;
+    p.uvs = s->uvs_sorted;
+    p.textured = s->textured ? 1 : 0;
 }
 
 extern "C" {
@@ -377,11 +382,27 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
     for (int64_t i = 0; i < d->n_tris; i++)
         if (d->mat_ids[i] >= (uint32_t)d->n_materials)
             return fail(c, OXR_ERR_INVALID, "mat_ids[" + std::to_string(i) + "] out of range");
+    if (d->n_textures < 0 || (d->n_textures > 0 && !d->textures))
+        return fail(c, OXR_ERR_INVALID, "n_textures / textures inconsistent");
+    int64_t n_texels = 0;
+    for (int i = 0; i < d->n_textures; i++) {
+        const oxr_texture& t = d->textures[i];
+        if (t.width < 1 || t.height < 1 || !t.rgba8 || (int64_t)t.width * t.height > ((int64_t)1 << 28))
+            return fail(c, OXR_ERR_INVALID, "texture size out of range or pixels missing");
+        n_texels += (int64_t)t.width * t.height;
+    }
+    if (n_texels >= ((int64_t)1 << 31)) return fail(c, OXR_ERR_INVALID, "textures hold more than 2^31 texels");
+    bool textured = false;
     for (int i = 0; i < d->n_materials; i++) {
         const oxr_material& m = d->materials[i];
         if (m.type < 0 || m.type > 7) return fail(c, OXR_ERR_INVALID, "material type out of range");
         if (m.in_medium >= d->n_media || m.out_medium >= d->n_media || m.in_medium < -1 || m.out_medium < -1)
             return fail(c, OXR_ERR_INVALID, "material medium index out of range");
+        for (int k = 0; k < 8; k++) {
+            if (m.tex[k] < -1 || m.tex[k] >= d->n_textures)
+                return fail(c, OXR_ERR_INVALID, "material texture id out of range");
+            textured = textured || m.tex[k] >= 0;
+        }
     }
     for (int i = 0; i < d->n_lights; i++)
         if (d->lights[i].medium >= d->n_media || d->lights[i].medium < -1)
@@ -413,6 +434,35 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
         (e = up(s->lights, d->lights, sizeof(oxr_light) * d->n_lights)) == hipSuccess &&
         (e = up(s->media, d->media, sizeof(oxr_medium) * d->n_media)) == hipSuccess &&
         (e = up(s->env, d->env_rgba, 16 * env_px)) == hipSuccess) {
+    }
+    if (e == hipSuccess && textured) {
+        // the images back to back, and the texture coordinates (defaults
+        // (0,0), (1,0), (0,1) per triangle when none are given)
+        std::vector<int4> info(d->n_textures);
+        int64_t off = 0;
+        for (int i = 0; i < d->n_textures; i++) {
+            info[i] = make_int4((int)off, d->textures[i].width, d->textures[i].height, 0);
+            off += (int64_t)d->textures[i].width * d->textures[i].height;
+        }
+        std::vector<float> uv_default;
+        const float* uvs = d->uvs;
+        if (!uvs) {
+            uv_default.resize(6 * (size_t)n);
+            for (int64_t t = 0; t < n; t++) {
+                float* q = uv_default.data() + 6 * t;
+                q[0] = 0.0f; q[1] = 0.0f; q[2] = 1.0f; q[3] = 0.0f; q[4] = 0.0f; q[5] = 1.0f;
+            }
+            uvs = uv_default.data();
+        }
+        s->textured = true;
+        if ((e = dalloc(&s->texels, (size_t)n_texels)) == hipSuccess &&
+            (e = dalloc(&s->tex_info, (size_t)d->n_textures)) == hipSuccess &&
+            (e = dalloc(&s->uvs, 6 * (size_t)n)) == hipSuccess &&
+            (e = up(s->tex_info, info.data(), sizeof(int4) * d->n_textures)) == hipSuccess &&
+            (e = up(s->uvs, uvs, 24 * (size_t)n)) == hipSuccess) {
+            for (int i = 0; i < d->n_textures && e == hipSuccess; i++)
+                e = up(s->texels + info[i].x, d->textures[i].rgba8, 4 * (size_t)info[i].y * info[i].z);
+        }
     }
     if (e != hipSuccess) {
         free_scene(s);
@@ -450,6 +500,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         OXR_HIP(c, dalloc(&s->mat_sorted, n));
         OXR_HIP(c, dalloc(&s->cover_boxes, 2 * (size_t)kCoverBoxes));
         OXR_HIP(c, dalloc(&s->cover_count, 1));
+        if (s->textured) OXR_HIP(c, dalloc(&s->uvs_sorted, 6 * (size_t)n));
     }
     void* scratch = nullptr;
     size_t sb = bvh_scratch_bytes(n);
@@ -461,8 +512,8 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
     if (n > 0) {
         e = bvh_build(s->positions, n, s->bvh, scratch, c->stream);
         if (e == hipSuccess)
-            e = gather_sorted_attributes(s->normals, s->mat_ids, s->bvh.order, n, s->normals_sorted, s->mat_sorted,
-                                         c->stream);
+            e = gather_sorted_attributes(s->normals, s->mat_ids, s->uvs, s->bvh.order, n, s->normals_sorted,
+                                         s->mat_sorted, s->uvs_sorted, c->stream);
     }
     if (e == hipSuccess) e = build_cover_set(BvhView{s->bvh.pairs, s->bvh.tris, n}, s->cover_boxes, s->cover_count,
                                              c->stream);

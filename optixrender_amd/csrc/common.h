@@ -63,8 +63,10 @@ size_t env_scratch_floats(int w, int h);
 hipError_t tonemap(const float4* d_in, uint32_t* d_out, int64_t n, hipStream_t s);
 hipError_t material_batch(const oxr_material* d_mat, const float* d_lut_bsdf, const float* d_lut_albedo, int mode,
                           const float* d_in, int64_t n, float* d_out, hipStream_t s);
-hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_mat_ids, const uint32_t* d_order,
-                                    int64_t n, float* d_normals_sorted, uint32_t* d_mat_sorted, hipStream_t s);
+// d_uvs / d_uvs_sorted may be null (no textures).
+hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_mat_ids, const float* d_uvs,
+                                    const uint32_t* d_order, int64_t n, float* d_normals_sorted,
+                                    uint32_t* d_mat_sorted, float* d_uvs_sorted, hipStream_t s);
 
 // One render request's share of a wavefront window.  A window holds
 // `frames` frames of every pixel of each request, laid out request after
@@ -155,6 +157,13 @@ struct WaveParams {
     int env_w, env_h;
     const float* lut_bsdf;
     const float* lut_albedo;
+    // material textures: RGBA8 texels of every image, tex_info[id] = (first
+    // texel, width, height); uvs: 6 floats per sorted triangle.  textured = 0:
+    // no material uses a texture (uvs and texels unused)
+    const uint32_t* texels;
+    const int4* tex_info;
+    const float* uvs;
+    int textured;
     int cam_medium;
     int max_depth, rr_start;
     // path pool: `pool` slots, shard k owns slots [k*region, (k+1)*region)

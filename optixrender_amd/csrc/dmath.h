@@ -160,7 +160,9 @@ OXR_D f2 filter_jitter(f2 s, int type) {
 // build's definition of tex2D with cudaFilterModeLinear/cudaAddressModeWrap
 // (Scene.cpp:340-351, 380-391).
 OXR_D int wrap_i(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
-OXR_D float4 bilinear(const float4* __restrict__ img, int w, int h, float u, float v) {
+// texel(i): the float4 value of texel i (row-major).
+template <class Texel>
+OXR_D float4 bilinear_fetch(int w, int h, float u, float v, Texel texel) {
     if (!(u == u)) u = 0.0f;
     if (!(v == v)) v = 0.0f;
     u = u - floorf(u);
@@ -170,8 +172,8 @@ OXR_D float4 bilinear(const float4* __restrict__ img, int w, int h, float u, flo
     float ax = x - fx, ay = y - fy;
     int x0 = wrap_i((int)fx, w), x1 = wrap_i((int)fx + 1, w);
     int y0 = wrap_i((int)fy, h), y1 = wrap_i((int)fy + 1, h);
-    float4 a = img[(size_t)y0 * w + x0], b = img[(size_t)y0 * w + x1];
-    float4 c = img[(size_t)y1 * w + x0], d = img[(size_t)y1 * w + x1];
+    float4 a = texel((size_t)y0 * w + x0), b = texel((size_t)y0 * w + x1);
+    float4 c = texel((size_t)y1 * w + x0), d = texel((size_t)y1 * w + x1);
     float bx = 1.0f - ax, by = 1.0f - ay;
     float4 r;
     r.x = (a.x * bx + b.x * ax) * by + (c.x * bx + d.x * ax) * ay;
@@ -179,6 +181,28 @@ OXR_D float4 bilinear(const float4* __restrict__ img, int w, int h, float u, flo
     r.z = (a.z * bx + b.z * ax) * by + (c.z * bx + d.z * ax) * ay;
     r.w = (a.w * bx + b.w * ax) * by + (c.w * bx + d.w * ax) * ay;
     return r;
+}
+OXR_D float4 bilinear(const float4* __restrict__ img, int w, int h, float u, float v) {
+    return bilinear_fetch(w, h, u, v, [&](size_t i) { return img[i]; });
+}
+// RGBA8 texel read as cudaReadModeNormalizedFloat does: byte / 255.
+OXR_D float4 unorm8x4(uint32_t c) {
+    return make_float4((float)(c & 255u) / 255.0f, (float)((c >> 8) & 255u) / 255.0f,
+                       (float)((c >> 16) & 255u) / 255.0f, (float)(c >> 24) / 255.0f);
+}
+// Material texture: tex2D<float4> with linear filtering, wrap addressing and
+// normalised-float reads of an RGBA8 image (Renderer.cpp:95-144).
+OXR_D float4 bilinear_rgba8(const uint32_t* __restrict__ img, int w, int h, float u, float v) {
+    return bilinear_fetch(w, h, u, v, [&](size_t i) { return unorm8x4(img[i]); });
+}
+
+// NormalFromTangentToWorld (Utils.h:85-94).
+OXR_D f3 normal_from_tangent(f3 n, f3 t) {
+    t = normalize(t * 2.0f - 1.0f);
+    const f3 up = fabsf(n.z) < 0.9f ? f3(0.0f, 0.0f, 1.0f) : f3(1.0f, 0.0f, 0.0f);
+    const f3 tx = normalize(cross(up, n));
+    const f3 ty = normalize(cross(n, tx));
+    return normalize(tx * t.x + ty * t.y + n * t.z);
 }
 
 }  // namespace oxr

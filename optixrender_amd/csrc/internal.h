@@ -31,6 +31,14 @@ struct SceneDev {
     float env_lum_sum = 0.0f;  // sequential luminance sum of the environment (host, at upload)
     BvhBuffers bvh;
     float4* cover_boxes = nullptr;  // cover set of the BVH (2 float4 per box, kCoverBoxes at most)
+    // material textures (only when a material names one): RGBA8 texels of all
+    // images, (first texel, width, height, 0) per image, texture coordinates
+    // (6 floats per triangle) in upload and in sorted order
+    uint32_t* texels = nullptr;
+    int4* tex_info = nullptr;
+    float* uvs = nullptr;
+    float* uvs_sorted = nullptr;
+    bool textured = false;
     int* cover_count = nullptr;
     bool built = false;
     float build_ms = 0.0f;

@@ -155,13 +155,18 @@ __global__ void k_material_batch(const oxr_material* m, Lut lut, int mode, const
 }
 
 __global__ void k_gather(const float* __restrict__ nrm, const uint32_t* __restrict__ mid,
-                         const uint32_t* __restrict__ order, int64_t n, float* nrm_s, uint32_t* mid_s) {
+                         const float* __restrict__ uv, const uint32_t* __restrict__ order, int64_t n, float* nrm_s,
+                         uint32_t* mid_s, float* uv_s) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t t = order[k];
 #pragma unroll
     for (int c = 0; c < 9; c++) nrm_s[9 * k + c] = nrm[9 * (size_t)t + c];
     mid_s[k] = mid[t];
+    if (uv) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) uv_s[6 * k + c] = uv[6 * (size_t)t + c];
+    }
 }
 
 // ------------------------------------------------------------ wavefront
@@ -575,6 +580,40 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
     }
 }
 
+// __closesthit__radiance's texture overrides (devicePrograms.cu:86-115): the
+// texture coordinate interpolated as the reference does, then every texture
+// the material names replaces its parameter; a normal map bends Ns.
+OXR_D void apply_textures(const WaveParams& p, uint32_t leaf, float w, float u, float v, oxr_material& m, f3& Ns) {
+    const float* t = p.uvs + 6 * (size_t)leaf;
+    const float tu = w * t[0] + u * t[2] + v * t[4];
+    const float tv = w * t[1] + u * t[3] + v * t[5];
+    auto tex = [&](int id) {
+        const int4 ti = p.tex_info[id];
+        return bilinear_rgba8(p.texels + ti.x, ti.y, ti.z, tu, tv);
+    };
+    if (m.tex[0] >= 0) {
+        const float4 c = tex(m.tex[0]);
+        m.albedo[0] = c.x;
+        m.albedo[1] = c.y;
+        m.albedo[2] = c.z;
+    }
+    if (m.tex[1] >= 0) m.roughness = tex(m.tex[1]).x;
+    if (m.tex[2] >= 0) m.anisotropy = tex(m.tex[2]).x;
+    if (m.tex[3] >= 0) {
+        const float4 c = tex(m.tex[3]);
+        m.specular[0] = c.x;
+        m.specular[1] = c.y;
+        m.specular[2] = c.z;
+    }
+    if (m.tex[4] >= 0) m.metallic = tex(m.tex[4]).x;
+    if (m.tex[5] >= 0) m.coat_roughness_u = tex(m.tex[5]).x;
+    if (m.tex[6] >= 0) m.coat_roughness_v = tex(m.tex[6]).x;
+    if (m.tex[7] >= 0) {
+        const float4 c = tex(m.tex[7]);
+        Ns = normal_from_tangent(Ns, f3(c.x, c.y, c.z));
+    }
+}
+
 struct ShadowOut {
     bool emit;
     f3 o, d, c;
@@ -645,10 +684,14 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
 
     // __closesthit__radiance / __miss__radiance (devicePrograms.cu:71-159)
     const uint32_t leaf = MT < 0 ? kNoHit : __float_as_uint(hit.y);
+    // the hit's material, with its texture overrides (a copy per hit, as
+    // __closesthit__radiance's `Material material = sbtData.material`)
+    oxr_material mat;
     Surf s;
-    s.m = nullptr;
+    s.m = &mat;
     s.front = true;
     s.n = f3(0.0f);
+    bool surface = false;
     f3 pos = org;
     float distance;
     int mi_in, mi_out;
@@ -657,8 +700,10 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         const float4* tb = p.bvh.tris + 3 * (size_t)leaf;
         float u = hit.z, v = hit.w, w = 1.0f - u - v;
         f3 Ns = normalize(w * f3(nb[0], nb[1], nb[2]) + u * f3(nb[3], nb[4], nb[5]) + v * f3(nb[6], nb[7], nb[8]));
+        mat = p.mats[p.mat_ids[leaf]];
+        surface = true;
+        if (p.textured) apply_textures(p, leaf, w, u, v, mat, Ns);
         f3 P = w * xyz(tb[0]) + u * xyz(tb[1]) + v * xyz(tb[2]);
-        s.m = p.mats + p.mat_ids[leaf];
         s.front = dot(dir, Ns) < 0.0f;                                      // Utils.h:70-73
         s.n = s.front ? Ns : -Ns;
         distance = length(P - org);
@@ -736,7 +781,7 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
     }
 
     if (!scattered) {
-        if (!hitLight && s.m == nullptr && closest == kFltMax && medium_index != -1) {
+        if (!hitLight && !surface && closest == kFltMax && medium_index != -1) {
             // a ray inside a medium that left the scene unscattered: the
             // reference would shade stale surface data with zero throughput
             alive = false;
@@ -1292,10 +1337,10 @@ hipError_t material_batch(const oxr_material* m, const float* lb, const float* l
     return hipGetLastError();
 }
 
-hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const uint32_t* order, int64_t n,
-                                    float* nrm_s, uint32_t* mid_s, hipStream_t s) {
+hipError_t gather_sorted_attributes(const float* nrm, const uint32_t* mid, const float* uv, const uint32_t* order,
+                                    int64_t n, float* nrm_s, uint32_t* mid_s, float* uv_s, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    k_gather<<<blocks(n, 256), 256, 0, s>>>(nrm, mid, order, n, nrm_s, mid_s);
+    k_gather<<<blocks(n, 256), 256, 0, s>>>(nrm, mid, uv_s ? uv : nullptr, order, n, nrm_s, mid_s, uv_s);
     return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ class SceneData:
     media: list = field(default_factory=list)
     env: np.ndarray | None = None    # (h, w, 4) float32, row 0 at v ~ 0
     camera: abi.Camera | None = None
+    uvs: np.ndarray | None = None    # (n, 3, 2) float32; None: (0,0), (1,0), (0,1) per triangle
+    textures: list = field(default_factory=list)  # (h, w, 4) uint8 images, row 0 at v ~ 0
 
     @property
     def n_tris(self) -> int:
@@ -48,6 +50,18 @@ class SceneData:
         media = (abi.Medium * max(len(self.media), 1))(*self.media)
         env = None if self.env is None else np.ascontiguousarray(self.env, dtype=np.float32)
         return pos, nrm, mid, mats, lights, media, env
+
+    def texture_arrays(self):
+        """(uvs or None, a Texture array or None, arrays to keep alive)."""
+        uvs = None if self.uvs is None else np.ascontiguousarray(self.uvs, dtype=np.float32).reshape(-1)
+        if not self.textures:
+            return uvs, None, (uvs,)
+        imgs = [np.ascontiguousarray(t, dtype=np.uint8) for t in self.textures]
+        for im in imgs:
+            if im.ndim != 3 or im.shape[2] != 4:
+                raise ValueError("a texture is an (h, w, 4) uint8 array")
+        tex = (abi.Texture * len(imgs))(*[abi.Texture(im.shape[1], im.shape[0], abi.ptr(im)) for im in imgs])
+        return uvs, tex, (uvs, imgs, tex)
 
 
 class Context:
@@ -80,11 +94,15 @@ class Context:
     # -- scenes
     def upload(self, scene: SceneData) -> int:
         pos, nrm, mid, mats, lights, media, env = scene.arrays()
+        uvs, tex, _keep = scene.texture_arrays()
         d = abi.SceneDesc()
         d.n_tris = scene.n_tris
         d.positions = abi.ptr(pos)
         d.normals = abi.ptr(nrm)
-        d.uvs = None
+        d.uvs = None if uvs is None else abi.ptr(uvs)
+        if tex is not None:
+            d.n_textures = len(tex)
+            d.textures = C.cast(tex, C.c_void_p)
         d.mat_ids = abi.ptr(mid)
         d.n_materials = len(scene.materials)
         d.materials = C.cast(mats, C.c_void_p)

@@ -18,6 +18,12 @@ PCG64 seeded per scene.
       (sigma_t 0.5, albedo 0.9, HG g 0.3) around a dielectric sphere.
   sphere_field_5m: 61 icospheres at subdivision 6 (4,997,120 triangles), a
       resident scene of the serving benchmark.
+  lit_sphere_field / cornell_two_lights: sphere and quad lights in view of
+      the camera, alone or beside the sky.
+  textured_cornell: the Cornell box with every material texture slot in use
+      (albedo, roughness, anisotropy, specular, metallic, both coat
+      roughnesses, normal maps) on RGBA8 images of odd sizes, texture
+      coordinates beyond [0, 1] on the floor (wrap addressing).
 """
 from __future__ import annotations
 
@@ -111,6 +117,112 @@ def cornell_spheres(subdiv: int = 3) -> SceneData:
     return SceneData(pos, nrm, mid, mats, lights, [], None, cam)
 
 
+def quad_uvs(scale: float = 1.0) -> np.ndarray:
+    """Texture coordinates of quad()'s two triangles: p0 (0,0), p1 (s,0),
+    p2 (s,s), p3 (0,s)."""
+    c = np.array([[0, 0], [scale, 0], [scale, scale], [0, scale]], dtype=np.float64)
+    return np.stack([c[[0, 1, 2]], c[[0, 2, 3]]])
+
+
+def default_uvs(n: int) -> np.ndarray:
+    """(0,0), (1,0), (0,1) per triangle, as LoadMesh gives an untextured face
+    (Scene.cpp:160-165)."""
+    return np.broadcast_to(np.array([[0, 0], [1, 0], [0, 1]], dtype=np.float64), (n, 3, 2)).copy()
+
+
+def sphere_uvs(unit: np.ndarray) -> np.ndarray:
+    """Latitude-longitude coordinates of points on the unit sphere."""
+    u = np.arctan2(unit[..., 2], unit[..., 0]) / (2.0 * math.pi) + 0.5
+    v = np.arccos(np.clip(unit[..., 1], -1.0, 1.0)) / math.pi
+    return np.stack([u, v], axis=-1)
+
+
+def rgba8(rgb: np.ndarray) -> np.ndarray:
+    """(h, w, 3) values in [0, 1] -> (h, w, 4) uint8 with alpha 255."""
+    h, w = rgb.shape[:2]
+    out = np.full((h, w, 4), 255, np.uint8)
+    out[..., :3] = np.clip(np.rint(rgb * 255.0), 0, 255).astype(np.uint8)
+    return out
+
+
+def normal_map(height: np.ndarray, strength: float) -> np.ndarray:
+    """Tangent-space normal map of a periodic height image, encoded n*0.5+0.5."""
+    dx = (np.roll(height, -1, axis=1) - np.roll(height, 1, axis=1)) * 0.5
+    dy = (np.roll(height, -1, axis=0) - np.roll(height, 1, axis=0)) * 0.5
+    n = np.stack([-strength * dx, -strength * dy, np.ones_like(height)], axis=-1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return rgba8(n * 0.5 + 0.5)
+
+
+def textured_cornell(subdiv: int = 3, seed: int = 3, with_uvs: bool = True) -> SceneData:
+    """The Cornell box with textured materials (Scene::AddMesh's eight texture
+    slots, Scene.cpp:22-57; lookups at devicePrograms.cu:86-115):
+
+      floor       plastic: albedo checker, specular and roughness images,
+                  coordinates 0..3 (each image repeats 3x3 times)
+      back wall   diffuse grey with a bumpy normal map
+      right wall  clear-coated conductor: coat roughness u and v images
+      left sphere conductor: roughness and anisotropy images
+      right sphere metal workflow: albedo, metallic and normal images
+
+    with_uvs=False leaves the coordinates to the default (0,0), (1,0), (0,1)
+    per triangle."""
+    rng = np.random.default_rng(seed)
+    sc = cornell_spheres(subdiv)
+    n = sc.n_tris
+    uvs = default_uvs(n)
+    uvs[0:2] = quad_uvs(3.0)     # floor
+    uvs[4:6] = quad_uvs(1.0)     # back wall
+    uvs[10:12] = quad_uvs(1.0)   # right wall
+    ico = icosphere(subdiv)
+    m = len(ico)
+    unit = ico.copy()
+    unit /= np.linalg.norm(unit, axis=-1, keepdims=True)
+    uvs[12:12 + m] = sphere_uvs(unit)
+    uvs[12 + m:12 + 2 * m] = sphere_uvs(unit)
+    # images (odd sizes: the wrap is not a power-of-two mask)
+    yy, xx = np.mgrid[0:40, 0:48]
+    checker = ((xx // 6 + yy // 5) % 2).astype(np.float64)
+    albedo = np.stack([0.15 + 0.7 * checker, 0.2 + 0.5 * checker, 0.6 - 0.4 * checker], axis=-1)
+    t_albedo = rgba8(albedo)
+    t_rough = rgba8(np.repeat(rng.uniform(0.1, 0.8, (29, 37, 1)), 3, axis=-1))
+    t_spec = rgba8(rng.uniform(0.2, 1.0, (17, 23, 3)))
+    hy, hx = np.mgrid[0:32, 0:32] * (2.0 * math.pi / 32.0)
+    t_bump = normal_map(np.sin(3 * hx) * np.cos(2 * hy), 1.5)
+    t_coat_u = rgba8(np.repeat(rng.uniform(0.05, 0.6, (13, 11, 1)), 3, axis=-1))
+    t_coat_v = rgba8(np.repeat(rng.uniform(0.05, 0.6, (11, 13, 1)), 3, axis=-1))
+    t_aniso = rgba8(np.repeat(rng.uniform(0.0, 0.8, (21, 19, 1)), 3, axis=-1))
+    t_metal = rgba8(np.repeat(rng.uniform(0.0, 1.0, (9, 15, 1)), 3, axis=-1))
+    t_sph_alb = rgba8(rng.uniform(0.3, 0.95, (24, 48, 3)))
+    sy, sx = np.mgrid[0:24, 0:48] * (2.0 * math.pi / np.array([24.0, 48.0]))[:, None, None]
+    t_sph_bump = normal_map(np.sin(5 * sx) * np.sin(2 * sy), 1.0)
+    textures = [t_albedo, t_rough, t_spec, t_bump, t_coat_u, t_coat_v, t_aniso, t_metal, t_sph_alb, t_sph_bump]
+
+    def tex(m, **slots):
+        order = ("albedo", "roughness", "anisotropy", "specular", "metallic", "coat_u", "coat_v", "normal")
+        for k, v in slots.items():
+            m.tex[order.index(k)] = v
+        return m
+
+    mats = list(sc.materials)
+    mats.append(tex(abi.material(abi.PLASTIC, roughness=0.3), albedo=0, roughness=1, specular=2))       # 5
+    mats.append(tex(abi.material(abi.DIFFUSE, albedo=(0.7, 0.7, 0.7), roughness=0.0), normal=3))        # 6
+    mats.append(tex(abi.material(abi.CLEARCOAT_CONDUCTOR, roughness=0.3), coat_u=4, coat_v=5))          # 7
+    mats.append(tex(abi.material(abi.CONDUCTOR, roughness=0.3, anisotropy=0.5), roughness=1, anisotropy=6))  # 8
+    mats.append(tex(abi.material(abi.METAL_WORKFLOW, roughness=0.25), albedo=8, metallic=7, normal=9))  # 9
+    mid = sc.mat_ids.copy()
+    mid[0:2] = 5
+    mid[4:6] = 6
+    mid[10:12] = 7
+    mid[12:12 + m] = 8
+    mid[12 + m:12 + 2 * m] = 9
+    sc.mat_ids = mid
+    sc.materials = mats
+    sc.textures = textures
+    sc.uvs = uvs.astype(np.float32) if with_uvs else None
+    return sc
+
+
 def foggy_cornell(phase_type: int = 1, g: float = 0.3) -> SceneData:
     """The Cornell scene filled with a homogeneous medium (camera, walls and
     light all inside medium 0): exercises free-flight sampling, medium NEE
@@ -180,6 +292,32 @@ def sphere_field(seed: int = 1, subdiv: int = 6, count: int = 12, env_w: int = 2
     env = sky_env(env_w, env_h)
     cam = abi.camera((0.0, 0.0, 30.0), (0.0, 0.0, 0.0))
     return SceneData(pos, nrm, mid, mats, [], [], env, cam)
+
+
+def lit_sphere_field(seed: int = 1, subdiv: int = 2, env_w: int = 256, env_h: int = 128,
+                     with_env: bool = True) -> SceneData:
+    """The sphere field with two analytic lights in the camera's view: a
+    sphere light (radius 2, in the sky beside the field) and a one-sided 10 x
+    10 quad light at height 10 facing down (Light.h:5-141); with_env=False removes the
+    sky, so a camera ray that misses the geometry meets a light or nothing."""
+    sc = sphere_field(seed=seed, subdiv=subdiv, env_w=env_w, env_h=env_h)
+    sc.lights = [
+        abi.light(1, radiance=(6.0, 5.0, 3.0), position=(14.0, 7.0, -5.0), radius=2.0),
+        abi.light(0, double_side=0, radiance=(2.0, 2.5, 3.0), position=(-15.0, 10.0, 0.0), u=(10.0, 0.0, 0.0),
+                  v=(0.0, 0.0, -10.0)),
+    ]
+    if not with_env:
+        sc.env = None
+    return sc
+
+
+def cornell_two_lights(subdiv: int = 3) -> SceneData:
+    """The Cornell scene with a small sphere light (radius 0.12) beside the
+    ceiling quad light: light selection over two lights, cone sampling, and
+    the reference's light_pdf overwrite in lightTrace."""
+    sc = cornell_spheres(subdiv)
+    sc.lights = list(sc.lights) + [abi.light(1, radiance=(8.0, 6.0, 4.0), position=(0.5, 1.5, 1.0), radius=0.12)]
+    return sc
 
 
 def fbm(n: int, octaves: int = 6, seed: int = 1, base: int = 4) -> np.ndarray:

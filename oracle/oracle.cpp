@@ -33,8 +33,14 @@
  *                           integrals, sample/eval consistency.
  *   env cache               exhaustive lower_bound check on small maps.
  *   tone map                closed form on a value grid.
- *   whole image             converges to analytic white-furnace / Lambert
- *                           values on closed scenes.
+ *   textures                texel centres, channel and row order, wrap,
+ *                           a float64 bilinear; NormalFromTangentToWorld
+ *                           frames and rotation invariants; zero-valued
+ *                           textures equal zeroed parameters.
+ *   direct lighting         whole renders of a Lambertian floor under one
+ *                           sphere or quad light equal the closed-form
+ *                           irradiance (NEE + MIS-weighted BSDF hits); two
+ *                           lights show the reference's light_pdf quirk.
  *   parity unpinned: the Kulla-Conty table values and the exact
  *   AverageFresnelConductor/AverageFresnelDielectric fits (only bounds and
  *   monotonicity are checked), the anisotropic SampleGGX branch, and the
@@ -214,7 +220,9 @@ inline V2 filter_jitter(V2 s, int type) {                                       
 // FP32 software bilinear with wrap on normalised coordinates (stands for
 // tex2D with cudaFilterModeLinear + cudaAddressModeWrap, Scene.cpp:340-351).
 inline int wrap_index(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
-inline void bilinear(const float* img, int w, int h, float u, float v, float* out) {
+// texel(i, c): channel c of texel i (row-major)
+template <typename Texel>
+inline void bilinear_fetch(int w, int h, float u, float v, Texel texel, float* out) {
     if (!(u == u)) u = 0.0f;
     if (!(v == v)) v = 0.0f;
     u = u - floorf(u);
@@ -225,15 +233,21 @@ inline void bilinear(const float* img, int w, int h, float u, float v, float* ou
     float ax = x - fx, ay = y - fy;
     int x0 = wrap_index((int)fx, w), x1 = wrap_index((int)fx + 1, w);
     int y0 = wrap_index((int)fy, h), y1 = wrap_index((int)fy + 1, h);
-    const float* t00 = img + 4 * ((size_t)y0 * w + x0);
-    const float* t10 = img + 4 * ((size_t)y0 * w + x1);
-    const float* t01 = img + 4 * ((size_t)y1 * w + x0);
-    const float* t11 = img + 4 * ((size_t)y1 * w + x1);
+    const size_t i00 = (size_t)y0 * w + x0, i10 = (size_t)y0 * w + x1;
+    const size_t i01 = (size_t)y1 * w + x0, i11 = (size_t)y1 * w + x1;
     for (int c = 0; c < 4; c++) {
-        float top = t00[c] * (1.0f - ax) + t10[c] * ax;
-        float bot = t01[c] * (1.0f - ax) + t11[c] * ax;
+        float top = texel(i00, c) * (1.0f - ax) + texel(i10, c) * ax;
+        float bot = texel(i01, c) * (1.0f - ax) + texel(i11, c) * ax;
         out[c] = top * (1.0f - ay) + bot * ay;
     }
+}
+inline void bilinear(const float* img, int w, int h, float u, float v, float* out) {
+    bilinear_fetch(w, h, u, v, [&](size_t i, int c) { return img[4 * i + c]; }, out);
+}
+// Material textures: RGBA8 with cudaReadModeNormalizedFloat (byte / 255),
+// linear filter, wrap, normalised coordinates (Renderer.cpp:95-144).
+inline void bilinear_rgba8(const uint8_t* img, int w, int h, float u, float v, float* out) {
+    bilinear_fetch(w, h, u, v, [&](size_t i, int c) { return (float)img[4 * i + c] / 255.0f; }, out);
 }
 
 // ------------------------------------------------------------- Microfacet.h
@@ -426,6 +440,7 @@ struct Mat {
     V3 specular;
     float metallic, coat_weight, coat_ru, coat_rv;
     int in_medium, out_medium;
+    int tex[8];  // albedo roughness anisotropy specular metallic coat_u coat_v normal
 };
 Mat to_mat(const orc_material& m) {
     Mat r;
@@ -445,6 +460,7 @@ Mat to_mat(const orc_material& m) {
     r.coat_rv = m.coat_roughness_v;
     r.in_medium = m.in_medium;
     r.out_medium = m.out_medium;
+    for (int i = 0; i < 8; i++) r.tex[i] = m.tex[i];
     return r;
 }
 struct Isect {                                                                   // Utils.h:62-74
@@ -1447,7 +1463,47 @@ struct Scene {
     int env_w = 0, env_h = 0;
     const float* env = nullptr;
     std::vector<float> env_cache;
+    const float* uvs = nullptr;
+    std::vector<orc_texture> textures;
+    bool textured = false;
 };
+
+// NormalFromTangentToWorld (Utils.h:85-94).
+V3 normal_from_tangent(V3 n, V3 t) {
+    t = normalize(t * 2.0f - 1.0f);
+    V3 up = fabsf(n.z) < 0.9f ? V3(0.0f, 0.0f, 1.0f) : V3(1.0f, 0.0f, 0.0f);
+    V3 tx = normalize(cross(up, n));
+    V3 ty = normalize(cross(n, tx));
+    return normalize(tx * t.x + ty * t.y + n * t.z);
+}
+
+// Texture overrides of __closesthit__radiance (devicePrograms.cu:86-115):
+// interpolated texture coordinate, then each texture the material names
+// replaces its parameter; the normal map bends Ns.
+void apply_textures(const Scene& S, uint32_t pid, float w, float u, float v, Mat& m, V3& Ns) {
+    float tu, tv;
+    if (S.uvs) {
+        const float* t = S.uvs + 6 * (size_t)pid;
+        tu = w * t[0] + u * t[2] + v * t[4];
+        tv = w * t[1] + u * t[3] + v * t[5];
+    } else {  // Scene.cpp:160-165: (0,0), (1,0), (0,1)
+        tu = w * 0.0f + u * 1.0f + v * 0.0f;
+        tv = w * 0.0f + u * 0.0f + v * 1.0f;
+    }
+    float c[4];
+    auto tex = [&](int id) {
+        const orc_texture& t = S.textures[id];
+        bilinear_rgba8(t.rgba8, t.width, t.height, tu, tv, c);
+    };
+    if (m.tex[0] >= 0) { tex(m.tex[0]); m.albedo = V3(c[0], c[1], c[2]); }
+    if (m.tex[1] >= 0) { tex(m.tex[1]); m.roughness = c[0]; }
+    if (m.tex[2] >= 0) { tex(m.tex[2]); m.anisotropy = c[0]; }
+    if (m.tex[3] >= 0) { tex(m.tex[3]); m.specular = V3(c[0], c[1], c[2]); }
+    if (m.tex[4] >= 0) { tex(m.tex[4]); m.metallic = c[0]; }
+    if (m.tex[5] >= 0) { tex(m.tex[5]); m.coat_ru = c[0]; }
+    if (m.tex[6] >= 0) { tex(m.tex[6]); m.coat_rv = c[0]; }
+    if (m.tex[7] >= 0) { tex(m.tex[7]); Ns = normal_from_tangent(Ns, V3(c[0], c[1], c[2])); }
+}
 struct Cam {                                                                     // Renderer.cpp:796-813
     V3 position, direction, horizontal, vertical;
     int medium;
@@ -1525,6 +1581,7 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
     is.mat = nullptr;
     is.in_medium = is.out_medium = cam.medium;
     bool first = true;
+    Mat hit_mat;  // the hit's material after its textures (is.mat points here then)
     V3 pix_albedo(0.0f), pix_normal(0.0f);
     float pix_depth = 1.0f;
     const bool has_env = S.env_w > 0;
@@ -1540,6 +1597,11 @@ PathOut trace_path(const Scene& S, const Cam& cam, const orc_render_params& P, i
             V3 Ns = normalize(w * load3(n) + h.u * load3(n + 3) + h.v * load3(n + 6));
             V3 P0 = w * load3(p) + h.u * load3(p + 3) + h.v * load3(p + 6);
             const Mat* m = &S.mats[S.mat_ids[pid]];
+            if (S.textured) {
+                hit_mat = *m;
+                apply_textures(S, pid, w, h.u, h.v, hit_mat, Ns);
+                m = &hit_mat;
+            }
             is.set_face_normal(ray.d, Ns);
             is.distance = length(P0 - is.position);
             is.position = P0;
@@ -1721,6 +1783,10 @@ void load_scene(Scene& S, const orc_scene* sc) {
     S.env_w = sc->env_w;
     S.env_h = sc->env_h;
     S.env = sc->env_rgba;
+    S.uvs = sc->uvs;
+    for (int i = 0; i < sc->n_textures; i++) S.textures.push_back(sc->textures[i]);
+    for (const Mat& m : S.mats)
+        for (int i = 0; i < 8; i++) S.textured = S.textured || m.tex[i] >= 0;
     if (S.env_w > 0 && S.env_h > 0) {
         S.env_cache.resize((size_t)4 * S.env_w * S.env_h);
         env_cache(S.env_w, S.env_h, S.env, S.env_cache.data());
@@ -1889,6 +1955,13 @@ void orc_filter_jitter(int type, float u0, float u1, float* out2) {
 }
 void orc_env_cache(int w, int h, const float* rgba, float* cache) { env_cache(w, h, rgba, cache); }
 void orc_bilinear(int w, int h, const float* rgba, float u, float v, float* out4) { bilinear(rgba, w, h, u, v, out4); }
+void orc_texture_sample(int w, int h, const uint8_t* rgba8, float u, float v, float* out4) {
+    bilinear_rgba8(rgba8, w, h, u, v, out4);
+}
+void orc_normal_from_tangent(const float* n, const float* t, float* out3) {
+    V3 r = normal_from_tangent(load3(n), load3(t));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
 
 // PostProcessing.cu:3-48
 void orc_tonemap(int64_t n, const float* rgba, uint32_t* out) {

@@ -77,6 +77,11 @@ typedef struct orc_camera {
     float cos_fovy;
 } orc_camera;
 
+typedef struct orc_texture {
+    int32_t width, height;
+    const uint8_t* rgba8;       /* width*height*4 bytes, row 0 = v~0 */
+} orc_texture;
+
 typedef struct orc_scene {
     int64_t n_tris;
     const float* positions;     /* 9 floats per triangle (de-indexed) */
@@ -90,6 +95,9 @@ typedef struct orc_scene {
     const orc_medium* media;
     int32_t env_w, env_h;       /* 0 = no environment */
     const float* env_rgba;      /* env_w*env_h*4 floats, row 0 = v~0 */
+    const float* uvs;           /* 6 floats per triangle, or NULL: (0,0) (1,0) (0,1) */
+    int32_t n_textures;
+    const orc_texture* textures;
 } orc_scene;
 
 typedef struct orc_render_params {
@@ -145,6 +153,10 @@ void orc_phase_sample(const orc_medium* m, const float* V, uint32_t* rng_state,
 void orc_filter_jitter(int type, float u0, float u1, float* out2);
 void orc_env_cache(int w, int h, const float* rgba, float* cache_rgba);
 void orc_bilinear(int w, int h, const float* rgba, float u, float v, float* out4);
+/* material texture lookup: RGBA8 texels read as byte/255, then as orc_bilinear */
+void orc_texture_sample(int w, int h, const uint8_t* rgba8, float u, float v, float* out4);
+/* NormalFromTangentToWorld: n (unit), t = texel RGB in [0,1] */
+void orc_normal_from_tangent(const float* n, const float* t, float* out3);
 void orc_tonemap(int64_t n, const float* rgba, uint32_t* out);
 
 /* ---- whole render ---- */

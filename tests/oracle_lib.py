@@ -26,6 +26,7 @@ class OrcScene(C.Structure):
         ("n_tris", C.c_int64), ("positions", _P), ("normals", _P), ("mat_ids", _P),
         ("n_materials", C.c_int32), ("materials", _P), ("n_lights", C.c_int32), ("lights", _P),
         ("n_media", C.c_int32), ("media", _P), ("env_w", C.c_int32), ("env_h", C.c_int32), ("env_rgba", _P),
+        ("uvs", _P), ("n_textures", C.c_int32), ("textures", _P),
     ]
 
 
@@ -68,6 +69,8 @@ def lib():
         L.orc_filter_jitter.argtypes = [C.c_int, C.c_float, C.c_float, _P]
         L.orc_env_cache.argtypes = [C.c_int, C.c_int, _P, _P]
         L.orc_bilinear.argtypes = [C.c_int, C.c_int, _P, C.c_float, C.c_float, _P]
+        L.orc_texture_sample.argtypes = [C.c_int, C.c_int, _P, C.c_float, C.c_float, _P]
+        L.orc_normal_from_tangent.argtypes = [_P, _P, _P]
         L.orc_tonemap.argtypes = [C.c_int64, _P, _P]
         L.orc_render.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_render.restype = C.c_int
@@ -226,6 +229,20 @@ def bilinear(env, u, v):
     return out
 
 
+def texture_sample(img, u, v):
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    out = np.zeros(4, np.float32)
+    lib().orc_texture_sample(w, h, p(img), u, v, p(out))
+    return out
+
+
+def normal_from_tangent(n, t):
+    out = np.zeros(3, np.float32)
+    lib().orc_normal_from_tangent(p(np.asarray(n, np.float32)), p(np.asarray(t, np.float32)), p(out))
+    return out
+
+
 def tonemap(rgba):
     rgba = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
     out = np.zeros(rgba.shape[0], np.uint32)
@@ -244,7 +261,11 @@ def _scene_struct(scene: SceneData):
     if env is not None:
         s.env_h, s.env_w = env.shape[0], env.shape[1]
         s.env_rgba = p(env)
-    keep = (pos, nrm, mid, mats, lights, media, env)
+    uvs, tex, tkeep = scene.texture_arrays()
+    s.uvs = None if uvs is None else p(uvs)
+    if tex is not None:
+        s.n_textures, s.textures = len(tex), C.cast(tex, _P)
+    keep = (pos, nrm, mid, mats, lights, media, env, tkeep)
     return s, keep
 
 

@@ -83,6 +83,23 @@ def test_sphere_field_cameras(ctx, field, where):
         assert st.camera_rays_resolved == 96 * 64 * 3 and st.extension_rays == 0
 
 
+@pytest.mark.parametrize("with_env", [True, False])
+def test_lit_sphere_field_cull_is_exact(ctx, with_env):
+    # analytic lights in the sky: ruled-out camera rays that meet a sphere or
+    # quad light are shaded by the sky pass exactly as the wavefront would
+    sc = scenes.lit_sphere_field(with_env=with_env)
+    sid = ctx.upload(sc)
+    try:
+        ctx.build_bvh(sid)
+        print(f"[cull] lit sphere field env={with_env}", flush=True)
+        (a, st), (b, _) = render_both(ctx, sid, sc.camera, 160, 90, spp=3, max_depth=6, rr_start=0)
+        assert_same(a, b)
+        assert st.camera_rays_resolved > 0
+        assert a[0][..., :3].max() > 0   # the lights are in view
+    finally:
+        ctx.release(sid)
+
+
 def test_cornell_nothing_ruled_out(ctx, cornell):
     sc, sid = cornell
     print("[cull] cornell", flush=True)

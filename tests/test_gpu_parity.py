@@ -210,10 +210,12 @@ def per_path_check(gpu, ref, label):
     return frac
 
 
-def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None):
+def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None, tail=None, normals=None, filt=abi.FILTER_GAUSSIAN):
     """GPU render and oracle render of the same scene.  pool: path slots of
     the GPU pool (None = default); a pool smaller than spp*w*h restarts
-    every slot many times."""
+    every slot many times.  tail: the tail-mode threshold (None = default;
+    0 keeps every path in the per-material kernels).  normals: a list that
+    receives the (GPU, oracle) normal AOVs.  filt: the pixel filter."""
     sid = ctx.upload(scene)
     base = ctx.get_tuning()
     try:
@@ -221,23 +223,29 @@ def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None):
         ctx.select(sid)
         ctx.resize(w, h)
         ctx.set_camera(scene.camera)
-        if pool is not None:
+        if pool is not None or tail is not None:
             t = ctx.get_tuning()
-            t.pool_paths = pool
+            if pool is not None:
+                t.pool_paths = pool
+            if tail is not None:
+                t.tail_paths = tail
             ctx.set_tuning(t)
-        ctx.render(spp=spp, max_depth=depth, rr_start=rr, flags=abi.OXR_FLAG_KEEP_PATH_RADIANCE)
+        ctx.render(spp=spp, max_depth=depth, rr_start=rr, filter=filt, flags=abi.OXR_FLAG_KEEP_PATH_RADIANCE)
         paths = ctx.download_path_radiance(spp, w, h)
         color = ctx.download_aov(0, w, h)
         albedo = ctx.download_aov(1, w, h)
+        normal = ctx.download_aov(2, w, h)
         dz = ctx.download_depth(w, h)
     finally:
         ctx.set_tuning(base)
         ctx.release(sid)
-    rc, ra, rn, rd, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, keep_paths=True,
-                                           with_depth=True)
+    rc, ra, rn, rd, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, filt=filt,
+                                           keep_paths=True, with_depth=True)
     # first-hit depth AOV: same traversal, same distance arithmetic
     close = np.abs(dz - rd) <= 1e-6 * np.abs(rd) + 1e-7
     assert close.mean() >= PASS_FRACTION, f"depth AOV differs on {np.count_nonzero(~close)} pixels"
+    if normals is not None:
+        normals.extend([normal, rn])
     return paths, rp, color, rc, albedo, ra
 
 
@@ -269,9 +277,86 @@ def test_render_heightfield_medium_per_path(ctx, pool):
     assert per_path_check(paths, rp, "heightfield_medium") >= PASS_FRACTION
 
 
+@pytest.mark.parametrize("with_env", [True, False])
+def test_render_lit_sphere_field_per_path(ctx, with_env):
+    # a sphere light and a one-sided quad light in the camera's view, with
+    # and without the sky: camera rays that miss the spheres meet the lights
+    # in the window's sky pass, the rest through the wavefront
+    sc = scenes.lit_sphere_field(with_env=with_env)
+    print(f"[render] lit sphere field env={with_env} 96x54x2", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 96, 54, 2, 16, 0)
+    assert per_path_check(paths, rp, "lit_sphere_field") >= PASS_FRACTION
+
+
+@pytest.mark.parametrize("tail", [None, 0])
+def test_render_cornell_two_lights_per_path(ctx, tail):
+    sc = scenes.cornell_two_lights()
+    print(f"[render] cornell two lights 64x64x4 tail={tail}", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3, tail=tail)
+    assert per_path_check(paths, rp, "cornell_two_lights") >= PASS_FRACTION
+
+
+def test_render_fog_sphere_light_per_path(ctx):
+    # a sphere light inside the medium: medium next-event estimation towards
+    # a cone-sampled light, and free flights that meet it
+    sc = scenes.foggy_cornell(phase_type=1, g=0.3)
+    sc.lights = list(sc.lights) + [abi.light(1, radiance=(8.0, 6.0, 4.0), position=(0.5, 1.5, 1.0), radius=0.12,
+                                             medium=0)]
+    print("[render] fog with a sphere light 48x48x3", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 48, 48, 3, 8, 2)
+    assert per_path_check(paths, rp, "fog_sphere_light") >= PASS_FRACTION
+
+
 @pytest.mark.parametrize("phase,g", [(0, 0.0), (1, 0.3), (1, -0.6)])
 def test_render_fog_per_path(ctx, phase, g):
     sc = scenes.foggy_cornell(phase_type=phase, g=g)
     print(f"[render] fog phase={phase} g={g} 48x48x3", flush=True)
     paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 48, 48, 3, 8, 2)
     assert per_path_check(paths, rp, f"fog{phase}") >= PASS_FRACTION
+
+
+@pytest.mark.parametrize("uvs,tail", [(True, None), (True, 0), (False, None)])
+def test_render_textured_cornell_per_path(ctx, uvs, tail):
+    # every texture slot in use (albedo, roughness, anisotropy, specular,
+    # metallic, both coat roughnesses, normal maps), odd image sizes, texture
+    # coordinates beyond [0, 1]; tail=0 keeps every path in the per-material
+    # kernels, the default carries these few paths in the tail kernel.
+    # The box filter draws the camera jitter without transcendentals, so the
+    # camera rays, their hits and hence the first-hit texture lookups and
+    # normal maps are bit-identical: the albedo and normal AOVs must match bit
+    # for bit.  Later bounces hit points an ulp apart (device vs host
+    # transcendentals in BSDF sampling), and a texture turns that into a
+    # parameter change of (texture gradient) x (shift): the per-path
+    # tolerance is 1e-3 here.  Without coordinates every small sphere
+    # triangle spans half of each image, the gradient per unit length is ~50x
+    # larger and a few paths take another branch (a lobe choice or Russian
+    # roulette draw near its threshold): 1e-2 on 99.5% of paths there.  The
+    # oracle itself, with a 1-ulp error planted in a quarter of its
+    # transcendental results, differs from the unplanted oracle on this scene
+    # by as much (full: 7 paths beyond 1e-3; no coordinates: 41 beyond 1e-2).
+    sc = scenes.textured_cornell(with_uvs=uvs)
+    print(f"[render] textured cornell uvs={uvs} tail={tail} 64x64x4 box filter", flush=True)
+    normals = []
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3, tail=tail, normals=normals,
+                                                    filt=abi.FILTER_BOX)
+    assert np.array_equal(albedo.view(np.uint32), ra.view(np.uint32))
+    assert np.array_equal(normals[0].view(np.uint32), normals[1].view(np.uint32))
+    tol, frac, img = (1e-3, PASS_FRACTION, 1e-4) if uvs else (1e-2, 0.995, 2e-3)
+    a = paths.reshape(-1, 3).astype(np.float64)
+    b = rp.reshape(-1, 3).astype(np.float64)
+    ok = np.all(np.abs(a - b) <= tol * np.maximum(np.abs(a), np.abs(b)) + 1e-7, axis=1)
+    dimg = np.abs(color[..., :3] - rc[..., :3]).mean() / np.abs(rc[..., :3]).mean()
+    print(f"[textured] within {tol}: {ok.mean():.6f} diverged={int((~ok).sum())} image rel diff {dimg:.2e}",
+          flush=True)
+    assert ok.mean() >= frac
+    assert dimg <= img
+
+
+def test_texture_ids_are_validated(ctx):
+    sc = scenes.textured_cornell()
+    sc.materials[5].tex[0] = len(sc.textures)   # one past the last image
+    with pytest.raises(abi.OxrError):
+        ctx.upload(sc)
+    sc.materials[5].tex[0] = -2
+    with pytest.raises(abi.OxrError):
+        ctx.upload(sc)
