@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 #include <stdexcept>
 
@@ -109,12 +110,52 @@ void TriangleMesh::LoadMesh(const std::string& objFile, const Material& mat, int
     if (faces == 0) throw std::runtime_error("Could not read OBJ model from " + objFile + " : no faces");
 }
 
+namespace {
+
+uint32_t le32(const uint8_t* p) { return p[0] | (p[1] << 8) | (p[2] << 16) | ((uint32_t)p[3] << 24); }
+
+// Uncompressed 24- or 32-bit BMP.  Rows end up bottom first, as stbi's
+// flipped load leaves them.
+void read_bmp(const std::string& fileName, std::ifstream& in, std::vector<uint8_t>& pixel, vec2i& res) {
+    std::vector<uint8_t> f((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto bad = [&](const char* why) { throw std::runtime_error("texture " + fileName + ": " + why); };
+    if (f.size() < 54 || f[0] != 'B' || f[1] != 'M') bad("not a BMP file");
+    const uint32_t off = le32(&f[10]);
+    const int32_t w = (int32_t)le32(&f[18]), h_signed = (int32_t)le32(&f[22]);
+    const int bpp = f[28] | (f[29] << 8);
+    const uint32_t compression = le32(&f[30]);
+    if (w <= 0 || h_signed == 0 || w > (1 << 15) || h_signed > (1 << 15) || h_signed < -(1 << 15))
+        bad("bad BMP size");
+    if ((bpp != 24 && bpp != 32) || (compression != 0 && !(compression == 3 && bpp == 32)))
+        bad("only uncompressed 24- and 32-bit BMP files are supported");
+    const int h = h_signed < 0 ? -h_signed : h_signed;
+    const size_t stride = ((size_t)w * (bpp / 8) + 3) & ~(size_t)3;
+    if (off > f.size() || stride * h > f.size() - off) bad("truncated BMP");
+    pixel.resize((size_t)w * h * 4);
+    for (int y = 0; y < h; y++) {
+        const int row = h_signed > 0 ? y : h - 1 - y;  // bottom-up files already start at the bottom
+        const uint8_t* s = &f[off + stride * y];
+        for (int x = 0; x < w; x++) {
+            uint8_t* d = &pixel[4 * ((size_t)row * w + x)];
+            const uint8_t* px = s + (size_t)x * (bpp / 8);
+            d[0] = px[2]; d[1] = px[1]; d[2] = px[0]; d[3] = 255;  // BGR(A) -> RGBA
+        }
+    }
+    res = vec2i(w, h);
+}
+
+}  // namespace
+
 Texture::Texture(const std::string& fileName) {
     std::ifstream in(fileName, std::ios::binary);
     if (!in) throw std::runtime_error("Could not read texture " + fileName);
+    if (in.peek() == 'B') {
+        read_bmp(fileName, in, pixel, resolution);
+        return;
+    }
     std::string magic;
     in >> magic;
-    if (magic != "P6") throw std::runtime_error("texture " + fileName + ": only binary PPM (P6) is supported");
+    if (magic != "P6") throw std::runtime_error("texture " + fileName + ": only binary PPM (P6) and BMP are supported");
     int w, h, maxv;
     in >> w >> h >> maxv;
     in.get();
@@ -274,6 +315,11 @@ void Scene::AddTriangles(const std::vector<float>& positions, const std::vector<
         mesh->texcoord.push_back(vec2f(corner == 1 ? 1.0f : 0.0f, corner == 2 ? 1.0f : 0.0f));
     }
     for (int t = 0; t < (int)positions.size() / 9; t++) mesh->index.push_back(vec3i(3 * t, 3 * t + 1, 3 * t + 2));
+    meshes.push_back(mesh);
+    for (auto& v : mesh->vertex) bounds.extend(v);
+}
+
+void Scene::AddMesh(TriangleMesh* mesh) {
     meshes.push_back(mesh);
     for (auto& v : mesh->vertex) bounds.extend(v);
 }

@@ -109,7 +109,7 @@ struct TextureFile {
 };
 
 // RGBA8 image, rows flipped like stbi_set_flip_vertically_on_load(true).
-// Reads binary PPM (P6) and PAM-less formats only; other formats throw.
+// Reads binary PPM (P6) and uncompressed 24/32-bit BMP; other formats throw.
 struct Texture {
     Texture(const std::string& fileName);
     std::vector<uint8_t> pixel;
@@ -139,6 +139,9 @@ public:
     // De-indexed triangle soup straight from memory (9 floats per triangle).
     void AddTriangles(const std::vector<float>& positions, const std::vector<float>& normals, const Material& material,
                       int out_med = -1, int in_med = -1);
+    // A mesh loaded by the caller (the scene file's transformed shapes); the
+    // scene takes ownership.
+    void AddMesh(TriangleMesh* mesh);
     void AddTexture(const std::string& fileName);
     void AddLight(const Light& l);
     void AddMedium(const Medium& m);

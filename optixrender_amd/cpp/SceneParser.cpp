@@ -1,6 +1,7 @@
 #include "SceneParser.h"
 
 #include <cctype>
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <sstream>
@@ -97,7 +98,46 @@ Value parse(const std::string& text) {
     return v;
 }
 
+vec3f Value::vec3(const std::string& k, vec3f dflt) const {
+    const Value* v = get(k);
+    if (!v) return dflt;
+    if (v->kind == Number) return vec3f((float)v->num);
+    if (v->kind != Array || v->arr.size() != 3 || v->arr[0].kind != Number || v->arr[1].kind != Number ||
+        v->arr[2].kind != Number)
+        throw std::runtime_error("json: '" + k + "' is not a number or a 3-number array");
+    return vec3f((float)v->arr[0].num, (float)v->arr[1].num, (float)v->arr[2].num);
+}
+
 }  // namespace oxr_json
+
+namespace {
+
+const oxr_json::Value* array_of(const oxr_json::Value& data, const char* key) {
+    const oxr_json::Value* v = data.get(key);
+    if (!v) return nullptr;
+    if (v->kind != oxr_json::Value::Array) throw std::runtime_error(std::string("json: '") + key + "' is not an array");
+    return v;
+}
+
+std::string need_string(const oxr_json::Value& o, const char* key, const char* what) {
+    const oxr_json::Value* v = o.get(key);
+    if (!v || v->kind != oxr_json::Value::String)
+        throw std::runtime_error(std::string("json: ") + what + " without a string '" + key + "'");
+    return v->str;
+}
+
+// Rotation by rx about x, then ry about y, then rz about z (degrees).
+vec3f rotate_xyz(vec3f p, vec3f deg) {
+    const float k = 3.14159265358979323846f / 180.0f;
+    float c = std::cos(deg.x * k), s = std::sin(deg.x * k);
+    p = vec3f(p.x, c * p.y - s * p.z, s * p.y + c * p.z);
+    c = std::cos(deg.y * k); s = std::sin(deg.y * k);
+    p = vec3f(c * p.x + s * p.z, p.y, -s * p.x + c * p.z);
+    c = std::cos(deg.z * k); s = std::sin(deg.z * k);
+    return vec3f(c * p.x - s * p.y, s * p.x + c * p.y, p.z);
+}
+
+}  // namespace
 
 void SceneParser::LoadFromJson(const std::string& fn, Scene& scene, bool& is_succeed) {
     is_succeed = false;
@@ -106,17 +146,189 @@ void SceneParser::LoadFromJson(const std::string& fn, Scene& scene, bool& is_suc
     std::stringstream buf;
     buf << in.rdbuf();
     oxr_json::Value data = oxr_json::parse(buf.str());
+    size_t slash = fn.find_last_of('/');
+    baseDir = slash == std::string::npos ? std::string() : fn.substr(0, slash + 1);
     is_succeed = true;
     Parse(data, scene);
 }
 
-void SceneParser::Parse(const oxr_json::Value& data, Scene&) {
+void SceneParser::Parse(const oxr_json::Value& data, Scene& scene) {
     width = (int)data.number("width", 1024);
     height = (int)data.number("height", 1024);
     depth = (int)data.number("depth", 4);
     const oxr_json::Value* f = data.get("filter");
     oxr_json::Value empty;
     ParseFilter(f ? *f : empty);
+    // media first (materials' shapes and lights name them), then materials,
+    // shapes (the bounds a default look_at needs), lights and the camera
+    ParseMedia(data, scene);
+    ParseMaterials(data, scene);
+    ParseShapes(data, scene);
+    ParseLights(data, scene);
+    ParseCamera(data, scene);
+}
+
+std::string SceneParser::path(const std::string& p) const {
+    return (p.empty() || p[0] == '/') ? p : baseDir + p;
+}
+
+int SceneParser::medium(const std::string& name) const {
+    if (name.empty()) return -1;
+    auto it = mediumIds.find(name);
+    if (it == mediumIds.end()) throw std::runtime_error("json: unknown medium '" + name + "'");
+    return it->second;
+}
+
+void SceneParser::ParseMedia(const oxr_json::Value& data, Scene& scene) {
+    const oxr_json::Value* arr = array_of(data, "medium");
+    if (!arr) return;
+    for (const oxr_json::Value& m : arr->arr) {
+        std::string name = need_string(m, "name", "medium");
+        if (m.string("type", "homogeneous") != "homogeneous")
+            throw std::runtime_error("json: medium '" + name + "': only homogeneous media exist (Medium.h)");
+        Medium md;
+        std::string phase = m.string("phase", "isotropic");
+        if (phase == "isotropic") md.phase.type = PhaseType::Isotropic;
+        else if (phase == "henyey_greenstein" || phase == "hg") md.phase.type = PhaseType::HenyeyGreenstein;
+        else throw std::runtime_error("json: medium '" + name + "': unknown phase '" + phase + "'");
+        md.phase.g = m.vec3("g", vec3f(0.0f));
+        md.sigma_a = m.vec3("sigma_a", md.sigma_a);
+        md.sigma_s = m.vec3("sigma_s", md.sigma_s);
+        md.scale = (float)m.number("scale", md.scale);
+        if (mediumIds.count(name)) throw std::runtime_error("json: medium '" + name + "' defined twice");
+        mediumIds[name] = (int)scene.mediums.size();
+        scene.AddMedium(md);
+    }
+}
+
+void SceneParser::ParseMaterials(const oxr_json::Value& data, Scene& scene) {
+    const oxr_json::Value* arr = array_of(data, "material");
+    if (!arr) return;
+    struct Kind { const char* name; MaterialType type; int smooth; };  // smooth: 0 no, 1 near-mirror, 2 Lambert
+    static const Kind kinds[] = {
+        {"diffuse", Diffuse, 0}, {"rough_diffuse", Diffuse, 0}, {"smooth_diffuse", Diffuse, 2},
+        {"conductor", Conductor, 0}, {"rough_conductor", Conductor, 0}, {"smooth_conductor", Conductor, 1},
+        {"dielectric", Dielectric, 0}, {"rough_dielectric", Dielectric, 0}, {"smooth_dielectric", Dielectric, 1},
+        {"plastic", Plastic, 0}, {"rough_plastic", Plastic, 0}, {"smooth_plastic", Plastic, 1},
+        {"metal_workflow", MetalWorkflow, 0}, {"thin_dielectric", ThinDielectric, 0},
+        {"clearcoated_conductor", ClearCoatedConductor, 0}, {"diffuse_transmitter", DiffuseTransmitter, 0},
+    };
+    for (const oxr_json::Value& m : arr->arr) {
+        std::string name = need_string(m, "name", "material");
+        std::string type = need_string(m, "type", "material");
+        const Kind* kind = nullptr;
+        for (const Kind& k : kinds)
+            if (type == k.name) kind = &k;
+        if (!kind) throw std::runtime_error("json: material '" + name + "': unknown type '" + type + "'");
+        Material mat;
+        mat.type = kind->type;
+        if (kind->smooth == 1) mat.roughness = kSmoothRoughness;
+        if (kind->smooth == 2) mat.roughness = 0.0f;
+        // a parameter: its constant, or an image when p_texture_type is "image"
+        auto image = [&](const std::string& p, int& id) {
+            std::string tt = m.string(p + "_texture_type", m.get(p + "_path") ? "image" : "constant");
+            if (tt == "constant") return false;
+            if (tt != "image")
+                throw std::runtime_error("json: material '" + name + "': " + p + "_texture_type '" + tt + "'");
+            std::string file = path(need_string(m, (p + "_path").c_str(), ("material '" + name + "'").c_str()));
+            auto it = textureIds.find(file);
+            if (it == textureIds.end()) {
+                it = textureIds.emplace(file, (int)scene.textures.size()).first;
+                scene.AddTexture(file);
+            }
+            id = it->second;
+            return true;
+        };
+        if (!image("albedo", mat.albedoTextureID)) mat.albedo = m.vec3("albedo", mat.albedo);
+        if (!image("roughness", mat.roughnessTextureID) && kind->smooth == 0)
+            mat.roughness = (float)m.number("roughness", mat.roughness);
+        if (!image("anisotropy", mat.anisotropyTextureID)) mat.anisotropy = (float)m.number("anisotropy", mat.anisotropy);
+        if (!image("specular", mat.specularTextureID)) mat.specular = m.vec3("specular", mat.specular);
+        if (!image("metallic", mat.metallicTextureID)) mat.metallic = (float)m.number("metallic", mat.metallic);
+        if (!image("coat_roughness_u", mat.coat_roughness_uTextureID))
+            mat.coat_roughness_u = (float)m.number("coat_roughness_u", mat.coat_roughness_u);
+        if (!image("coat_roughness_v", mat.coat_roughness_vTextureID))
+            mat.coat_roughness_v = (float)m.number("coat_roughness_v", mat.coat_roughness_v);
+        image("normal", mat.normalTextureID);
+        mat.eta = m.vec3("eta", mat.eta);
+        mat.k = m.vec3("k", mat.k);
+        mat.int_ior = (float)m.number("int_ior", mat.int_ior);
+        mat.ext_ior = (float)m.number("ext_ior", mat.ext_ior);
+        const oxr_json::Value* nl = m.get("nonlinear");
+        if (nl && nl->kind == oxr_json::Value::Bool) mat.nonlinear = nl->b;
+        mat.coat_weight = (float)m.number("coat_weight", mat.coat_weight);
+        if (materials.count(name)) throw std::runtime_error("json: material '" + name + "' defined twice");
+        materials[name] = mat;
+    }
+}
+
+void SceneParser::ParseShapes(const oxr_json::Value& data, Scene& scene) {
+    const oxr_json::Value* arr = array_of(data, "shape");
+    if (!arr) return;
+    for (const oxr_json::Value& s : arr->arr) {
+        if (s.string("type", "mesh") != "mesh") throw std::runtime_error("json: only mesh shapes exist");
+        std::string file = path(need_string(s, "mesh_path", "shape"));
+        Material mat;
+        std::string mname = s.string("material", "");
+        if (!mname.empty()) {
+            auto it = materials.find(mname);
+            if (it == materials.end()) throw std::runtime_error("json: unknown material '" + mname + "'");
+            mat = it->second;
+        }
+        const vec3f t = s.vec3("translate", vec3f(0.0f)), r = s.vec3("rotate", vec3f(0.0f));
+        const vec3f sc = s.vec3("scale", vec3f(1.0f));
+        if (sc.x == 0.0f || sc.y == 0.0f || sc.z == 0.0f) throw std::runtime_error("json: shape '" + file + "': zero scale");
+        TriangleMesh* mesh = new TriangleMesh();
+        try {
+            mesh->LoadMesh(file, mat, medium(s.string("medium_ext", "")), medium(s.string("medium_int", "")));
+        } catch (...) {
+            delete mesh;
+            throw;
+        }
+        // p' = translate + R (scale * p); normals by the inverse transpose, R (n / scale)
+        for (vec3f& v : mesh->vertex) v = t + rotate_xyz(v * sc, r);
+        for (vec3f& n : mesh->normal) n = normalize(rotate_xyz(n / sc, r));
+        scene.AddMesh(mesh);
+    }
+}
+
+void SceneParser::ParseLights(const oxr_json::Value& data, Scene& scene) {
+    const oxr_json::Value* arr = array_of(data, "light");
+    if (!arr) return;
+    for (const oxr_json::Value& l : arr->arr) {
+        std::string type = need_string(l, "type", "light");
+        if (type == "infinite") {
+            HdrTexture* env = new HdrTexture(path(need_string(l, "hdr_path", "infinite light")));
+            const float k = (float)l.number("scale", 1.0);
+            for (size_t i = 0; i < env->hdr.size(); i++)
+                if (i % 4 != 3) env->hdr[i] *= k;
+            scene.AddEnv(env);
+            continue;
+        }
+        Light lt;
+        if (type == "quad") lt.type = LightType::Quad;
+        else if (type == "sphere") lt.type = LightType::Sphere;
+        else throw std::runtime_error("json: unknown light type '" + type + "'");
+        lt.radiance = l.vec3("radiance", lt.radiance);
+        lt.position = l.vec3("position", lt.position);
+        lt.u = l.vec3("u", lt.u);
+        lt.v = l.vec3("v", lt.v);
+        lt.radius = (float)l.number("radius", lt.radius);
+        const oxr_json::Value* ds = l.get("double_side");
+        if (ds) lt.doubleSide = ds->kind == oxr_json::Value::Bool ? ds->b : ds->num != 0.0;
+        lt.medium = medium(l.string("medium", ""));
+        scene.AddLight(lt);
+    }
+}
+
+void SceneParser::ParseCamera(const oxr_json::Value& data, const Scene& scene) {
+    const oxr_json::Value* c = data.get("camera");
+    if (!c || c->kind != oxr_json::Value::Object) return;
+    hasCamera = true;
+    camera.from = c->vec3("look_from", vec3f(0.0f, 0.0f, 1.0f));
+    camera.at = c->vec3("look_at", scene.meshes.empty() ? vec3f(0.0f) : scene.bounds.center());
+    camera.up = c->vec3("up", vec3f(0.0f, 1.0f, 0.0f));
+    camera.medium = medium(c->string("medium", ""));
 }
 
 void SceneParser::ParseFilter(const oxr_json::Value& data) {

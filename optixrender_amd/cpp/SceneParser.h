@@ -1,9 +1,33 @@
-// Scene-from-JSON (reference: core/include/SceneParser.h, core/src/SceneParser.cpp).
-// LoadFromJson(fn, scene, is_succeed) with the reference's settings
-// (width, height, depth, filter).  Unlike the reference, the filter type is
-// actually stored (SceneParser.cpp:27-45 assigned it to a shadowing local) and
-// depth is exposed for the renderer's max depth.  A small JSON reader of our
-// own replaces nlohmann::json.
+// Scene-from-JSON (reference: core/include/SceneParser.h, core/src/SceneParser.cpp,
+// schema of models/test.json:1-128).  LoadFromJson(fn, scene, is_succeed)
+// reads the reference's settings (width, height, depth, filter) and, unlike
+// the reference, which stops there (SceneParser.cpp:11-23), the whole scene:
+//
+//   camera    look_from, look_at (default: the centre of the shapes' bounds),
+//             up (default +y), medium (a medium name)
+//   light[]   infinite: hdr_path, scale (the map's radiance times scale);
+//             quad: position, u, v, radiance, double_side, medium;
+//             sphere: position, radius, radiance, medium
+//   shape[]   mesh: mesh_path (OBJ), translate, rotate (degrees about x, then
+//             y, then z), scale (per axis), material (a name), medium_int /
+//             medium_ext (names: the mesh's inside / outside medium)
+//   material[] type (diffuse, conductor, dielectric, plastic, metal_workflow,
+//             thin_dielectric, clearcoated_conductor, diffuse_transmitter;
+//             smooth_* and rough_* variants of the first four), name, and per
+//             parameter p: p (a constant), p_texture_type "constant" or
+//             "image" with p_path (PPM or BMP); the scalars eta, k, int_ior,
+//             ext_ior, nonlinear, coat_weight, and normal_path for a normal map
+//   medium[]  homogeneous: name, phase "isotropic" or "henyey_greenstein" with
+//             g, sigma_a, sigma_s, scale
+//
+// Paths are relative to the JSON file's folder.  The reference's Material
+// types have no smooth BSDFs (Scene.h:59-68): smooth_* maps to the rough type
+// at roughness kSmoothRoughness (a near-mirror GGX lobe), and smooth_diffuse
+// to Diffuse at roughness 0 (Lambert).  The filter type is actually stored
+// (SceneParser.cpp:27-45 assigned it to a shadowing local) and depth is
+// exposed for the renderer's max depth.  A missing name, file or unknown type
+// throws std::runtime_error.  A small JSON reader of our own replaces
+// nlohmann::json.
 #pragma once
 #include <map>
 #include <memory>
@@ -32,10 +56,14 @@ struct Value {
         const Value* v = get(k);
         return (v && v->kind == String) ? v->str : dflt;
     }
+    // a 3-number array, or one number for all three; dflt when absent
+    vec3f vec3(const std::string& k, vec3f dflt) const;
 };
 // Throws std::runtime_error with the byte offset on malformed input.
 Value parse(const std::string& text);
 }  // namespace oxr_json
+
+constexpr float kSmoothRoughness = 0.02f;
 
 class SceneParser {
 public:
@@ -43,7 +71,22 @@ public:
     void LoadFromJson(const std::string& fn, Scene& scene, bool& is_succeed);
     void Parse(const oxr_json::Value& data, Scene& scene);
     void ParseFilter(const oxr_json::Value& data);
+    void ParseMedia(const oxr_json::Value& data, Scene& scene);
+    void ParseMaterials(const oxr_json::Value& data, Scene& scene);
+    void ParseShapes(const oxr_json::Value& data, Scene& scene);
+    void ParseLights(const oxr_json::Value& data, Scene& scene);
+    void ParseCamera(const oxr_json::Value& data, const Scene& scene);
 
     int width = 1024, height = 1024, depth = 4;
     FilterType filterType = FilterType::Gaussian;
+    bool hasCamera = false;  // the file held a camera
+    Camera camera;
+    std::string baseDir;     // folder of the JSON file; relative paths start here
+
+private:
+    std::string path(const std::string& p) const;
+    int medium(const std::string& name) const;
+    std::map<std::string, int> mediumIds;
+    std::map<std::string, Material> materials;
+    std::map<std::string, int> textureIds;  // one image per distinct file
 };

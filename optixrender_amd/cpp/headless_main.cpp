@@ -4,7 +4,11 @@
 // (DownloadPixels) as a binary PPM.
 //
 //   headless [settings.json] [out.ppm] [frames] [obj...]
-// Without OBJ arguments the scene is a procedural Cornell box.  With
+// A settings file with shapes is a whole scene (SceneParser), camera
+// included.  Otherwise, without OBJ arguments the scene is a procedural
+// Cornell box.  OXR_HEADLESS_DUMP=<file> writes the parsed scene as JSON
+// (meshes after their transforms, materials, lights, media, environment,
+// camera) and exits before any device call.  With
 // OXR_HEADLESS_DENOISE=1 in the environment the frames are denoised
 // (denoiserOn, the viewer's 'D' key).  OBJ meshes take their textures from
 // OXR_HEADLESS_ALBEDO / _ROUGHNESS / _METALLIC / _NORMAL (PPM files), and
@@ -12,6 +16,7 @@
 // gives its meshes texture files and its scene an HDR (main.cpp:165-213).
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <vector>
@@ -54,6 +59,72 @@ static void cornell(Scene& scene) {
     scene.AddLight(l);
 }
 
+static void dump_scene(const std::string& file, const Scene& scene, const SceneParser& parser, const Camera& cam) {
+    std::ofstream o(file);
+    if (!o) throw std::runtime_error("cannot write " + file);
+    o.precision(9);
+    auto v3 = [&](vec3f v) { o << "[" << v.x << "," << v.y << "," << v.z << "]"; };
+    o << "{\"width\":" << parser.width << ",\"height\":" << parser.height << ",\"depth\":" << parser.depth
+      << ",\"filter\":" << (int)parser.filterType << ",\"camera\":{\"from\":";
+    v3(cam.from); o << ",\"at\":"; v3(cam.at); o << ",\"up\":"; v3(cam.up);
+    o << ",\"medium\":" << cam.medium << "},\"meshes\":[";
+    for (size_t m = 0; m < scene.meshes.size(); m++) {
+        const TriangleMesh* me = scene.meshes[m];
+        const Material& a = me->material;
+        o << (m ? "," : "") << "{\"in_medium\":" << me->in_medium << ",\"out_medium\":" << me->out_medium
+          << ",\"material\":{\"type\":" << (int)a.type << ",\"albedo\":";
+        v3(a.albedo);
+        o << ",\"roughness\":" << a.roughness << ",\"anisotropy\":" << a.anisotropy << ",\"eta\":";
+        v3(a.eta); o << ",\"k\":"; v3(a.k); o << ",\"specular\":"; v3(a.specular);
+        o << ",\"metallic\":" << a.metallic << ",\"int_ior\":" << a.int_ior << ",\"ext_ior\":" << a.ext_ior
+          << ",\"nonlinear\":" << (a.nonlinear ? 1 : 0) << ",\"coat_weight\":" << a.coat_weight
+          << ",\"coat_roughness_u\":" << a.coat_roughness_u << ",\"coat_roughness_v\":" << a.coat_roughness_v
+          << ",\"tex\":[" << a.albedoTextureID << ","
+          << a.roughnessTextureID << "," << a.anisotropyTextureID << "," << a.specularTextureID << ","
+          << a.metallicTextureID << "," << a.coat_roughness_uTextureID << "," << a.coat_roughness_vTextureID << ","
+          << a.normalTextureID << "]},\"vertex\":[";
+        for (size_t i = 0; i < me->vertex.size(); i++) { o << (i ? "," : ""); v3(me->vertex[i]); }
+        o << "],\"normal\":[";
+        for (size_t i = 0; i < me->normal.size(); i++) { o << (i ? "," : ""); v3(me->normal[i]); }
+        o << "],\"texcoord\":[";
+        for (size_t i = 0; i < me->texcoord.size(); i++)
+            o << (i ? "," : "") << "[" << me->texcoord[i].x << "," << me->texcoord[i].y << "]";
+        o << "]}";
+    }
+    o << "],\"textures\":[";
+    for (size_t t = 0; t < scene.textures.size(); t++) {
+        const Texture* tx = scene.textures[t];
+        o << (t ? "," : "") << "{\"width\":" << tx->resolution.x << ",\"height\":" << tx->resolution.y << ",\"rgba\":[";
+        for (size_t i = 0; i < tx->pixel.size(); i++) o << (i ? "," : "") << (int)tx->pixel[i];
+        o << "]}";
+    }
+    o << "],\"lights\":[";
+    for (size_t l = 0; l < scene.lights.size(); l++) {
+        const Light& L = scene.lights[l];
+        o << (l ? "," : "") << "{\"type\":" << (int)L.type << ",\"double_side\":" << (L.doubleSide ? 1 : 0)
+          << ",\"radiance\":";
+        v3(L.radiance); o << ",\"position\":"; v3(L.position); o << ",\"u\":"; v3(L.u); o << ",\"v\":"; v3(L.v);
+        o << ",\"radius\":" << L.radius << ",\"medium\":" << L.medium << "}";
+    }
+    o << "],\"media\":[";
+    for (size_t m = 0; m < scene.mediums.size(); m++) {
+        const Medium& M = scene.mediums[m];
+        o << (m ? "," : "") << "{\"phase\":" << (int)M.phase.type << ",\"g\":";
+        v3(M.phase.g); o << ",\"sigma_a\":"; v3(M.sigma_a); o << ",\"sigma_s\":"; v3(M.sigma_s);
+        o << ",\"scale\":" << M.scale << "}";
+    }
+    o << "],\"env\":";
+    if (scene.env) {
+        double sum = 0.0;
+        for (size_t i = 0; i < scene.env->hdr.size(); i++)
+            if (i % 4 != 3) sum += scene.env->hdr[i];
+        o << "{\"width\":" << scene.env->width << ",\"height\":" << scene.env->height << ",\"rgb_sum\":" << sum << "}";
+    } else {
+        o << "null";
+    }
+    o << "}\n";
+}
+
 int main(int ac, char** av) {
     try {
         std::string settings = ac > 1 ? av[1] : "";
@@ -82,8 +153,26 @@ int main(int ac, char** av) {
         camera.medium = -1;
         SceneParser parser;
         bool ok = false;
-        if (!settings.empty()) parser.LoadFromJson(settings, scene, ok);
+        if (!settings.empty()) {
+            Scene parsed;
+            parser.LoadFromJson(settings, parsed, ok);
+            if (ok && !parsed.meshes.empty()) {  // a whole scene: it replaces the default one
+                scene.meshes.swap(parsed.meshes);
+                scene.textures.swap(parsed.textures);
+                scene.lights.swap(parsed.lights);
+                scene.mediums.swap(parsed.mediums);
+                std::swap(scene.env, parsed.env);
+                scene.bounds = parsed.bounds;
+                if (parser.hasCamera) camera = parser.camera;
+                else camera = Camera{vec3f(0.2f, 0.2f, 0.2f), scene.bounds.center(), vec3f(0.f, 1.f, 0.f)};
+            }
+        }
         if (!ok) { parser.width = 256; parser.height = 256; }
+        const std::string dump = env_str("OXR_HEADLESS_DUMP");
+        if (!dump.empty()) {
+            dump_scene(dump, scene, parser, camera);
+            return 0;
+        }
         Renderer renderer(&scene);
         renderer.Resize(vec2i(parser.width, parser.height));
         renderer.SetCamera(camera);

@@ -40,6 +40,7 @@ inline vec3f operator*(vec3f a, vec3f b) { return {a.x * b.x, a.y * b.y, a.z * b
 inline vec3f operator*(vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 inline vec3f operator*(float s, vec3f a) { return {s * a.x, s * a.y, s * a.z}; }
 inline vec3f operator/(vec3f a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline vec3f operator/(vec3f a, vec3f b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
 inline vec3f operator-(vec3f a) { return {-a.x, -a.y, -a.z}; }
 inline float dot(vec3f a, vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline vec3f cross(vec3f a, vec3f b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
