@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg (p50/p99 latency at equal RMSE)")
+    ap.add_argument("--no-viewer", action="store_true", help="skip the viewer leg (1-spp frames with readback)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
@@ -152,6 +153,36 @@ def measured_traffic(workload: str, tuning: str):
     if "traffic_bytes_per_launch" not in fields:
         return None
     return int(float(fields["traffic_bytes_per_launch"]))
+
+
+def viewer(ctx, args, scene, frames=32):
+    """The interactive viewer's loop (main.cpp: Render() then DownloadPixels()
+    every frame): progressive 1-spp frames at the bench resolution, read back
+    by the synchronous copy into pageable memory, and by the streamed,
+    double-buffered page-locked readback (OXR_FLAG_PRESENT)."""
+    from optixrender_amd import abi
+
+    W, H = args.width, args.height
+    out = {"frames": frames, "spp_per_frame": 1}
+    ctx.set_camera(scene.camera)
+    ctx.render(spp=1, max_depth=args.depth, progressive=True, flags=abi.OXR_FLAG_PRESENT)   # warm: graphs, buffers
+    ctx.present(copy=False)
+    for name, flags in (("download", 0), ("present", abi.OXR_FLAG_PRESENT)):
+        ctx.set_camera(scene.camera)
+        render_ms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            st = ctx.render(spp=1, max_depth=args.depth, progressive=True, flags=flags)
+            render_ms += st.render_ms
+            px = ctx.download_rgba8(W, H) if name == "download" else ctx.present(copy=False)[0]
+        wall = (time.perf_counter() - t0) * 1e3 / frames
+        out[f"{name}_ms_per_frame"] = round(wall, 3)
+        out[f"{name}_device_ms_per_frame"] = round(render_ms / frames, 3)
+        del px
+    out["fps_download"] = round(1e3 / out["download_ms_per_frame"], 1)
+    out["fps_present"] = round(1e3 / out["present_ms_per_frame"], 1)
+    print(f"[bench] viewer: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -282,6 +313,8 @@ def main():
                               "traced_rays_per_s": round((counted.extension_rays - counted.camera_rays_resolved +
                                                           counted.shadow_rays) /
                                                   (ms_per_step / 1e3), 0)})
+    if not args.no_viewer and not args.profile_steps:
+        out["config"]["viewer"] = viewer(ctx, args, scene)
     if not args.no_serving and not args.profile_steps:
         out["config"]["serving"] = serving(ctx, args)
     if not args.no_cpu_baseline and not args.profile_steps:

@@ -120,6 +120,8 @@ typedef struct oxr_scene_desc {
 #define OXR_FLAG_EAGER 4u               /* launch kernels one by one instead of replaying a captured hipGraph */
 #define OXR_FLAG_STAGE_TIMING 8u        /* per-stage device times (HIP events between stages; implies eager) */
 #define OXR_FLAG_NO_CAMERA_CULL 16u     /* trace every camera ray (results are identical either way) */
+#define OXR_FLAG_PRESENT 32u            /* copy the finished RGBA8 frame to page-locked host memory on a copy
+                                           stream, overlapped with what follows (oxr_present) */
 
 /* One render call = spp progressive frames of one sample each (reference
    Render() with numPixelSamples = 1, Renderer.cpp:635-793). */
@@ -204,6 +206,16 @@ int oxr_set_camera(oxr_ctx* ctx, const oxr_camera* cam);
 int oxr_render(oxr_ctx* ctx, const oxr_render_params* params);
 int oxr_get_stats(oxr_ctx* ctx, oxr_render_stats* out);
 int oxr_download_rgba8(oxr_ctx* ctx, uint32_t* out, size_t count);
+/* The viewer's readback, streamed and double-buffered as the reference's
+   comment asks for (Renderer.cpp:787-792, in place of DownloadPixels'
+   synchronous copy, Renderer.cpp:878-880): the RGBA8 frame of the newest
+   render made with OXR_FLAG_PRESENT, in page-locked host memory owned by the
+   context (width*height pixels, row 0 at the bottom as oxr_download_rgba8).
+   Waits only for that frame's copy.  The pointer stays valid through the next
+   render with the flag and is overwritten by the one after it; oxr_resize and
+   oxr_destroy free it.  frame_id: that render's stats.frame_id.
+   OXR_ERR_STATE when no render with the flag ran since the last resize. */
+int oxr_present(oxr_ctx* ctx, const uint32_t** pixels, int32_t* width, int32_t* height, int32_t* frame_id);
 /* which: 0 colour, 1 albedo, 2 normal, 3 denoised colour (after a render
    with denoise = 1); count = width*height float4. */
 int oxr_download_aov(oxr_ctx* ctx, int which, float* out_rgba, size_t count);

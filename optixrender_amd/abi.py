@@ -17,6 +17,7 @@ OXR_FLAG_COUNT_TRAVERSAL = 2
 OXR_FLAG_EAGER = 4
 OXR_FLAG_STAGE_TIMING = 8
 OXR_FLAG_NO_CAMERA_CULL = 16
+OXR_FLAG_PRESENT = 32
 
 # MaterialType (core/include/Scene.h:59-68)
 DIFFUSE, CONDUCTOR, DIELECTRIC, PLASTIC, METAL_WORKFLOW, THIN_DIELECTRIC, CLEARCOAT_CONDUCTOR, DIFFUSE_TRANSMITTER = range(8)
@@ -307,6 +308,8 @@ SIGNATURES = {
     "oxr_render": (C.c_int, [_P, _P]),
     "oxr_get_stats": (C.c_int, [_P, _P]),
     "oxr_download_rgba8": (C.c_int, [_P, _P, C.c_size_t]),
+    "oxr_present": (C.c_int, [_P, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int32)]),
     "oxr_download_aov": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "oxr_download_depth": (C.c_int, [_P, _P, C.c_size_t]),
     "oxr_denoiser_load": (C.c_int, [_P, _P, C.c_int64]),

@@ -164,6 +164,7 @@ void Renderer::Render() {
         LoadDenoiser(path);
     }
     p.denoise = denoiserOn ? 1 : 0;  // Renderer.cpp:660-785 (our KPCNN in place of the OptiX model)
+    p.flags = presentOn ? OXR_FLAG_PRESENT : 0u;
     OXR_CHECK(oxr_render(ctx, &p));
     launchParams.frame.frameID = LastStats().frame_id;
 }
@@ -200,7 +201,17 @@ void Renderer::Resize(const vec2i& newSize) {
 
 void Renderer::DownloadPixels(uint32_t h_pixels[]) {
     size_t n = (size_t)launchParams.frame.size.x * launchParams.frame.size.y;
+    if (presentOn) {
+        std::memcpy(h_pixels, PresentPixels(), 4 * n);
+        return;
+    }
     OXR_CHECK(oxr_download_rgba8(oxr_cpp::default_context(), h_pixels, n));
+}
+
+const uint32_t* Renderer::PresentPixels() {
+    const uint32_t* px = nullptr;
+    OXR_CHECK(oxr_present(oxr_cpp::default_context(), &px, nullptr, nullptr, nullptr));
+    return px;
 }
 
 oxr_render_stats Renderer::LastStats() const {

@@ -44,6 +44,13 @@ public:
     // executable.
     void LoadDenoiser(const std::string& path);
     bool progressive = false;
+    // Streamed readback for a viewer: with presentOn, Render() copies each
+    // frame to page-locked memory on a copy stream (Renderer.cpp:787-792
+    // asks for streams and double-buffering), DownloadPixels reads from there,
+    // and PresentPixels() hands out the page-locked frame itself (valid
+    // through the next Render()).
+    bool presentOn = false;
+    const uint32_t* PresentPixels();
     LaunchParams launchParams;
 
 protected:

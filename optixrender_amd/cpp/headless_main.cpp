@@ -10,7 +10,8 @@
 // (meshes after their transforms, materials, lights, media, environment,
 // camera) and exits before any device call.  With
 // OXR_HEADLESS_DENOISE=1 in the environment the frames are denoised
-// (denoiserOn, the viewer's 'D' key).  OBJ meshes take their textures from
+// (denoiserOn, the viewer's 'D' key); OXR_HEADLESS_PRESENT=1 reads the frame
+// back through the streamed, page-locked path (presentOn).  OBJ meshes take their textures from
 // OXR_HEADLESS_ALBEDO / _ROUGHNESS / _METALLIC / _NORMAL (PPM files), and
 // OXR_HEADLESS_ENV names an environment map (.hdr or .pfm), as main.cpp
 // gives its meshes texture files and its scene an HDR (main.cpp:165-213).
@@ -179,6 +180,7 @@ int main(int ac, char** av) {
         renderer.progressive = true;
         const char* dn = std::getenv("OXR_HEADLESS_DENOISE");
         renderer.denoiserOn = dn && std::string(dn) == "1";
+        renderer.presentOn = env_str("OXR_HEADLESS_PRESENT") == "1";  // streamed readback
         renderer.launchParams.filterType = parser.filterType;
         std::vector<uint32_t> pixels((size_t)parser.width * parser.height);
         for (int f = 0; f < frames; f++) renderer.Render();

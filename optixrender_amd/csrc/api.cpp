@@ -308,11 +308,28 @@ int oxr_create(int device, oxr_ctx** out) {
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
     if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
-        hipEventCreate(&c->td0) != hipSuccess || hipEventCreate(&c->td1) != hipSuccess)
+        hipEventCreate(&c->td0) != hipSuccess || hipEventCreate(&c->td1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->present_done[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->present_done[1], hipEventDisableTiming) != hipSuccess)
         return bail(OXR_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP);
     *out = c;
     return OXR_OK;
 }
+
+namespace {
+// Frees the page-locked readback frames once no copy into them runs.
+void free_present(oxr_ctx* c) {
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (int k = 0; k < 2; k++) {
+        if (c->present_host[k]) (void)hipHostFree(c->present_host[k]);
+        c->present_host[k] = nullptr;
+        c->present_busy[k] = false;
+    }
+    c->present_next = 0;
+    c->present_newest = -1;
+}
+}  // namespace
 
 int oxr_destroy(oxr_ctx* c) {
     if (!c) return OXR_OK;
@@ -326,8 +343,10 @@ int oxr_destroy(oxr_ctx* c) {
     dfree(c->path_radiance); dfree(c->reqs); dfree(c->cover);
     dfree(c->dn_lanes); dfree(c->dn_bias); dfree(c->denoised);
     if (c->dn_scratch) (void)hipFree(c->dn_scratch);
-    for (hipEvent_t ev : {c->t0, c->t1, c->td0, c->td1})
+    free_present(c);
+    for (hipEvent_t ev : {c->t0, c->t1, c->td0, c->td1, c->present_done[0], c->present_done[1]})
         if (ev) (void)hipEventDestroy(ev);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OXR_OK;
@@ -630,6 +649,7 @@ int oxr_resize(oxr_ctx* c, int w, int h) {
     if (!c || w <= 0 || h <= 0 || (int64_t)w * h > (1LL << 30)) return fail(c, OXR_ERR_INVALID, "invalid size");
     OXR_HIP(c, hipSetDevice(c->device));
     OXR_HIP(c, hipStreamSynchronize(c->stream));
+    free_present(c);
     dfree(c->color); dfree(c->albedo); dfree(c->normal); dfree(c->rgba); dfree(c->depth);
     const size_t np = (size_t)w * h;
     OXR_HIP(c, dalloc(&c->color, np));
@@ -767,6 +787,9 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
         windows++;
     }
     c->denoised_valid = false;
+    // the tone map rewrites rgba: a streamed readback of the last frame must have read it
+    for (int k = 0; k < 2; k++)
+        if (c->present_busy[k]) OXR_HIP(c, hipStreamWaitEvent(c->stream, c->present_done[k], 0));
     if (prm->denoise) {
         OXR_HIP(c, ensure_denoise_buffers(c, (size_t)npix));
         OXR_HIP(c, hipEventRecord(c->td0, c->stream));
@@ -780,6 +803,21 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     }
     launches++;
     OXR_HIP(c, hipEventRecord(c->t1, c->stream));
+    const bool present = (prm->flags & OXR_FLAG_PRESENT) != 0;
+    if (present) {
+        // the frame's copy to page-locked memory runs on the copy stream after
+        // the tone map; the caller's next steps overlap it
+        const int k = c->present_next;
+        if (!c->present_host[k]) {
+            OXR_HIP(c, hipStreamSynchronize(c->copy_stream));
+            OXR_HIP(c, hipHostMalloc((void**)&c->present_host[k], 4 * (size_t)npix, hipHostMallocDefault));
+        }
+        OXR_HIP(c, hipStreamWaitEvent(c->copy_stream, c->t1, 0));
+        OXR_HIP(c, hipMemcpyAsync(c->present_host[k], c->rgba, 4 * (size_t)npix, hipMemcpyDeviceToHost,
+                                  c->copy_stream));
+        OXR_HIP(c, hipEventRecord(c->present_done[k], c->copy_stream));
+        c->present_busy[k] = true;
+    }
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long tv[kTravCounters];
     OXR_HIP(c, hipMemcpy(tv, c->eng.trav, sizeof(tv), hipMemcpyDeviceToHost));
@@ -817,6 +855,11 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.windows = windows;
     c->frame_id += prm->spp;
     st.frame_id = c->frame_id;
+    if (present) {
+        c->present_frame[c->present_next] = c->frame_id;
+        c->present_newest = c->present_next;
+        c->present_next ^= 1;
+    }
     c->path_radiance_valid = keep ? (size_t)3 * prm->spp * npix : 0;
     if (ov) return fail(c, OXR_ERR_STATE, "traversal stack overflow");
     return OXR_OK;
@@ -903,6 +946,18 @@ int oxr_download_rgba8(oxr_ctx* c, uint32_t* out, size_t count) {
     if (count != (size_t)c->width * c->height) return fail(c, OXR_ERR_INVALID, "count must equal width*height");
     OXR_HIP(c, hipStreamSynchronize(c->stream));
     OXR_HIP(c, hipMemcpy(out, c->rgba, 4 * count, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
+int oxr_present(oxr_ctx* c, const uint32_t** pixels, int32_t* width, int32_t* height, int32_t* frame_id) {
+    if (!c || !pixels) return OXR_ERR_INVALID;
+    if (c->present_newest < 0) return fail(c, OXR_ERR_STATE, "no render with OXR_FLAG_PRESENT since the last resize");
+    const int k = c->present_newest;
+    OXR_HIP(c, hipEventSynchronize(c->present_done[k]));
+    *pixels = c->present_host[k];
+    if (width) *width = c->width;
+    if (height) *height = c->height;
+    if (frame_id) *frame_id = c->present_frame[k];
     return OXR_OK;
 }
 

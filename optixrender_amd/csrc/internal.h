@@ -144,4 +144,13 @@ struct oxr_ctx {
     void* dn_scratch = nullptr;
     size_t dn_pixels = 0;  // capacity of denoised / dn_scratch
     bool denoised_valid = false;
+    // streamed readback (OXR_FLAG_PRESENT): two page-locked frames written in
+    // turn on copy_stream; present_done[k] marks frame k's copy complete
+    hipStream_t copy_stream = nullptr;
+    uint32_t* present_host[2] = {nullptr, nullptr};
+    hipEvent_t present_done[2] = {nullptr, nullptr};
+    int present_frame[2] = {0, 0};
+    int present_next = 0;      // buffer the next present render writes
+    int present_newest = -1;   // buffer of the newest present render, -1 none
+    bool present_busy[2] = {false, false};  // a copy into k was enqueued (rgba must not change before it ends)
 };

@@ -188,6 +188,16 @@ class Context:
         self.check(self._lib.oxr_autotune(self.h, C.byref(p), C.byref(t), C.byref(ms)), "oxr_autotune")
         return t, ms.value
 
+    def present(self, copy: bool = True):
+        """(pixels (h, w) uint32, frame_id) of the newest render made with
+        OXR_FLAG_PRESENT.  copy=False returns a view of the context's
+        page-locked buffer, valid through the next render with the flag."""
+        px = C.POINTER(C.c_uint32)()
+        w, h, fid = C.c_int32(), C.c_int32(), C.c_int32()
+        self.check(self._lib.oxr_present(self.h, C.byref(px), C.byref(w), C.byref(h), C.byref(fid)), "oxr_present")
+        view = np.ctypeslib.as_array(px, shape=(h.value, w.value))
+        return (view.copy() if copy else view), fid.value
+
     def download_rgba8(self, w: int, h: int) -> np.ndarray:
         out = np.zeros(w * h, np.uint32)
         self.check(self._lib.oxr_download_rgba8(self.h, abi.ptr(out), out.size), "oxr_download_rgba8")

@@ -87,3 +87,18 @@ def test_headless_textured_obj(tmp_path):
     left, right = mid[:, int(0.28 * w): int(0.42 * w)], mid[:, int(0.58 * w): int(0.72 * w)]
     assert left[..., 0].mean() > 2 * left[..., 2].mean()     # red half on the left
     assert right[..., 2].mean() > 2 * right[..., 0].mean()   # blue half on the right
+
+
+@pytest.mark.gpu
+def test_headless_present_readback_is_the_same_image(tmp_path):
+    # presentOn: the frame comes back through the streamed page-locked copy
+    import os
+    imgs = []
+    for present in ("0", "1"):
+        out = tmp_path / f"p{present}.ppm"
+        env = dict(os.environ, OXR_HEADLESS_PRESENT=present)
+        r = subprocess.run([str(HEADLESS), str(ROOT / "tests" / "data" / "settings.json"), str(out), "3"],
+                           capture_output=True, text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        imgs.append(read_ppm(out))
+    assert imgs[0].mean() > 10 and np.array_equal(imgs[0], imgs[1])

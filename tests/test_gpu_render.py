@@ -148,3 +148,50 @@ def test_autotune_picks_a_valid_tuning(ctx, cornell):
         assert ctx.get_tuning().as_dict() == t.as_dict()
     finally:
         ctx.set_tuning(base)
+
+
+def test_present_is_the_frame_and_double_buffered(ctx, cornell):
+    # the streamed readback hands out exactly the tone-mapped frame; a view of
+    # one frame survives the next present render and is reused by the one after
+    sc, sid = cornell
+    P = abi.OXR_FLAG_PRESENT
+    print("[present] cornell 48x40", flush=True)
+    _, px1, st1 = render(ctx, sc, sid, spp=2, max_depth=8, rr_start=3, progressive=True, flags=P)
+    v1, f1 = ctx.present(copy=False)
+    assert f1 == st1.frame_id == 2
+    assert np.array_equal(v1, px1)
+    keep1 = v1.copy()
+    st2 = ctx.render(spp=1, max_depth=8, rr_start=3, progressive=True, flags=P)
+    v2, f2 = ctx.present(copy=False)
+    assert f2 == st2.frame_id == 3
+    assert np.array_equal(v2, ctx.download_rgba8(W, H))
+    assert np.array_equal(v1, keep1)                 # the other buffer: untouched by frame 3
+    st3 = ctx.render(spp=1, max_depth=8, rr_start=3, progressive=True, flags=P)
+    v3, f3 = ctx.present(copy=False)
+    assert f3 == st3.frame_id == 4 and v3.ctypes.data == v1.ctypes.data   # frame 4 reuses frame 2's buffer
+    assert np.array_equal(v3, ctx.download_rgba8(W, H))
+    # a render without the flag leaves the newest presented frame as it was
+    ctx.render(spp=1, max_depth=8, rr_start=3, progressive=True)
+    v4, f4 = ctx.present()
+    assert f4 == 4 and np.array_equal(v4, v3)
+
+
+def test_present_needs_a_present_render(ctx, cornell):
+    sc, sid = cornell
+    ctx.select(sid)
+    ctx.resize(W, H)                                 # frees the page-locked frames
+    ctx.set_camera(sc.camera)
+    with pytest.raises(abi.OxrError):
+        ctx.present()
+    ctx.render(spp=1, max_depth=4)
+    with pytest.raises(abi.OxrError):
+        ctx.present()
+
+
+def test_present_of_a_denoised_frame(ctx, cornell):
+    sc, sid = cornell
+    from optixrender_amd import denoise
+    ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
+    _, px, _ = render(ctx, sc, sid, spp=2, max_depth=8, rr_start=3, denoise=True, flags=abi.OXR_FLAG_PRESENT)
+    v, _ = ctx.present()
+    assert np.array_equal(v, px)
