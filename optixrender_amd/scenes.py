@@ -294,6 +294,42 @@ def sphere_field(seed: int = 1, subdiv: int = 6, count: int = 12, env_w: int = 2
     return SceneData(pos, nrm, mid, mats, [], [], env, cam)
 
 
+def cornell_all_materials(subdiv: int = 2) -> SceneData:
+    """The Cornell box with eight small icospheres, one per material type
+    (Material.h: Diffuse, Conductor, Dielectric, Plastic, MetalWorkflow,
+    ThinDielectric, ClearCoatedConductor, DiffuseTransmitter), in two rows of
+    four on the floor."""
+    sc = cornell_spheres(subdiv)
+    n_walls = 12
+    pos = [sc.positions[:n_walls].astype(np.float64)]
+    mid = [sc.mat_ids[:n_walls]]
+    mats = list(sc.materials[:3])
+    kinds = [
+        abi.material(abi.DIFFUSE, albedo=(0.8, 0.6, 0.3), roughness=0.5),
+        abi.material(abi.CONDUCTOR, roughness=0.35, anisotropy=0.3),
+        abi.material(abi.DIELECTRIC, roughness=0.2, int_ior=1.5, ext_ior=1.0),
+        abi.material(abi.PLASTIC, roughness=0.3, albedo=(0.2, 0.5, 0.8)),
+        abi.material(abi.METAL_WORKFLOW, roughness=0.4, albedo=(0.9, 0.7, 0.4), metallic=0.6),
+        abi.material(abi.THIN_DIELECTRIC, roughness=0.25, int_ior=1.4, ext_ior=1.0),
+        abi.material(abi.CLEARCOAT_CONDUCTOR, roughness=0.4, coat_weight=0.8, coat_roughness_u=0.15,
+                     coat_roughness_v=0.3),
+        abi.material(abi.DIFFUSE_TRANSMITTER, albedo=(0.6, 0.7, 0.5)),
+    ]
+    ico = icosphere(subdiv)
+    for i, m in enumerate(kinds):
+        r = 0.2
+        c = (-0.75 + 0.5 * (i % 4), r, -0.3 + 0.8 * (i // 4))
+        pos.append(ico * r + np.asarray(c))
+        mid.append(np.full(len(ico), len(mats), np.uint32))
+        mats.append(m)
+    p = np.concatenate(pos).astype(np.float32)
+    sc.positions = p
+    sc.normals = flat_normals(np.concatenate(pos)).astype(np.float32)
+    sc.mat_ids = np.concatenate(mid)
+    sc.materials = mats
+    return sc
+
+
 def lit_sphere_field(seed: int = 1, subdiv: int = 2, env_w: int = 256, env_h: int = 128,
                      with_env: bool = True) -> SceneData:
     """The sphere field with two analytic lights in the camera's view: a
