@@ -150,7 +150,7 @@ void dfree(T*& p) {
 
 void free_scene(SceneDev* s) {
     dfree(s->positions); dfree(s->normals); dfree(s->mat_ids); dfree(s->normals_sorted); dfree(s->mat_sorted);
-    dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache);
+    dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache); dfree(s->env_pdf);
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
     dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
     dfree(s->cover_boxes); dfree(s->cover_count);
@@ -259,6 +259,7 @@ void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_b
     p.n_media = s->n_media;
     p.env = s->env;
     p.env_cache = s->env_cache;
+    p.env_pdf = s->env_pdf;
     p.env_w = s->env_w;
     p.env_h = s->env_h;
     p.lut_bsdf = lut_bsdf;
@@ -447,6 +448,7 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
         (e = dalloc(&s->mat_ids, n)) == hipSuccess && (e = dalloc(&s->mats, d->n_materials)) == hipSuccess &&
         (e = dalloc(&s->lights, d->n_lights)) == hipSuccess && (e = dalloc(&s->media, d->n_media)) == hipSuccess &&
         (e = dalloc(&s->env, env_px)) == hipSuccess && (e = dalloc(&s->env_cache, env_px)) == hipSuccess &&
+        (e = dalloc(&s->env_pdf, env_px)) == hipSuccess &&
         (e = up(s->positions, d->positions, 36 * n)) == hipSuccess &&
         (e = up(s->normals, d->normals, 36 * n)) == hipSuccess && (e = up(s->mat_ids, d->mat_ids, 4 * n)) == hipSuccess &&
         (e = up(s->mats, d->materials, sizeof(oxr_material) * d->n_materials)) == hipSuccess &&
@@ -543,7 +545,8 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         es = hipMemcpy(&bad, bvh_error_word(scratch, n), sizeof(bad), hipMemcpyDeviceToHost);
     // the environment cache reuses the scratch only after the BVH error word was read
     if (e == hipSuccess && es == hipSuccess && s->env_w > 0) {
-        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_lum_sum, s->env_cache, (float*)scratch, c->stream);
+        e = env_cache_build(s->env, s->env_w, s->env_h, s->env_lum_sum, s->env_cache, s->env_pdf, (float*)scratch,
+                            c->stream);
         if (e == hipSuccess) es = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(scratch);

@@ -56,7 +56,8 @@ hipError_t trace_rays(const BvhView& b, const float* d_rays, int64_t n, int any,
 inline size_t gstack_words(int grid) { return (size_t)(kStackTotal - kStackLdsMin) * grid * kTraceBlock; }
 // lum_sum: the sequential float sum of the texel luminances in row-major
 // order (the reference's host loop), computed by env_luminance_sum.
-hipError_t env_cache_build(const float4* d_env, int w, int h, float lum_sum, float4* d_cache, float* d_scratch,
+hipError_t env_cache_build(const float4* d_env, int w, int h, float lum_sum, float4* d_cache, float4* d_env_pdf,
+                           float* d_scratch,
                            hipStream_t s);
 float env_luminance_sum(const float* rgba, int64_t texels);
 size_t env_scratch_floats(int w, int h);
@@ -154,6 +155,7 @@ struct WaveParams {
     int n_media;
     const float4* env;
     const float4* env_cache;
+    const float4* env_pdf;  // per texel (env r, g, b, cache pdf): the miss shade and env NEE read both
     int env_w, env_h;
     const float* lut_bsdf;
     const float* lut_albedo;

@@ -27,6 +27,7 @@ struct SceneDev {
     int n_media = 0;
     float4* env = nullptr;
     float4* env_cache = nullptr;
+    float4* env_pdf = nullptr;  // (env rgb, cache pdf) per texel
     int env_w = 0, env_h = 0;
     float env_lum_sum = 0.0f;  // sequential luminance sum of the environment (host, at upload)
     BvhBuffers bvh;

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_sky_shade(WaveParams p) {
         } else {
             if (p.env_w > 0) {                                              // devicePrograms.cu:354-373
                 f2 uv = sphere_to_plane(dir);
-                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                float4 e = bilinear(p.env_pdf, p.env_w, p.env_h, uv.x, uv.y);
                 const f3 lr(e.x, e.y, e.z);
                 const float mw = 1.0f;
                 radiance = radiance + mw * history * lr;
@@ -796,14 +796,14 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
         } else if (closest == kFltMax && medium_index == -1) {              // devicePrograms.cu:354-373
             if (p.env_w > 0) {
                 f2 uv = sphere_to_plane(dir);
-                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                float4 e = bilinear(p.env_pdf, p.env_w, p.env_h, uv.x, uv.y);  // radiance and cache pdf
                 light_radiance = f3(e.x, e.y, e.z);
                 float mw = 1.0f;
                 if (bounce != 0) {
                     // the environment pdf only enters the MIS weight, which is
-                    // 1 for camera rays: their cache lookup is skipped
+                    // 1 for camera rays
                     light_pdf = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
-                    light_pdf *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
+                    light_pdf *= e.w;
                     pre_pdf *= mult_trans_pdf;
                     mw = power_heuristic(pre_pdf, light_pdf);
                 }
@@ -821,8 +821,8 @@ OXR_D bool shade_path(const WaveParams& p, const Lut& lut, uint32_t pid, float4 
                 f2 uv{uvt.x, uvt.y};
                 f3 sd = plane_to_sphere(uv);
                 float lp = (float)(p.env_w * p.env_h) / (2.0f * sqr(kPi));
-                lp *= bilinear(p.env_cache, p.env_w, p.env_h, uv.x, uv.y).z;
-                float4 e = bilinear(p.env, p.env_w, p.env_h, uv.x, uv.y);
+                const float4 e = bilinear(p.env_pdf, p.env_w, p.env_h, uv.x, uv.y);  // radiance and cache pdf
+                lp *= e.w;
                 f3 lr(e.x, e.y, e.z);
                 f3 tv; float tp; int tm;
                 float cd = kFltMax;
@@ -1309,8 +1309,18 @@ hipError_t trace_rays(const BvhView& b, const float* rays, int64_t n, int any, f
 
 size_t env_scratch_floats(int w, int h) { return (size_t)3 * w * h + 2 * (size_t)w + 16; }
 
-hipError_t env_cache_build(const float4* env, int w, int h, float lum_sum, float4* cache, float* scratch,
-                           hipStream_t s) {
+// The environment radiance and the cache's pdf channel side by side: the
+// miss shade and next-event estimation read both at one coordinate, so one
+// bilinear footprint of 16-B texels serves both (the per-channel arithmetic
+// is the same as two separate lookups).
+__global__ void k_env_pack(const float4* __restrict__ env, const float4* __restrict__ cache, int64_t m,
+                           float4* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = make_float4(env[i].x, env[i].y, env[i].z, cache[i].z);
+}
+
+hipError_t env_cache_build(const float4* env, int w, int h, float lum_sum, float4* cache, float4* env_pdf,
+                           float* scratch, hipStream_t s) {
     int64_t m = (int64_t)w * h;
     float* lum = scratch;
     float* pdf = lum + m;
@@ -1321,6 +1331,7 @@ hipError_t env_cache_build(const float4* env, int w, int h, float lum_sum, float
     k_env_columns<<<blocks(w, 64), 64, 0, s>>>(lum, lum_sum, w, h, pdf, marg, cdfy);
     k_env_cdfx<<<1, 64, 0, s>>>(marg, w, cdfx);
     k_env_search<<<blocks(m, 256), 256, 0, s>>>(pdf, cdfx, cdfy, w, h, cache);
+    k_env_pack<<<blocks(m, 256), 256, 0, s>>>(env, cache, m, env_pdf);
     return hipGetLastError();
 }
 
