@@ -172,13 +172,16 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
         float tn0, tn1;
         bool h0 = box_test(r, lo0, hi0, tmin, h.t, tn0);
         bool h1 = box_test(r, lo1, hi1, tmin, h.t, tn1);
-        if (h0 && (c0 & kLeafBit)) {
-            h0 = false;
-            if (leaf_test(c0 & ~kLeafBit) && ANY) break;
-        }
-        if (h1 && (c1 & kLeafBit)) {
-            h1 = false;
-            if (leaf_test(c1 & ~kLeafBit) && ANY) break;
+        // Leaf children: child 0's triangle, then child 1's.  One block
+        // serves every lane with a leaf on either side and a second only the
+        // lanes with two, so a wave whose lanes reach leaves on different
+        // sides runs the triangle test once instead of twice.
+        const bool l0 = h0 && (c0 & kLeafBit), l1 = h1 && (c1 & kLeafBit);
+        h0 = h0 && !l0;
+        h1 = h1 && !l1;
+        if (l0 || l1) {
+            if (leaf_test((l0 ? c0 : c1) & ~kLeafBit) && ANY) break;
+            if (l0 && l1 && leaf_test(c1 & ~kLeafBit) && ANY) break;
         }
         if (h0 && h1) {
             // closest hit: nearer child first.  Any hit: farther child first --
