@@ -163,8 +163,10 @@ OXR_D int wrap_i(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
 // texel(i): the float4 value of texel i (row-major).
 template <class Texel>
 OXR_D float4 bilinear_fetch(int w, int h, float u, float v, Texel texel) {
-    if (!(u == u)) u = 0.0f;
-    if (!(v == v)) v = 0.0f;
+    // a NaN or infinite coordinate (from a NaN direction or garbage texture
+    // coordinates) reads at 0
+    if (!(fabsf(u) <= kFltMax)) u = 0.0f;
+    if (!(fabsf(v) <= kFltMax)) v = 0.0f;
     u = u - floorf(u);
     v = v - floorf(v);
     float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;

@@ -223,8 +223,8 @@ inline int wrap_index(int i, int n) { int m = i % n; return m < 0 ? m + n : m; }
 // texel(i, c): channel c of texel i (row-major)
 template <typename Texel>
 inline void bilinear_fetch(int w, int h, float u, float v, Texel texel, float* out) {
-    if (!(u == u)) u = 0.0f;
-    if (!(v == v)) v = 0.0f;
+    if (!(fabsf(u) <= FLT_MAX)) u = 0.0f;  // NaN or infinite coordinates read at 0
+    if (!(fabsf(v) <= FLT_MAX)) v = 0.0f;
     u = u - floorf(u);
     v = v - floorf(v);
     float x = u * (float)w - 0.5f;

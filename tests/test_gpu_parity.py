@@ -364,6 +364,18 @@ def test_render_textured_cornell_per_path(ctx, uvs, tail):
     assert dimg <= img
 
 
+def test_non_finite_texture_coordinates(ctx):
+    # NaN and infinite texture coordinates read the texel at 0 on both sides
+    sc = scenes.textured_cornell()
+    sc.uvs = sc.uvs.copy()
+    sc.uvs[0, :, 0] = np.nan
+    sc.uvs[1, :, 1] = np.inf
+    sc.uvs[4, 1, :] = -np.inf
+    print("[render] textured cornell, non-finite texture coordinates", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 48, 48, 2, 4, 3, filt=abi.FILTER_BOX)
+    assert np.array_equal(albedo.view(np.uint32), ra.view(np.uint32))
+
+
 def test_texture_ids_are_validated(ctx):
     sc = scenes.textured_cornell()
     sc.materials[5].tex[0] = len(sc.textures)   # one past the last image

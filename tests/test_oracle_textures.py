@@ -80,6 +80,15 @@ def test_wrap_is_periodic():
     assert abs(got[0] - 100 / 255.0) < 1e-6
 
 
+def test_non_finite_coordinates_read_at_zero():
+    rng = np.random.default_rng(6)
+    img = rng.integers(0, 256, (5, 7, 4), dtype=np.uint8)
+    ref = orc.texture_sample(img, 0.0, 0.25)
+    for bad in (float("nan"), float("inf"), -float("inf")):
+        assert np.array_equal(orc.texture_sample(img, bad, 0.25), ref)
+    assert np.array_equal(orc.texture_sample(img, 0.5, float("inf")), orc.texture_sample(img, 0.5, 0.0))
+
+
 def test_matches_float64_bilinear_on_random_images():
     rng = np.random.default_rng(3)
     for h, w in [(1, 1), (1, 9), (13, 1), (29, 37), (64, 48)]:
