@@ -114,3 +114,33 @@ def test_denoised_request_equals_denoised_render(ctx, cornell):
     assert np.array_equal(col.view(np.uint32), ctx.download_aov(3, w, h).view(np.uint32))
     assert np.array_equal(px, ctx.download_rgba8(w, h))
 
+
+
+def test_mixed_scenes_in_one_tick(ctx, cornell):
+    # requests over different resident scenes share ticks: a textured scene,
+    # a lit sphere field (sky pass, lights in view) and the Cornell box; each
+    # result equals its single render
+    _, cid = cornell
+    tex = scenes.textured_cornell(subdiv=2)
+    lit = scenes.lit_sphere_field(subdiv=1)
+    tid, lid = ctx.upload(tex), ctx.upload(lit)
+    ctx.build_bvh(tid)
+    ctx.build_bvh(lid)
+    reqs = [(tid, 40, 32, 3, tex.camera), (lid, 48, 27, 2, lit.camera), (cid, 32, 32, 2, scenes.cornell_spheres().camera),
+            (tid, 24, 24, 4, tex.camera)]
+    srv = Server(ctx, budget_bytes=1 << 30, max_active=8, tick_paths=4000, max_frame_pixels=64 * 64)
+    try:
+        ids = [srv.submit(s, w, h, spp, cam, max_depth=8) for s, w, h, spp, cam in reqs]
+        for rid in ids:
+            assert srv.wait(rid, 60000) == 0
+        got = [srv.result_color(rid) for rid in ids]
+        assert srv.stats().ticks >= 2
+    finally:
+        srv.close()
+    try:
+        for (s, w, h, spp, cam), col in zip(reqs, got):
+            ref, _ = single(ctx, s, w, h, spp, cam)
+            assert np.array_equal(col.view(np.uint32), ref.view(np.uint32))
+    finally:
+        ctx.release(tid)
+        ctx.release(lid)
