@@ -1006,8 +1006,11 @@ OXR_D void tail_shadow(const WaveParams& p, float4* recs, const ShadowOut& s, ui
 
 // Tail mode (k_trace sets the flag when queue x holds few paths): the paths
 // listed under every shade class of parity x are carried to their end here,
-// each lane one path: shade (run-time material dispatch), trace its shadow
-// rays at once, finish it or trace its next ray, and so on; then the slot
+// each lane one path at a time: shade (run-time material dispatch), trace its
+// shadow rays at once, finish it or trace its next ray, and so on; a lane
+// whose path ended takes the next listed path (one atomic per wave), so the
+// launch lasts about as long as its longest path rather than the sum of the
+// longest path of each 64-path chunk a wave is given.  A finished path's slot
 // starts a new sample if any is left (queue x^1).  The operations and their
 // order per path are those of the wavefront, so the result is the same.
 // Launched on the trace grid: the traversal stacks are that grid's.
@@ -1017,67 +1020,77 @@ __global__ __launch_bounds__(kTraceBlock) void k_tail(WaveParams p, int x) {
     if (!*ctr_at(p.ctr, kCtrTailMode)) return;
     __shared__ uint32_t lds[S * kTraceBlock];
     const int k = wave_shard();
-    int first[kShadeClasses + 1];
+    int first[kShadeClasses + 1];  // the classes' lists back to back, in paths
     first[0] = 0;
     for (int c = 0; c < kShadeClasses; c++)
-        first[c + 1] = first[c] + ((*ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + c) * kShards + k) + 63) >> 6);
-    const int chunks = first[kShadeClasses];
-    if (chunks == 0) return;
-    int* fetch = ctr_at(p.ctr, kCtrFetchClass + k);  // class 0's counter: the class kernels are idle
+        first[c + 1] = first[c] + *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + c) * kShards + k);
+    const int total = first[kShadeClasses];
+    if (total == 0) return;
+    int* fetch = ctr_at(p.ctr, kCtrFetchClass + k);  // class 0's counter (in paths): the class kernels are idle
     const uint32_t* q = p.queue[x];
     Stack<S> st(lds, p.gstack, (int64_t)gridDim.x * blockDim.x);
     Lut lut{p.lut_bsdf, p.lut_albedo};
     uint32_t nn = 0, nt = 0, sn = 0, stt = 0, ext_rays = 0, sh_rays = 0;
+    bool have = false, fresh = false, listed_left = true;
+    uint32_t pid = 0, g_fresh = 0;
+    float4 hit = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     while (true) {
-        const int j = wave_next(fetch);
-        if (j >= chunks) break;
-        int c = 0;
-        while (j >= first[c + 1]) c++;
-        const int n = *ctr_at(p.ctr, kCtrClass + (x * kShadeClasses + c) * kShards + k);
-        const int slot = ((j - first[c]) << 6) + lane_id();
-        const bool active = slot < n;
-        uint32_t pid = 0;
-        if (active) {
-            const uint32_t qp = p.cls_queue[((size_t)c * kShards + k) * p.region + slot];
-            pid = q[qp];
-            bool fresh = is_fresh(p, qp);
-            uint32_t g_fresh = fresh ? fresh_sample(p, x, qp) : 0u;
-            float4 hit = (c == 0 || c == kClassCameraMiss) ? make_float4(kFltMax, __uint_as_float(kNoHit), 0.0f, 0.0f)
-                                                           : p.hits[qp];
-            while (true) {
-                ShadowOut sev, slv;
-                sev.emit = slv.emit = false;
-                float4 rad;
-                bool aov_default = false;
-                const bool cont = shade_path<kMatAny, MEDIA>(p, lut, pid, hit, fresh, g_fresh, sev, slv, rad,
-                                                             aov_default);
-                fresh = false;
-                const bool final = !cont && !sev.emit && !slv.emit;
-                if (p.env_w > 0) tail_shadow<COUNT>(p, p.sh_env, sev, pid, !final, st, sn, stt, sh_rays);
-                if (p.n_lights != 0) tail_shadow<COUNT>(p, p.sh_light, slv, pid, !final, st, sn, stt, sh_rays);
-                if (!cont) {
-                    const uint32_t g = __float_as_uint(rad.w);
-                    if (final) {
-                        p.s_rad[g] = make_float4(rad.x, rad.y, rad.z, aov_default ? 1.0f : 0.0f);
-                    } else {
-                        float4 r = rad;
-                        add_pending(p, pid, r);  // as k_regen finishes a path
-                        p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
-                    }
-                    if (aov_default && !final) {
-                        p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                        p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                        p.s_dep[g] = 1.0f;
-                    }
-                    break;
+        // lanes without a path take the next listed ones
+        const uint64_t need = __ballot(!have);
+        if (need != 0 && listed_left) {
+            int base = 0;
+            if (lane_id() == 0) base = atomicAdd(fetch, (int)__popcll(need));
+            base = __shfl(base, 0);
+            listed_left = base + (int)__popcll(need) < total;
+            const int e = base + (int)__popcll(need & lanes_below());
+            if (!have && e < total) {
+                int c = 0;
+                while (e >= first[c + 1]) c++;
+                const uint32_t qp = p.cls_queue[((size_t)c * kShards + k) * p.region + (e - first[c])];
+                pid = q[qp];
+                fresh = is_fresh(p, qp);
+                g_fresh = fresh ? fresh_sample(p, x, qp) : 0u;
+                hit = (c == 0 || c == kClassCameraMiss) ? make_float4(kFltMax, __uint_as_float(kNoHit), 0.0f, 0.0f)
+                                                        : p.hits[qp];
+                have = true;
+            }
+        }
+        if (__ballot(have) == 0) break;
+        bool ended = false;
+        if (have) {
+            ShadowOut sev, slv;
+            sev.emit = slv.emit = false;
+            float4 rad;
+            bool aov_default = false;
+            const bool cont = shade_path<kMatAny, MEDIA>(p, lut, pid, hit, fresh, g_fresh, sev, slv, rad, aov_default);
+            fresh = false;
+            const bool final = !cont && !sev.emit && !slv.emit;
+            if (p.env_w > 0) tail_shadow<COUNT>(p, p.sh_env, sev, pid, !final, st, sn, stt, sh_rays);
+            if (p.n_lights != 0) tail_shadow<COUNT>(p, p.sh_light, slv, pid, !final, st, sn, stt, sh_rays);
+            if (!cont) {
+                const uint32_t g = __float_as_uint(rad.w);
+                if (final) {
+                    p.s_rad[g] = make_float4(rad.x, rad.y, rad.z, aov_default ? 1.0f : 0.0f);
+                } else {
+                    float4 r = rad;
+                    add_pending(p, pid, r);  // as k_regen finishes a path
+                    p.s_rad[g] = make_float4(r.x, r.y, r.z, 0.0f);
                 }
+                if (aov_default && !final) {
+                    p.s_nrm[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                    p.s_alb[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                    p.s_dep[g] = 1.0f;
+                }
+                have = false;
+                ended = true;
+            } else {
                 ext_rays++;
                 const f3 org = xyz(p.path[4 * (size_t)pid + 0]), dir = xyz(p.path[4 * (size_t)pid + 1]);
                 const Hit h = traverse<false, COUNT>(p.bvh, org, dir, kEps, kFltMax, st, nn, nt, p.overflow);
                 hit = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
             }
         }
-        restart(p, k, x ^ 1, active, pid);
+        restart(p, k, x ^ 1, ended, pid);
     }
     wave_count_add(&p.traversal[2], ext_rays);
     wave_count_add(&p.traversal[3], sh_rays);
