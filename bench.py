@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg (p50/p99 latency at equal RMSE)")
     ap.add_argument("--no-viewer", action="store_true", help="skip the viewer leg (1-spp frames with readback)")
+    ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser leg (KPCNN time per 1080p frame)")
+    ap.add_argument("--profile-denoiser", action="store_true",
+                    help="only the denoiser leg: 4-spp renders denoised at the bench resolution (for rocprofv3)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
@@ -185,6 +188,33 @@ def viewer(ctx, args, scene, frames=32):
     return out
 
 
+DN_FLOP_PER_PIXEL = 75_456 + 150   # five 3x3 convolutions (2*9*Cin*32 each) + the 5x5 kernel application
+MFMA_BF16_PEAK_TFLOPS = 2500.0     # MI355X dense BF16
+
+
+def denoiser(ctx, args, scene, frames=8):
+    """The KPCNN on the bench frame: 4-spp renders with denoise = 1; the
+    render statistics time the five BF16 MFMA convolutions and the kernel
+    application with HIP events (denoise_ms)."""
+    from optixrender_amd import denoise
+
+    W, H = args.width, args.height
+    ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
+    ctx.set_camera(scene.camera)
+    ms = []
+    for i in range(frames + 1):
+        st = ctx.render(spp=4, max_depth=args.depth, denoise=True)
+        if i > 0:
+            ms.append(st.denoise_ms)
+    avg = float(np.mean(ms))
+    tflops = DN_FLOP_PER_PIXEL * W * H / (avg / 1e3) / 1e12
+    out = {"frames": frames, "pixels": W * H, "ms_per_frame": round(avg, 4), "flop_per_pixel": DN_FLOP_PER_PIXEL,
+           "achieved_tflops": round(tflops, 1), "peak_tflops_bf16": MFMA_BF16_PEAK_TFLOPS,
+           "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)}
+    print(f"[bench] denoiser: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -219,6 +249,11 @@ def main():
         t, ms = ctx.autotune(spp=4, max_depth=depth, rr_start=0)
         ctx.set_camera(scene.camera)
         print(f"[bench] autotune: {t.as_dict()} ({ms:.2f} ms per 4-spp probe)", file=sys.stderr, flush=True)
+
+    if args.profile_denoiser:
+        print(json.dumps({"denoiser": denoiser(ctx, args, scene)}), flush=True)
+        ctx.close()
+        return
 
     def step(flags=abi.OXR_FLAG_STAGE_TIMING):
         # stage timing: kernels launched one by one with HIP events between stages
@@ -315,6 +350,8 @@ def main():
                                                   (ms_per_step / 1e3), 0)})
     if not args.no_viewer and not args.profile_steps:
         out["config"]["viewer"] = viewer(ctx, args, scene)
+    if not args.no_denoiser and not args.profile_steps:
+        out["config"]["denoiser"] = denoiser(ctx, args, scene)
     if not args.no_serving and not args.profile_steps:
         out["config"]["serving"] = serving(ctx, args)
     if not args.no_cpu_baseline and not args.profile_steps:
