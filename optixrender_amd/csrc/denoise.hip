@@ -72,7 +72,9 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                                                  float4* __restrict__ out) {
     constexpr int STEPS = CIN / 16;
     constexpr int R = kDnRows;
+    constexpr int SROW = 40;  // staging row of one pixel's 32 outputs, padded (bf16)
     __shared__ float logits[LAST ? 4 : 1][32][33];
+    __shared__ __align__(16) uint16_t stage[LAST ? 1 : 4][32 * SROW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tile = blockIdx.x * 4 + wave;
     const int row_groups = (h + R - 1) / R;
@@ -110,12 +112,14 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
         }
     }
     if (!LAST) {
-        if (!live || x >= w) return;
+        // Each row's outputs go through the wave's LDS staging area so that the
+        // global writes are whole 16-byte pieces of consecutive pixels (one
+        // contiguous 1 KB per store instruction).  Only this wave touches its
+        // area, and a wave's LDS operations complete in order.
+        uint16_t* sw = stage[wave];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int y = y0 + r;
-            if (y >= h) break;
-            uint16_t* dst = out_act + ((int64_t)y * w + x) * 32;
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const int co = 8 * g + 4 * half;
@@ -123,8 +127,17 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                                     ((uint32_t)f2bf(fmaxf(acc[r][4 * g + 1] + bias[co + 1], 0.0f)) << 16);
                 const uint32_t hi = (uint32_t)f2bf(fmaxf(acc[r][4 * g + 2] + bias[co + 2], 0.0f)) |
                                     ((uint32_t)f2bf(fmaxf(acc[r][4 * g + 3] + bias[co + 3], 0.0f)) << 16);
-                *reinterpret_cast<uint2*>(dst + co) = make_uint2(lo, hi);
+                *reinterpret_cast<uint2*>(sw + px * SROW + co) = make_uint2(lo, hi);
             }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int c = lane + 64 * k, p = c >> 2, part = c & 3;  // pixel p of the row, 8 channels
+                const uint4 v = *reinterpret_cast<const uint4*>(sw + p * SROW + 8 * part);
+                if (live && y < h && x0 + p < w)
+                    *reinterpret_cast<uint4*>(out_act + ((int64_t)y * w + x0 + p) * 32 + 8 * part) = v;
+            }
+            __builtin_amdgcn_wave_barrier();  // the next row reuses the area
         }
         return;
     }
