@@ -59,16 +59,23 @@ def parse():
 
 
 def cpu_baseline(scene, args):
-    """The CPU oracle on the same workload: full frames, as many spp as fit ~10 s."""
+    """The CPU oracle on the same workload: full frames, as many spp as fit
+    about 12 s (the per-frame cost and the fixed cost of the oracle's own LBVH
+    build and env cache are measured first, with 1 and 3 spp)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib  # checker only: the baseline leg, never the measured path
 
     threads = min(os.cpu_count() or 1, 16)
     W, H, depth = args.width, args.height, args.depth
-    t0 = time.perf_counter()
-    oracle_lib.render_stats(scene, W, H, 1, max_depth=depth, threads=threads)
-    first = time.perf_counter() - t0
-    spp = int(max(1, min(args.spp, 10.0 // max(first, 1e-3))))
+    def timed(n):
+        t = time.perf_counter()
+        oracle_lib.render_stats(scene, W, H, n, max_depth=depth, threads=threads)
+        return time.perf_counter() - t
+
+    t1, t3 = timed(1), timed(3)
+    per_spp = max((t3 - t1) / 2.0, 1e-3)
+    fixed = max(t1 - per_spp, 0.0)
+    spp = int(max(1, min(args.spp, (12.0 - fixed) / per_spp)))
     t0 = time.perf_counter()
     stats = oracle_lib.render_stats(scene, W, H, spp, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0

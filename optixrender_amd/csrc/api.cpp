@@ -148,13 +148,20 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
-void free_scene(SceneDev* s) {
-    dfree(s->positions); dfree(s->normals); dfree(s->mat_ids); dfree(s->normals_sorted); dfree(s->mat_sorted);
-    dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache); dfree(s->env_pdf);
+// The buffers oxr_scene_build_bvh allocates (freed again before a build that
+// follows a failed one, so a retry after an allocation failure leaks nothing).
+void free_build(SceneDev* s) {
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
     dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
+    dfree(s->normals_sorted); dfree(s->mat_sorted); dfree(s->uvs_sorted);
     dfree(s->cover_boxes); dfree(s->cover_count);
-    dfree(s->texels); dfree(s->tex_info); dfree(s->uvs); dfree(s->uvs_sorted);
+}
+
+void free_scene(SceneDev* s) {
+    free_build(s);
+    dfree(s->positions); dfree(s->normals); dfree(s->mat_ids);
+    dfree(s->mats); dfree(s->lights); dfree(s->media); dfree(s->env); dfree(s->env_cache); dfree(s->env_pdf);
+    dfree(s->texels); dfree(s->tex_info); dfree(s->uvs);
     delete s;
 }
 
@@ -505,6 +512,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
     OXR_HIP(c, hipSetDevice(c->device));
     const int64_t n = s->n;
     if (!s->built) {
+        free_build(s);  // what a failed earlier attempt left
         BvhBuffers& b = s->bvh;
         int64_t ni = n > 1 ? n - 1 : 0;
         int64_t nodes = n > 0 ? 2 * n - 1 : 0;

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                                     ((uint32_t)f2bf(fmaxf(acc[r][4 * g + 3] + bias[co + 3], 0.0f)) << 16);
                 *reinterpret_cast<uint2*>(sw + px * SROW + co) = make_uint2(lo, hi);
             }
-            __builtin_amdgcn_wave_barrier();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no compiler motion of the LDS accesses across
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 const int c = lane + 64 * k, p = c >> 2, part = c & 3;  // pixel p of the row, 8 channels
@@ -137,7 +138,8 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                 if (live && y < h && x0 + p < w)
                     *reinterpret_cast<uint4*>(out_act + ((int64_t)y * w + x0 + p) * 32 + 8 * part) = v;
             }
-            __builtin_amdgcn_wave_barrier();  // the next row reuses the area
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no compiler motion of the LDS accesses across
+        __builtin_amdgcn_wave_barrier();  // the next row reuses the area
         }
         return;
     }
@@ -151,12 +153,17 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                 const int co = 8 * g + 4 * half + i;
                 logits[wave][px][co] = acc[r][4 * g + i] + bias[co];
             }
-        __syncthreads();
-        if (live && half == 0 && x < w && y < h) {
+        // The wave's own logits only: a wave's LDS operations complete in order.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no compiler motion of the LDS accesses across
+        __builtin_amdgcn_wave_barrier();
+        // Both halves of the wave share the pixel's 25 taps (rows 0-2 and 3-4
+        // of the 5x5 kernel); the partial sums meet across the halves.
+        float sum = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+        if (live && x < w && y < h) {
             float m = logits[wave][px][0];
             for (int t = 1; t < 25; t++) m = fmaxf(m, logits[wave][px][t]);
-            float sum = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-            for (int t = 0; t < 25; t++) {
+            const int t0 = half ? 15 : 0, t1 = half ? 25 : 15;
+            for (int t = t0; t < t1; t++) {
                 const float e = expf(logits[wave][px][t] - m);
                 const int sy = min(max(y + t / 5 - 2, 0), h - 1), sx = min(max(x + t % 5 - 2, 0), w - 1);
                 const float4 c = color[(int64_t)sy * w + sx];
@@ -165,9 +172,14 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t* __restrict__ in
                 cg += e * c.y;
                 cb += e * c.z;
             }
-            out[(int64_t)y * w + x] = make_float4(cr / sum, cg / sum, cb / sum, 1.0f);
         }
-        __syncthreads();  // the next row's logits reuse the array
+        sum += __shfl_xor(sum, 32);
+        cr += __shfl_xor(cr, 32);
+        cg += __shfl_xor(cg, 32);
+        cb += __shfl_xor(cb, 32);
+        if (live && half == 0 && x < w && y < h) out[(int64_t)y * w + x] = make_float4(cr / sum, cg / sum, cb / sum, 1.0f);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no compiler motion of the LDS accesses across
+        __builtin_amdgcn_wave_barrier();  // the next row's logits reuse the array
     }
 }
 
