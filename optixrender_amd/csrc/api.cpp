@@ -309,7 +309,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.trace_blocks_per_cu = 8;
     c->tuning.shade_blocks_per_cu = 8;
     c->tuning.stack_lds = 16;
-    c->tuning.iters_per_poll = 4;
+    c->tuning.iters_per_poll = 2;
     c->tuning.pool_paths = 1 << 24;
     c->tuning.window_samples = 1 << 27;
     c->tuning.tail_paths = 1 << 18;

@@ -65,7 +65,7 @@ struct oxr_server {
     size_t budget = 0, wave_bytes = 0, in_use = 0;
     int max_active = 0;
     int64_t tick_paths = 0, cap = 0, pool = 0;
-    int trace_grid = 0, shade_grid = 0, sky_grid = 0, stack_lds = 32, iters_per_poll = 4;
+    int trace_grid = 0, shade_grid = 0, sky_grid = 0, stack_lds = 32, iters_per_poll = 2;
     int64_t tail_paths = 0;
     WaveEngine eng;  // shared path pool; a tick is one sample window
     void* dn_scratch = nullptr;  // denoiser activations for one frame of max_frame_pixels
