@@ -225,8 +225,8 @@ def denoiser(ctx, args, scene, frames=8):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is "
-                                   "design-only, see DESIGN.md)"}))
+        print(json.dumps({"error": "the renderer runs on one GPU only; --gpus must be 1 (sharding a frame "
+                                   "over several GPUs by image tiles is designed, not built)"}))
         sys.exit(2)
     from optixrender_amd import abi, scenes
     from optixrender_amd.renderer import Context
