@@ -289,8 +289,10 @@ typedef struct oxr_tuning {
 int oxr_get_tuning(oxr_ctx* ctx, oxr_tuning* out);
 int oxr_set_tuning(oxr_ctx* ctx, const oxr_tuning* t);
 /* Coordinate-descent search over oxr_tuning: renders `probe` on the selected
-   scene for each candidate value of one knob at a time, keeps the fastest,
-   applies it and returns it with its render time.  Resets frameID. */
+   scene for each candidate value of one knob at a time (LDS stack entries,
+   trace and shade blocks per CU, pool slots, tail_paths, iterations per
+   poll), keeps the fastest, applies it and returns it with its render time.
+   Resets frameID. */
 int oxr_autotune(oxr_ctx* ctx, const oxr_render_params* probe, oxr_tuning* best, double* best_ms);
 
 /* ---- building blocks exposed for parity tests ---- */

@@ -921,8 +921,9 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
     const std::vector<int> stacks = {8, 16, 32}, tracebpc = {4, 8, 12, 16}, shadebpc = {2, 4, 8, 16};
     const std::vector<int64_t> pools = {1 << 21, 1 << 22, 1 << 23, 1 << 24};
     const std::vector<int64_t> tails = {0, 1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 19};
-    const size_t counts[5] = {stacks.size(), tracebpc.size(), shadebpc.size(), pools.size(), tails.size()};
-    for (int knob = 0; knob < 5; knob++) {
+    const std::vector<int> polls = {2, 4, 8};
+    const size_t counts[6] = {stacks.size(), tracebpc.size(), shadebpc.size(), pools.size(), tails.size(), polls.size()};
+    for (int knob = 0; knob < 6; knob++) {
         const size_t nc = counts[knob];
         for (size_t i = 0; i < nc; i++) {
             oxr_tuning t = cur;
@@ -931,6 +932,7 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
             if (knob == 2) t.shade_blocks_per_cu = shadebpc[i];
             if (knob == 3) t.pool_paths = pools[i];
             if (knob == 4) t.tail_paths = tails[i];
+            if (knob == 5) t.iters_per_poll = polls[i];
             if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0.0;
             rc = time_it(t, ms);

@@ -145,6 +145,7 @@ def test_autotune_picks_a_valid_tuning(ctx, cornell):
         ctx.set_camera(sc.camera)
         t, ms = ctx.autotune(spp=2, max_depth=8, rr_start=3)
         assert t.stack_lds in (8, 16, 32) and 1 <= t.trace_blocks_per_cu <= 16 and ms > 0
+        assert t.iters_per_poll in (2, 4, 8)
         assert ctx.get_tuning().as_dict() == t.as_dict()
     finally:
         ctx.set_tuning(base)
