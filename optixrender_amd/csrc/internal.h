@@ -86,7 +86,7 @@ struct WaveEngine {
 struct WindowRun {
     int trace_grid = 0, shade_grid = 0;
     int sky_grid = 0;  // the sky shading beside the path loop
-    int iters_per_poll = 4;
+    int iters_per_poll = 2;
     bool count = false, eager = false, stage_timing = false;
     bool set_req0 = false;  // write frame0/offset/frames into request 0 on the device
     int frame0 = 0, path_frame_offset = 0, frames = 0;
