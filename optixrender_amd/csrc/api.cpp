@@ -498,7 +498,10 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
                     std::string("scene upload: ") + hipGetErrorString(e));
     }
     uint32_t id = c->next_id++;
-    c->scenes[id] = s;
+    {
+        std::lock_guard<std::mutex> lk(c->scenes_mu);
+        c->scenes[id] = s;
+    }
     if (c->cur == 0) c->cur = id;
     *scene_id = id;
     return OXR_OK;
@@ -573,6 +576,9 @@ int oxr_scene_release(oxr_ctx* c, uint32_t id) {
     if (!c) return OXR_ERR_INVALID;
     auto it = c->scenes.find(id);
     if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    std::lock_guard<std::mutex> lk(c->scenes_mu);
+    if (it->second->server_refs.load() > 0)
+        return fail(c, OXR_ERR_STATE, "scene is in use by a render server's requests");
     (void)hipStreamSynchronize(c->stream);
     free_scene(it->second);
     c->scenes.erase(it);

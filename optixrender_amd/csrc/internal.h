@@ -3,7 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,6 +45,9 @@ struct SceneDev {
     int* cover_count = nullptr;
     bool built = false;
     float build_ms = 0.0f;
+    // server requests (submitted and not yet let go by the server's worker)
+    // that render this scene: oxr_scene_release refuses the scene meanwhile
+    std::atomic<int> server_refs{0};
 };
 
 
@@ -113,6 +118,7 @@ struct oxr_ctx {
     int num_cu = 256;
     std::string err;
     std::map<uint32_t, oxr::SceneDev*> scenes;
+    std::mutex scenes_mu;  // guards `scenes` against a server's submit on another thread
     uint32_t next_id = 1;
     uint32_t cur = 0;
     float* lut_bsdf = nullptr;

@@ -48,6 +48,12 @@ struct Request {
     uint8_t* cover = nullptr;  // camera-ray cover mask (one byte per 8x8-pixel tile)
     Clock::time_point submitted, finished;
     std::string error;
+    // The worker holds a request from submit until it has dropped it from the
+    // pending queue, the active list and any tick (done, failed or cancelled
+    // and dropped).  A request released by the caller while still held is
+    // only marked; the worker frees it when it lets go.
+    bool held = true;
+    bool released = false;
 };
 
 // colour, albedo, normal (float4), depth (float), RGBA8, the denoised colour,
@@ -109,6 +115,26 @@ void free_request_buffers(oxr_server* s, Request* q) {
     q->bytes = 0;
 }
 
+// The worker lets go of a request (under the lock): it is in no list and no
+// tick any more.  Its hold on the scene ends, and a request the caller has
+// already released is freed now.
+void let_go(oxr_server* s, Request* q) {
+    if (!q->held) return;
+    q->held = false;
+    q->scene->server_refs.fetch_sub(1);
+    if (q->released) {
+        free_request_buffers(s, q);
+        s->all.erase(q->id);
+        delete q;
+    }
+}
+
+// A request the caller may still name (not released).
+Request* find_request(oxr_server* s, uint64_t id) {
+    auto it = s->all.find(id);
+    return it == s->all.end() || it->second->released ? nullptr : it->second;
+}
+
 bool same_group(const Request* a, const Request* b) {
     return a->scene == b->scene && a->r.max_depth == b->r.max_depth && a->r.rr_start == b->r.rr_start &&
            a->r.camera.medium == b->r.camera.medium;
@@ -118,7 +144,7 @@ bool same_group(const Request* a, const Request* b) {
 void admit(oxr_server* s) {
     while (!s->pending.empty() && (int)s->active.size() < s->max_active) {
         Request* q = s->pending.front();
-        if (q->state == kCancelled) { s->pending.pop_front(); continue; }
+        if (q->state == kCancelled) { s->pending.pop_front(); let_go(s, q); continue; }
         size_t need = request_bytes(q->r);
         if (s->in_use + need + s->wave_bytes > s->budget) break;
         hipError_t e;
@@ -137,6 +163,7 @@ void admit(oxr_server* s) {
             q->error = std::string("allocation: ") + hipGetErrorString(e);
             s->pending.pop_front();
             s->done_cv.notify_all();
+            let_go(s, q);
             continue;
         }
         q->bytes = need;
@@ -225,8 +252,10 @@ void worker_loop(oxr_server* s) {
         // drop cancelled active requests
         for (auto it = s->active.begin(); it != s->active.end();) {
             if ((*it)->state == kCancelled) {
-                free_request_buffers(s, *it);
+                Request* q = *it;
+                free_request_buffers(s, q);
                 it = s->active.erase(it);
+                let_go(s, q);
             } else {
                 ++it;
             }
@@ -269,7 +298,10 @@ void worker_loop(oxr_server* s) {
         }
         if (total > s->cap || group.empty()) {  // cannot happen: submit bounds a frame by cap - tick_paths
             for (Request* q : group) { q->state = kFailed; q->error = "tick exceeds wavefront capacity"; }
-            for (Request* q : group) s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+            for (Request* q : group) {
+                s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+                let_go(s, q);
+            }
             s->done_cv.notify_all();
             continue;
         }
@@ -294,6 +326,7 @@ void worker_loop(oxr_server* s) {
                 s->latencies.push_back(std::chrono::duration<double, std::milli>(q->finished - q->submitted).count());
             }
             s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+            let_go(s, q);
         }
         s->done_cv.notify_all();
     }
@@ -357,6 +390,7 @@ int oxr_server_destroy(oxr_server* s) {
     (void)hipSetDevice(s->ctx->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& kv : s->all) {
+        if (kv.second->held) kv.second->scene->server_refs.fetch_sub(1);
         free_request_buffers(s, kv.second);
         delete kv.second;
     }
@@ -374,6 +408,7 @@ const char* oxr_server_last_error(const oxr_server* s) { return s ? s->err.c_str
 int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id) {
     if (!s || !r || !id) return OXR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
+    std::lock_guard<std::mutex> slk(s->ctx->scenes_mu);  // the context thread may upload or release scenes
     auto it = s->ctx->scenes.find(r->scene_id);
     if (it == s->ctx->scenes.end() || !it->second->built) {
         s->err = "submit: unknown or unbuilt scene";
@@ -400,6 +435,7 @@ int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id) {
     q->id = s->next_id++;
     q->r = *r;
     q->scene = it->second;
+    q->scene->server_refs.fetch_add(1);  // oxr_scene_release refuses the scene until the worker lets go
     q->submitted = Clock::now();
     s->pending.push_back(q);
     s->all[q->id] = q;
@@ -412,18 +448,19 @@ int oxr_submit(oxr_server* s, const oxr_request* r, uint64_t* id) {
 int oxr_poll(oxr_server* s, uint64_t id, int* state) {
     if (!s || !state) return OXR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return OXR_ERR_INVALID;
-    *state = it->second->state;
+    Request* q = find_request(s, id);
+    if (!q) return OXR_ERR_INVALID;
+    *state = q->state;
     return OXR_OK;
 }
 
 int oxr_wait(oxr_server* s, uint64_t id, int timeout_ms) {
     if (!s) return OXR_ERR_INVALID;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return OXR_ERR_INVALID;
-    Request* q = it->second;
+    Request* q = find_request(s, id);
+    if (!q) return OXR_ERR_INVALID;
+    // q stays allocated while this thread waits: only oxr_release_request,
+    // which a caller does not overlap with a wait on the same id, frees it
     auto pred = [&] { return q->state == kDone || q->state == kCancelled || q->state == kFailed; };
     if (timeout_ms < 0) s->done_cv.wait(lk, pred);
     else if (!s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) return OXR_ERR_STATE;
@@ -437,9 +474,8 @@ int oxr_wait(oxr_server* s, uint64_t id, int timeout_ms) {
 int oxr_cancel(oxr_server* s, uint64_t id) {
     if (!s) return OXR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return OXR_ERR_INVALID;
-    Request* q = it->second;
+    Request* q = find_request(s, id);
+    if (!q) return OXR_ERR_INVALID;
     if (q->state == kPending || q->state == kActive) {
         q->state = kCancelled;
         s->cancelled++;
@@ -450,9 +486,8 @@ int oxr_cancel(oxr_server* s, uint64_t id) {
 
 static int copy_result(oxr_server* s, uint64_t id, void* out, size_t bytes_per_px, size_t count, int which) {
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end() || !out) return OXR_ERR_INVALID;
-    Request* q = it->second;
+    Request* q = find_request(s, id);
+    if (!q || !out) return OXR_ERR_INVALID;
     if (q->state != kDone) return OXR_ERR_STATE;
     if (count != (size_t)q->r.width * q->r.height) return OXR_ERR_INVALID;
     const void* src = which == 0 ? (const void*)q->rgba : (q->r.denoise ? (const void*)q->denoised : (const void*)q->color);
@@ -470,13 +505,19 @@ int oxr_result_color(oxr_server* s, uint64_t id, float* out, size_t count) {
 int oxr_release_request(oxr_server* s, uint64_t id) {
     if (!s) return OXR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return OXR_ERR_INVALID;
-    Request* q = it->second;
+    Request* q = find_request(s, id);
+    if (!q) return OXR_ERR_INVALID;
     if (q->state == kPending || q->state == kActive) return OXR_ERR_STATE;
+    if (q->held) {
+        // cancelled, but still in the pending queue, the active list or the
+        // tick that is running: the worker frees it when it drops it
+        q->released = true;
+        s->cv.notify_all();
+        return OXR_OK;
+    }
     free_request_buffers(s, q);
+    s->all.erase(q->id);
     delete q;
-    s->all.erase(it);
     s->cv.notify_all();
     return OXR_OK;
 }

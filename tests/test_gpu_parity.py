@@ -12,13 +12,11 @@ import numpy as np
 import pytest
 
 import oracle_lib
+from parity import PASS_FRACTION, REL_TOL, _render_pair, per_path_check
 from optixrender_amd import abi, scenes
 from optixrender_amd.renderer import SceneData
 
 pytestmark = pytest.mark.gpu
-
-REL_TOL = 1e-4
-PASS_FRACTION = 0.999
 
 
 def soup_scene(pos, n_mats=1):
@@ -200,63 +198,16 @@ def test_material_eval_and_sample(ctx, name):
     assert ok_s / inp.shape[0] >= PASS_FRACTION, ok_s
 
 
-def per_path_check(gpu, ref, label):
-    a = gpu.reshape(-1, 3).astype(np.float64)
-    b = ref.reshape(-1, 3).astype(np.float64)
-    nan_a = np.isnan(a).any(1)
-    nan_b = np.isnan(b).any(1)
-    ok = np.all(np.abs(a - b) <= REL_TOL * np.maximum(np.abs(a), np.abs(b)) + 1e-7, axis=1)
-    ok = np.where(nan_a | nan_b, nan_a == nan_b, ok)
-    frac = ok.mean()
-    print(f"[{label}] paths={ok.size} within 1e-4: {frac:.6f}  diverged={int((~ok).sum())}", flush=True)
-    return frac
-
-
-def _render_pair(ctx, scene, w, h, spp, depth, rr, pool=None, tail=None, normals=None, filt=abi.FILTER_GAUSSIAN):
-    """GPU render and oracle render of the same scene.  pool: path slots of
-    the GPU pool (None = default); a pool smaller than spp*w*h restarts
-    every slot many times.  tail: the tail-mode threshold (None = default;
-    0 keeps every path in the per-material kernels).  normals: a list that
-    receives the (GPU, oracle) normal AOVs.  filt: the pixel filter."""
-    sid = ctx.upload(scene)
-    base = ctx.get_tuning()
-    try:
-        ctx.build_bvh(sid)
-        ctx.select(sid)
-        ctx.resize(w, h)
-        ctx.set_camera(scene.camera)
-        if pool is not None or tail is not None:
-            t = ctx.get_tuning()
-            if pool is not None:
-                t.pool_paths = pool
-            if tail is not None:
-                t.tail_paths = tail
-            ctx.set_tuning(t)
-        ctx.render(spp=spp, max_depth=depth, rr_start=rr, filter=filt, flags=abi.OXR_FLAG_KEEP_PATH_RADIANCE)
-        paths = ctx.download_path_radiance(spp, w, h)
-        color = ctx.download_aov(0, w, h)
-        albedo = ctx.download_aov(1, w, h)
-        normal = ctx.download_aov(2, w, h)
-        dz = ctx.download_depth(w, h)
-    finally:
-        ctx.set_tuning(base)
-        ctx.release(sid)
-    rc, ra, rn, rd, rp = oracle_lib.render(scene, w, h, spp, max_depth=depth, rr_start=rr, filt=filt,
-                                           keep_paths=True, with_depth=True)
-    # first-hit depth AOV: same traversal, same distance arithmetic
-    close = np.abs(dz - rd) <= 1e-6 * np.abs(rd) + 1e-7
-    assert close.mean() >= PASS_FRACTION, f"depth AOV differs on {np.count_nonzero(~close)} pixels"
-    if normals is not None:
-        normals.extend([normal, rn])
-    return paths, rp, color, rc, albedo, ra
-
-
-@pytest.mark.parametrize("pool", [None, 2048])
-def test_render_cornell_per_path(ctx, pool):
+@pytest.mark.parametrize("pool,filt", [(None, abi.FILTER_GAUSSIAN), (2048, abi.FILTER_GAUSSIAN),
+                                       (None, abi.FILTER_BOX), (None, abi.FILTER_TENT),
+                                       (None, abi.FILTER_TRIANGLE)])
+def test_render_cornell_per_path(ctx, pool, filt):
+    # every pixel filter (Filter.h:5-40: box, tent, triangle, Gaussian) jitters
+    # the camera rays as the oracle does
     sc = scenes.cornell_spheres()
-    print(f"[render] cornell 64x64x4 pool={pool}", flush=True)
-    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3, pool)
-    assert per_path_check(paths, rp, "cornell") >= PASS_FRACTION
+    print(f"[render] cornell 64x64x4 pool={pool} filter={filt}", flush=True)
+    paths, rp, color, rc, albedo, ra = _render_pair(ctx, sc, 64, 64, 4, 8, 3, pool, filt=filt)
+    assert per_path_check(paths, rp, f"cornell filter {filt}") >= PASS_FRACTION
     assert np.nanmax(np.abs(albedo - ra)) < 1e-5
 
 

@@ -116,10 +116,11 @@ def test_denoised_request_equals_denoised_render(ctx, cornell):
 
 
 
-def test_mixed_scenes_in_one_tick(ctx, cornell):
-    # requests over different resident scenes share ticks: a textured scene,
-    # a lit sphere field (sky pass, lights in view) and the Cornell box; each
-    # result equals its single render
+def test_requests_over_several_scenes_equal_single_renders(ctx, cornell):
+    # requests over different resident scenes in one server: a textured
+    # scene, a lit sphere field (sky pass, lights in view) and the Cornell
+    # box.  A tick serves one scene's requests; each result equals its
+    # single render
     _, cid = cornell
     tex = scenes.textured_cornell(subdiv=2)
     lit = scenes.lit_sphere_field(subdiv=1)
@@ -144,3 +145,55 @@ def test_mixed_scenes_in_one_tick(ctx, cornell):
     finally:
         ctx.release(tid)
         ctx.release(lid)
+
+
+def test_cancel_and_release_during_a_tick(ctx, cornell):
+    # A request cancelled and released while its tick is running (and one
+    # still pending) is only forgotten by the caller's side; the worker frees
+    # it when it drops it.  The server keeps serving afterwards, and the
+    # scene cannot be released while a request of it is live.
+    sc, sid = cornell
+    cam = abi.camera((0, 1, 3), (0, 1, 0))
+    srv = Server(ctx, budget_bytes=1 << 31, max_active=1, tick_paths=1 << 22, max_frame_pixels=256 * 256)
+    try:
+        a = srv.submit(sid, 256, 256, 64, cam, max_depth=8)     # 4 M paths: one long tick
+        b = srv.submit(sid, 64, 64, 8, cam, max_depth=8)        # pending behind a (max_active 1)
+        with pytest.raises(abi.OxrError):
+            ctx.release(sid)                                     # live requests hold the scene
+        for _ in range(2000):
+            if srv.poll(a) == abi.REQ_ACTIVE:
+                break
+        srv.cancel(a)
+        srv.cancel(b)
+        srv.release(a)
+        srv.release(b)
+        with pytest.raises(abi.OxrError):
+            srv.poll(a)                                          # forgotten at once
+        with pytest.raises(abi.OxrError):
+            srv.poll(b)
+        c = srv.submit(sid, 40, 32, 3, cam, max_depth=8)
+        assert srv.wait(c, 60000) == 0
+        col = srv.result_color(c)
+        st = srv.stats()
+        assert st.cancelled == 2 and st.completed == 1
+        srv.release(c)
+    finally:
+        srv.close()
+    ref, _ = single(ctx, sid, 40, 32, 3, cam)
+    assert np.array_equal(col.view(np.uint32), ref.view(np.uint32))
+
+
+def test_scene_release_waits_for_server_requests(ctx):
+    sc = scenes.cornell_spheres(subdiv=1)
+    sid = ctx.upload(sc)
+    ctx.build_bvh(sid)
+    srv = Server(ctx, budget_bytes=1 << 30, max_active=2, tick_paths=4096, max_frame_pixels=64 * 64)
+    try:
+        rid = srv.submit(sid, 48, 48, 4, sc.camera, max_depth=8)
+        assert srv.wait(rid, 60000) == 0
+        # once done, the worker has let go of the request: the scene can go,
+        # and the finished image stays readable
+        ctx.release(sid)
+        assert srv.result_rgba8(rid).shape == (48, 48)
+    finally:
+        srv.close()
