@@ -127,7 +127,7 @@ Renderer::Renderer(const Scene* s) : scene(s) {
     d.lights = lights.data();
     d.n_media = (int32_t)media.size();
     d.media = media.data();
-    if (scene->env) {
+    if (scene->env && scene->env->width > 0) {  // an unreadable HDR leaves no environment
         d.env_w = scene->env->width;
         d.env_h = scene->env->height;
         d.env_rgba = scene->env->hdr.data();

@@ -13,6 +13,18 @@
 #include <sstream>
 #include <stdexcept>
 
+// Inside the reference's tree (common/ on the include path, CMakeLists.txt:37,
+// 64-68) the facade decodes images with the reference's own vendored
+// stb_image, exactly as its Texture and HdrTexture do (Scene.cpp:181-206):
+// PNG, JPG, BMP, PNM and Radiance HDR, rows flipped on load.  Built on its
+// own, it reads binary PPM, uncompressed BMP, Radiance HDR and PFM with the
+// readers below.
+#if __has_include("3rdParty/stb_image.h")
+#define STB_IMAGE_IMPLEMENTATION
+#include "3rdParty/stb_image.h"
+#define OXR_FACADE_STB 1
+#endif
+
 namespace {
 
 struct ObjIndex { int v = -1, t = -1, n = -1; };
@@ -147,6 +159,19 @@ void read_bmp(const std::string& fileName, std::ifstream& in, std::vector<uint8_
 }  // namespace
 
 Texture::Texture(const std::string& fileName) {
+#ifdef OXR_FACADE_STB
+    {
+        int w = 0, h = 0, comp_in = 0;
+        stbi_set_flip_vertically_on_load(true);
+        uint8_t* image = stbi_load(fileName.c_str(), &w, &h, &comp_in, STBI_rgb_alpha);
+        if (!image)
+            throw std::runtime_error("Could not read texture " + fileName + ": " + stbi_failure_reason());
+        pixel.assign(image, image + (size_t)w * h * 4);
+        stbi_image_free(image);
+        resolution = vec2i(w, h);
+        return;
+    }
+#endif
     std::ifstream in(fileName, std::ios::binary);
     if (!in) throw std::runtime_error("Could not read texture " + fileName);
     if (in.peek() == 'B') {
@@ -222,7 +247,32 @@ HdrTexture::HdrTexture(int w, int h, std::vector<float> rgba) : hdr(std::move(rg
 
 HdrTexture::HdrTexture(const std::string& fileName) {
     std::ifstream in(fileName, std::ios::binary);
-    if (!in) throw std::runtime_error("Could not read HDR " + fileName);
+    if (!in) {
+        // The reference attaches an environment without data when the file
+        // cannot be read (Scene.cpp:197-206); here that is no environment
+        // (misses are black) and a warning, so main.cpp, whose HDR is not in
+        // the reference's models/ folder, still runs.
+        std::fprintf(stderr, "warning: could not read HDR %s: no environment\n", fileName.c_str());
+        return;
+    }
+#ifdef OXR_FACADE_STB
+    {
+        in.close();
+        int w = 0, h = 0, comp_in = 0;
+        stbi_set_flip_vertically_on_load(true);
+        float* data = stbi_loadf(fileName.c_str(), &w, &h, &comp_in, STBI_rgb_alpha);
+        if (!data) {
+            std::fprintf(stderr, "warning: could not decode HDR %s (%s): no environment\n", fileName.c_str(),
+                         stbi_failure_reason());
+            return;
+        }
+        hdr.assign(data, data + (size_t)w * h * 4);
+        stbi_image_free(data);
+        width = w;
+        height = h;
+        return;
+    }
+#endif
     std::string first;
     std::getline(in, first);
     if (first == "PF") {  // PFM, bottom-to-top rows already

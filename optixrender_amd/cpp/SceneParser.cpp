@@ -139,26 +139,47 @@ vec3f rotate_xyz(vec3f p, vec3f deg) {
 
 }  // namespace
 
-void SceneParser::LoadFromJson(const std::string& fn, Scene& scene, bool& is_succeed) {
-    is_succeed = false;
+namespace {
+
+// The file's JSON, or false when it cannot be opened (SceneParser.h:31-35:
+// a missing file leaves is_succeed false).
+bool read_json(const std::string& fn, oxr_json::Value& out, std::string& baseDir) {
     std::ifstream in(fn);
-    if (!in.is_open()) return;  // SceneParser.h:31-35: missing file -> is_succeed = false
+    if (!in.is_open()) return false;
     std::stringstream buf;
     buf << in.rdbuf();
-    oxr_json::Value data = oxr_json::parse(buf.str());
+    out = oxr_json::parse(buf.str());
     size_t slash = fn.find_last_of('/');
     baseDir = slash == std::string::npos ? std::string() : fn.substr(0, slash + 1);
-    is_succeed = true;
-    Parse(data, scene);
+    return true;
 }
 
-void SceneParser::Parse(const oxr_json::Value& data, Scene& scene) {
+}  // namespace
+
+void SceneParser::LoadFromJson(const std::string& fn, Scene& scene, bool& is_succeed) {
+    (void)scene;  // the reference's parser reads settings only (SceneParser.cpp:11-23)
+    oxr_json::Value data;
+    is_succeed = read_json(fn, data, baseDir);
+    if (is_succeed) ParseSettings(data);
+}
+
+void SceneParser::LoadSceneFromJson(const std::string& fn, Scene& scene, bool& is_succeed) {
+    oxr_json::Value data;
+    is_succeed = read_json(fn, data, baseDir);
+    if (is_succeed) Parse(data, scene);
+}
+
+void SceneParser::ParseSettings(const oxr_json::Value& data) {
     width = (int)data.number("width", 1024);
     height = (int)data.number("height", 1024);
     depth = (int)data.number("depth", 4);
     const oxr_json::Value* f = data.get("filter");
     oxr_json::Value empty;
     ParseFilter(f ? *f : empty);
+}
+
+void SceneParser::Parse(const oxr_json::Value& data, Scene& scene) {
+    ParseSettings(data);
     // media first (materials' shapes and lights name them), then materials,
     // shapes (the bounds a default look_at needs), lights and the camera
     ParseMedia(data, scene);

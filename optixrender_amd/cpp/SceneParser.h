@@ -1,7 +1,14 @@
 // Scene-from-JSON (reference: core/include/SceneParser.h, core/src/SceneParser.cpp,
-// schema of models/test.json:1-128).  LoadFromJson(fn, scene, is_succeed)
-// reads the reference's settings (width, height, depth, filter) and, unlike
-// the reference, which stops there (SceneParser.cpp:11-23), the whole scene:
+// schema of models/test.json:1-128).
+//
+// LoadFromJson(fn, scene, is_succeed) is the reference's call: it reads the
+// settings (width, height, depth, filter) and leaves the scene alone
+// (SceneParser.cpp:11-23), so main.cpp, which builds its scene in code and
+// then reads models/test.json for the window size (main.cpp:233-235),
+// renders the same scene as with the reference.
+//
+// LoadSceneFromJson(fn, scene, is_succeed) reads the settings and the whole
+// scene the file describes:
 //
 //   camera    look_from, look_at (default: the centre of the shapes' bounds),
 //             up (default +y), medium (a medium name)
@@ -69,6 +76,8 @@ class SceneParser {
 public:
     SceneParser() = default;
     void LoadFromJson(const std::string& fn, Scene& scene, bool& is_succeed);
+    void LoadSceneFromJson(const std::string& fn, Scene& scene, bool& is_succeed);
+    void ParseSettings(const oxr_json::Value& data);
     void Parse(const oxr_json::Value& data, Scene& scene);
     void ParseFilter(const oxr_json::Value& data);
     void ParseMedia(const oxr_json::Value& data, Scene& scene);

@@ -8,13 +8,19 @@
 // included.  Otherwise, without OBJ arguments the scene is a procedural
 // Cornell box.  OXR_HEADLESS_DUMP=<file> writes the parsed scene as JSON
 // (meshes after their transforms, materials, lights, media, environment,
-// camera) and exits before any device call.  With
+// camera) and exits before any device call; with OXR_HEADLESS_DUMP_BRIEF=1
+// each mesh lists its triangle count and bounds instead of its vertices, and
+// each texture its size and byte sum instead of its texels.  With
 // OXR_HEADLESS_DENOISE=1 in the environment the frames are denoised
 // (denoiserOn, the viewer's 'D' key); OXR_HEADLESS_PRESENT=1 reads the frame
 // back through the streamed, page-locked path (presentOn).  OBJ meshes take their textures from
 // OXR_HEADLESS_ALBEDO / _ROUGHNESS / _METALLIC / _NORMAL (PPM files), and
 // OXR_HEADLESS_ENV names an environment map (.hdr or .pfm), as main.cpp
 // gives its meshes texture files and its scene an HDR (main.cpp:165-213).
+// OXR_HEADLESS_MAIN_SCENE=<models folder> builds main.cpp's own scene from
+// that folder with main.cpp's calls in main.cpp's order (head, body, base and
+// plane meshes with their texture files, the HDR, then test.json's settings
+// through LoadFromJson), the settings argument being ignored.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -60,7 +66,44 @@ static void cornell(Scene& scene) {
     scene.AddLight(l);
 }
 
-static void dump_scene(const std::string& file, const Scene& scene, const SceneParser& parser, const Camera& cam) {
+// main.cpp:165-246 with the models folder made explicit.
+static void main_scene(const std::string& models, Scene& scene, Camera& camera, SceneParser& parser, bool& ok) {
+    TextureFile textureFile;
+    Material material;
+    Medium m;
+    textureFile.albedoFile = models + "/01_Head_Base_Color.png";
+    textureFile.roughnessFile = models + "/01_Head_Roughness.png";
+    textureFile.metallicFile = models + "/01_Head_Metallic.png";
+    textureFile.normalFile = models + "/01_Head_Normal_DirectX.png";
+    scene.AddMedium(m);
+    scene.AddMesh(models + "/head.obj", material, textureFile, -1, -1);
+    textureFile.albedoFile = models + "/02_Body_Base_Color.png";
+    textureFile.roughnessFile = models + "/02_Body_Roughness.png";
+    textureFile.metallicFile = models + "/02_Body_Metallic.png";
+    textureFile.normalFile = models + "/02_Body_Normal_DirectX.png";
+    Material body;
+    body.type = MaterialType::Dielectric;
+    TextureFile none;
+    scene.AddMesh(models + "/body.obj", body, none, -1, 0);
+    textureFile.albedoFile = models + "/03_Base_Base_Color.png";
+    textureFile.metallicFile = models + "/03_Base_Metallic.png";
+    textureFile.roughnessFile = models + "/03_Base_Roughness.png";
+    textureFile.normalFile = models + "/03_Base_Normal_DirectX.png";
+    scene.AddMesh(models + "/base.obj", material, textureFile, -1, -1);
+    material = Material();
+    material.type = MaterialType::Diffuse;
+    textureFile.albedoFile = models + "/grid.jpg";
+    textureFile.roughnessFile = models + "/rusty_metal_sheet_rough_1k.png";
+    textureFile.normalFile = "";
+    scene.AddMesh(models + "/plane.obj", material, textureFile, -1, -1);
+    scene.AddEnv(models + "/spaichingen_hill_4k.hdr");
+    camera = Camera{vec3f(0.2f, 0.2f, 0.2f), scene.bounds.center(), vec3f(0.f, 1.f, 0.f)};
+    camera.medium = -1;
+    parser.LoadFromJson(models + "/test.json", scene, ok);
+}
+
+static void dump_scene(const std::string& file, const Scene& scene, const SceneParser& parser, const Camera& cam,
+                       bool brief) {
     std::ofstream o(file);
     if (!o) throw std::runtime_error("cannot write " + file);
     o.precision(9);
@@ -83,7 +126,14 @@ static void dump_scene(const std::string& file, const Scene& scene, const SceneP
           << ",\"tex\":[" << a.albedoTextureID << ","
           << a.roughnessTextureID << "," << a.anisotropyTextureID << "," << a.specularTextureID << ","
           << a.metallicTextureID << "," << a.coat_roughness_uTextureID << "," << a.coat_roughness_vTextureID << ","
-          << a.normalTextureID << "]},\"vertex\":[";
+          << a.normalTextureID << "]},\"triangles\":" << me->index.size();
+        if (brief) {
+            box3f b;
+            for (const vec3f& v : me->vertex) b.extend(v);
+            o << ",\"lower\":"; v3(b.lower); o << ",\"upper\":"; v3(b.upper); o << "}";
+            continue;
+        }
+        o << ",\"vertex\":[";
         for (size_t i = 0; i < me->vertex.size(); i++) { o << (i ? "," : ""); v3(me->vertex[i]); }
         o << "],\"normal\":[";
         for (size_t i = 0; i < me->normal.size(); i++) { o << (i ? "," : ""); v3(me->normal[i]); }
@@ -95,7 +145,14 @@ static void dump_scene(const std::string& file, const Scene& scene, const SceneP
     o << "],\"textures\":[";
     for (size_t t = 0; t < scene.textures.size(); t++) {
         const Texture* tx = scene.textures[t];
-        o << (t ? "," : "") << "{\"width\":" << tx->resolution.x << ",\"height\":" << tx->resolution.y << ",\"rgba\":[";
+        o << (t ? "," : "") << "{\"width\":" << tx->resolution.x << ",\"height\":" << tx->resolution.y;
+        if (brief) {
+            uint64_t sum = 0;
+            for (uint8_t b : tx->pixel) sum += b;
+            o << ",\"bytes\":" << tx->pixel.size() << ",\"byte_sum\":" << sum << "}";
+            continue;
+        }
+        o << ",\"rgba\":[";
         for (size_t i = 0; i < tx->pixel.size(); i++) o << (i ? "," : "") << (int)tx->pixel[i];
         o << "]}";
     }
@@ -154,9 +211,18 @@ int main(int ac, char** av) {
         camera.medium = -1;
         SceneParser parser;
         bool ok = false;
-        if (!settings.empty()) {
+        if (!env_str("OXR_HEADLESS_MAIN_SCENE").empty()) {
+            Scene own;
+            main_scene(env_str("OXR_HEADLESS_MAIN_SCENE"), own, camera, parser, ok);
+            scene.meshes.swap(own.meshes);
+            scene.textures.swap(own.textures);
+            scene.lights.swap(own.lights);
+            scene.mediums.swap(own.mediums);
+            std::swap(scene.env, own.env);
+            scene.bounds = own.bounds;
+        } else if (!settings.empty()) {
             Scene parsed;
-            parser.LoadFromJson(settings, parsed, ok);
+            parser.LoadSceneFromJson(settings, parsed, ok);
             if (ok && !parsed.meshes.empty()) {  // a whole scene: it replaces the default one
                 scene.meshes.swap(parsed.meshes);
                 scene.textures.swap(parsed.textures);
@@ -171,7 +237,7 @@ int main(int ac, char** av) {
         if (!ok) { parser.width = 256; parser.height = 256; }
         const std::string dump = env_str("OXR_HEADLESS_DUMP");
         if (!dump.empty()) {
-            dump_scene(dump, scene, parser, camera);
+            dump_scene(dump, scene, parser, camera, env_str("OXR_HEADLESS_DUMP_BRIEF") == "1");
             return 0;
         }
         Renderer renderer(&scene);

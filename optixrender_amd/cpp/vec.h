@@ -1,8 +1,18 @@
-// Small vector/box types with the component-wise semantics the reference's
-// host code relies on (the subset of common/gdt used by main.cpp, Scene.h and
-// Renderer.h: vec2i, vec2f, vec3f, vec3i, box3f, dot, cross, normalize,
-// length, box center/span/extend).
+// Vector/box types of the host facade.
+//
+// Built inside the reference's tree (common/gdt on the include path, as its
+// CMakeLists.txt:13-24 sets up for every target), the facade uses gdt's own
+// vec_t / box_t (gdt/math/AffineSpace.h, which the viewer's GLFWindow.h:7
+// includes too), so main.cpp and the viewer see one set of types.  Built on
+// its own, it defines the subset the facade and main.cpp use with the same
+// component-wise semantics: vec2i, vec2f, vec3f, vec3i, box3f, dot, cross,
+// normalize (v * 1/sqrt(dot(v, v)), gdt/math/vec.h:290-315), length, box
+// center/span/extend.
 #pragma once
+#if __has_include("gdt/math/AffineSpace.h")
+#include "gdt/math/AffineSpace.h"
+#define OXR_FACADE_USES_GDT 1
+#else
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -58,3 +68,4 @@ struct box3f {
 };
 
 }  // namespace gdt
+#endif
