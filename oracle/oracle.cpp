@@ -41,13 +41,19 @@
  *                           sphere or quad light equal the closed-form
  *                           irradiance (NEE + MIS-weighted BSDF hits); two
  *                           lights show the reference's light_pdf quirk.
- *   parity unpinned: the Kulla-Conty table values and the exact
- *   AverageFresnelConductor/AverageFresnelDielectric fits (only bounds and
- *   monotonicity are checked), the anisotropic SampleGGX branch, and the
- *   MetalWorkflow / ThinDielectric / ClearCoatedConductor /
- *   DiffuseTransmitter lobes beyond energy bounds and sample/eval
- *   consistency.  No compile of the reference's device headers is possible
- *   here (they need the OptiX SDK and CUDA headers).
+ *   Kulla-Conty table       the 128-point Hammersley estimate of the textbook
+ *                           GGX albedo, converging to an independent float64
+ *                           quadrature (test_oracle_pins.py).
+ *   AverageFresnel fits     numerical 2 int F(mu) mu dmu of the textbook
+ *                           (complex-index) Fresnel equations.
+ *   FresnelConductor        its values at normal and grazing incidence as
+ *                           written (first term + sin^2, not + cos^2).
+ *   SampleGGX               chi-square against D(h) cos(theta_h), both branches.
+ *   remaining lobes         sampled weights equal their closed forms (G1(L),
+ *                           eta^2 G1(L), G1(V), albedo), Fresnel lobe choice,
+ *                           white furnaces; sample/eval consistency.
+ *   No compile of the reference's device headers is possible here (they
+ *   need the OptiX SDK and CUDA headers).
  */
 #include "oracle.h"
 
@@ -1886,6 +1892,27 @@ void orc_brute_closest(int64_t n, const float* pos, int64_t n_rays, const float*
 }
 
 void orc_kulla_conty(float* bsdf_avg, float* albedo) { kulla_conty(bsdf_avg, albedo); }
+
+void orc_sample_ggx(float au, float av, float u0, float u1, float* h3) {
+    V3 h = sample_ggx(au, av, V2{u0, u1});
+    h3[0] = h.x; h3[1] = h.y; h3[2] = h.z;
+}
+void orc_fresnel_conductor(const float* V, const float* H, const float* eta, const float* k, float* out3) {
+    V3 r = fresnel_conductor(load3(V), load3(H), load3(eta), load3(k));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+float orc_fresnel_dielectric(const float* V, const float* H, float eta_inv) {
+    return fresnel_dielectric(load3(V), load3(H), eta_inv);
+}
+void orc_average_fresnel_conductor(const float* eta, const float* k, float* out3) {
+    V3 r = average_fresnel_conductor(load3(eta), load3(k));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+float orc_average_fresnel_dielectric(float eta) { return average_fresnel_dielectric(eta); }
+float orc_ggx_d(const float* H, float au, float av) { return ggx_d(load3(H), V3(0.0f, 0.0f, 1.0f), au, av); }
+float orc_smith_g1(const float* V, const float* H, float au, float av) {
+    return smith_g1(load3(V), load3(H), V3(0.0f, 0.0f, 1.0f), au, av);
+}
 
 void orc_material_eval(const orc_material* m, const float* n, int front_face, const float* V, const float* L,
                        float* f, float* pdf) {

@@ -136,6 +136,17 @@ void orc_brute_closest(int64_t n, const float* positions, int64_t n_rays, const 
 
 /* ---- shading building blocks ---- */
 void orc_kulla_conty(float* bsdf_avg /*64*64*/, float* albedo_avg /*64*/);
+/* Microfacet.h helpers in the local frame N = +z: SampleGGX (half vector),
+   FresnelConductor (V and H unit, eta/k per channel), FresnelDielectric,
+   AverageFresnelConductor, AverageFresnelDielectric, DistributionGGX,
+   GeometrySmith_1. */
+void orc_sample_ggx(float alpha_u, float alpha_v, float u0, float u1, float* h3);
+void orc_fresnel_conductor(const float* V, const float* H, const float* eta, const float* k, float* out3);
+float orc_fresnel_dielectric(const float* V, const float* H, float eta_inv);
+void orc_average_fresnel_conductor(const float* eta, const float* k, float* out3);
+float orc_average_fresnel_dielectric(float eta);
+float orc_ggx_d(const float* H, float alpha_u, float alpha_v);
+float orc_smith_g1(const float* V, const float* H, float alpha_u, float alpha_v);
 /* Evaluate / sample a material at a shading frame.  n = shading normal
    (already flipped to the incident side), front_face as in SetFaceNormal.
    out: f[3], pdf, (sample) L[3]; rng_state advanced by the sample call. */

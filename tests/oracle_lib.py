@@ -60,6 +60,17 @@ def lib():
         L.orc_bvh_any.argtypes = [_P, C.c_int64, _P, _P, _P]
         L.orc_brute_closest.argtypes = [C.c_int64, _P, C.c_int64, _P, _P]
         L.orc_kulla_conty.argtypes = [_P, _P]
+        L.orc_sample_ggx.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, _P]
+        L.orc_fresnel_conductor.argtypes = [_P, _P, _P, _P, _P]
+        L.orc_fresnel_dielectric.argtypes = [_P, _P, C.c_float]
+        L.orc_fresnel_dielectric.restype = C.c_float
+        L.orc_average_fresnel_conductor.argtypes = [_P, _P, _P]
+        L.orc_average_fresnel_dielectric.argtypes = [C.c_float]
+        L.orc_average_fresnel_dielectric.restype = C.c_float
+        L.orc_ggx_d.argtypes = [_P, C.c_float, C.c_float]
+        L.orc_ggx_d.restype = C.c_float
+        L.orc_smith_g1.argtypes = [_P, _P, C.c_float, C.c_float]
+        L.orc_smith_g1.restype = C.c_float
         L.orc_material_eval.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P]
         L.orc_material_sample.argtypes = [_P, _P, C.c_int, _P, _P, _P, _P, _P]
         L.orc_medium_sample_distance.argtypes = [_P, C.c_float, _P, _P]
@@ -161,6 +172,49 @@ def kulla_conty():
     a = np.zeros(64, np.float32)
     lib().orc_kulla_conty(p(b), p(a))
     return b.reshape(64, 64), a
+
+
+def _v3(x):
+    return np.ascontiguousarray(x, np.float32).reshape(3)
+
+
+def sample_ggx(au, av, u0, u1):
+    h = np.zeros(3, np.float32)
+    lib().orc_sample_ggx(au, av, u0, u1, p(h))
+    return h
+
+
+def fresnel_conductor(V, H, eta, k):
+    out = np.zeros(3, np.float32)
+    V, H, eta, k = _v3(V), _v3(H), _v3(eta), _v3(k)
+    lib().orc_fresnel_conductor(p(V), p(H), p(eta), p(k), p(out))
+    return out
+
+
+def fresnel_dielectric(V, H, eta_inv):
+    V, H = _v3(V), _v3(H)
+    return float(lib().orc_fresnel_dielectric(p(V), p(H), eta_inv))
+
+
+def average_fresnel_conductor(eta, k):
+    out = np.zeros(3, np.float32)
+    eta, k = _v3(eta), _v3(k)
+    lib().orc_average_fresnel_conductor(p(eta), p(k), p(out))
+    return out
+
+
+def average_fresnel_dielectric(eta):
+    return float(lib().orc_average_fresnel_dielectric(eta))
+
+
+def ggx_d(H, au, av):
+    H = _v3(H)
+    return float(lib().orc_ggx_d(p(H), au, av))
+
+
+def smith_g1(V, H, au, av):
+    V, H = _v3(V), _v3(H)
+    return float(lib().orc_smith_g1(p(V), p(H), au, av))
 
 
 def material_eval(m, n, front, V, L):
