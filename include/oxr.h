@@ -200,6 +200,19 @@ int oxr_scene_download_env_cache(oxr_ctx* ctx, uint32_t scene_id, float* cache_r
 int oxr_trace_rays(oxr_ctx* ctx, const float* rays, int64_t n, int any_hit, float* hits,
                    uint8_t* occluded, uint32_t* counts);
 
+/* Ray queries that allocate nothing.  No counterpart in the reference: for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  As oxr_trace_rays,
+   but rays (n*8 floats), hits (n*4 words) and occluded (n bytes) are DEVICE
+   pointers the caller owns, and no visit counts are returned.  The traversal
+   stack spill and the overflow flag live in a fixed device array of 32 MiB
+   per GPU, held until the process ends; n is at most 2^31 (OXR_ERR_INVALID
+   beyond).  Every call on one GPU shares that array: calls that overlap on
+   one GPU (two contexts or threads at once) race.  Blocks until the rays are
+   traced, like oxr_trace_rays. */
+int oxr_trace_rays_noalloc(oxr_ctx* ctx, const float* rays, int64_t n, int any_hit, float* hits,
+                           uint8_t* occluded);
+
 /* ---- renderer (Renderer::Resize/SetCamera/Render/DownloadPixels, Renderer.h:13-25) ---- */
 int oxr_resize(oxr_ctx* ctx, int width, int height);
 int oxr_set_camera(oxr_ctx* ctx, const oxr_camera* cam);

@@ -291,6 +291,7 @@ SIGNATURES = {
     "oxr_destroy": (C.c_int, [_P]),
     "oxr_last_error": (C.c_char_p, [_P]),
     "oxr_malloc": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "oxr_trace_rays_noalloc": (C.c_int, [_P, _P, _I64, C.c_int, _P, _P]),
     "oxr_free": (C.c_int, [_P, _P]),
     "oxr_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "oxr_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),

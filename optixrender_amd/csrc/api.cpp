@@ -301,6 +301,9 @@ int oxr_create(int device, oxr_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP);
+    // the fixed scratch of oxr_trace_rays_noalloc on this device (its address, once per device)
+    if (noalloc_scratch(device, &c->noalloc_scratch) != hipSuccess || hipSetDevice(device) != hipSuccess)
+        return bail(OXR_ERR_HIP);
     const KcTable& t = kc_table();
     if (dalloc(&c->lut_bsdf, 64 * 64) != hipSuccess || dalloc(&c->lut_albedo, 64) != hipSuccess) return bail(OXR_ERR_OOM);
     if (hipMemcpy(c->lut_bsdf, t.bsdf, sizeof(t.bsdf), hipMemcpyHostToDevice) != hipSuccess ||
@@ -657,6 +660,29 @@ int oxr_trace_rays(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float*
     if (e == hipSuccess) e = hipMemcpy(&ov, c->eng.overflow, sizeof(int), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
+    if (ov) return fail(c, OXR_ERR_STATE, ov == 2 ? "traversal visited more nodes than the tree holds"
+                                                  : "traversal stack overflow");
+    return OXR_OK;
+}
+
+int oxr_trace_rays_noalloc(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float* hits, uint8_t* occ) {
+    if (!c) return OXR_ERR_INVALID;
+    if (n < 0 || (n > 0 && !rays)) return fail(c, OXR_ERR_INVALID, "trace_rays_noalloc: invalid rays");
+    if (n > kNoAllocMaxRays)
+        return fail(c, OXR_ERR_INVALID, "trace_rays_noalloc: more than 2^31 rays in one call");
+    if (n > 0 && any_hit && !occ) return fail(c, OXR_ERR_INVALID, "trace_rays_noalloc: occluded output required");
+    if (n > 0 && !any_hit && !hits) return fail(c, OXR_ERR_INVALID, "trace_rays_noalloc: hits output required");
+    SceneDev* s = selected(c);
+    if (!s || !s->built) return fail(c, OXR_ERR_STATE, "no built scene selected");
+    if (!c->noalloc_scratch) return fail(c, OXR_ERR_STATE, "trace_rays_noalloc: no scratch on this device");
+    OXR_HIP(c, hipSetDevice(c->device));
+    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
+    hipError_t e = trace_rays_noalloc(bv, rays, n, any_hit, (float4*)hits, occ, c->noalloc_scratch, c->stream);
+    int ov = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&ov, c->noalloc_scratch + kNoAllocSpillWords, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("trace_rays_noalloc: ") + hipGetErrorString(e));
     if (ov) return fail(c, OXR_ERR_STATE, ov == 2 ? "traversal visited more nodes than the tree holds"
                                                   : "traversal stack overflow");
     return OXR_OK;

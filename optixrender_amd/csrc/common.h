@@ -54,6 +54,19 @@ hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void*
 hipError_t trace_rays(const BvhView& b, const float* d_rays, int64_t n, int any, float4* d_hits, uint8_t* d_occ,
                       uint32_t* d_counts, uint32_t* gstack, int grid, int* overflow, hipStream_t s);
 inline size_t gstack_words(int grid) { return (size_t)(kStackTotal - kStackLdsMin) * grid * kTraceBlock; }
+// Ray queries that allocate nothing (oxr_trace_rays_noalloc): k_trace_rays
+// on at most kNoAllocTraceGrid blocks, its stack spill and overflow word in a
+// fixed device array (render.hip), 32 MiB + 256 B per device.
+constexpr int kMaxDevices = 64;
+constexpr int kNoAllocTraceGrid = 1024;
+constexpr size_t kNoAllocSpillWords = (size_t)(kStackTotal - 32) * kNoAllocTraceGrid * kTraceBlock;
+constexpr size_t kNoAllocScratchWords = kNoAllocSpillWords + 64;
+constexpr int64_t kNoAllocMaxRays = (int64_t)1 << 31;  // rays per call
+// The device address of the fixed scratch on `device` (looked up once per
+// device with that device made current; the caller restores its device).
+hipError_t noalloc_scratch(int device, uint32_t** out);
+hipError_t trace_rays_noalloc(const BvhView& b, const float* d_rays, int64_t n, int any, float4* d_hits,
+                              uint8_t* d_occ, uint32_t* scratch_ptr, hipStream_t s);
 // lum_sum: the sequential float sum of the texel luminances in row-major
 // order (the reference's host loop), computed by env_luminance_sum.
 hipError_t env_cache_build(const float4* d_env, int w, int h, float lum_sum, float4* d_cache, float4* d_env_pdf,

@@ -140,6 +140,7 @@ struct oxr_ctx {
     oxr_render_stats stats;
     hipEvent_t t0 = nullptr, t1 = nullptr, td0 = nullptr, td1 = nullptr;
     oxr::ReqDesc* reqs = nullptr;  // device request table (single renders use entry 0)
+    uint32_t* noalloc_scratch = nullptr;  // this device's copy of the fixed scratch (oxr_trace_rays_noalloc)
     uint8_t* cover = nullptr;      // camera-ray cover mask of single renders
     size_t cover_tiles = 0;        // its capacity
     // denoiser: weights (oxr_denoiser_load), output and scratch of the last denoised render

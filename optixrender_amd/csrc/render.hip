@@ -16,6 +16,7 @@
 // counters (common.h); stage kernels are persistent and a wave takes 64 queue
 // entries of its shard per atomic.
 #include <algorithm>
+#include <mutex>
 #include "common.h"
 #include "shading.h"
 #include "traverse.h"
@@ -1317,6 +1318,54 @@ hipError_t trace_rays(const BvhView& b, const float* rays, int64_t n, int any, f
     if (n <= 0) return hipSuccess;
     if (any) k_trace_rays<true><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, counts, gstack, overflow);
     else k_trace_rays<false><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, counts, gstack, overflow);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ ray queries that allocate nothing
+// Scratch of oxr_trace_rays_noalloc, resident for the life of the process on
+// every device the library runs on: the traversal stack spill of
+// kNoAllocTraceGrid blocks of k_trace_rays (32 entries per lane in LDS, the
+// other kStackTotal - 32 here: entry i of grid lane g at g + i * lanes) and,
+// after it, the overflow word.  Host code never passes `scratch` itself: it
+// uses the device address noalloc_scratch() obtained with
+// hipGetSymbolAddress, one per device.
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+namespace {
+std::mutex g_noalloc_mu;
+uint32_t* g_noalloc_ptr[kMaxDevices] = {};
+
+__global__ void k_clear_word(uint32_t* w) {
+    if (threadIdx.x == 0) *w = 0u;
+}
+}  // namespace
+
+hipError_t noalloc_scratch(int device, uint32_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        hipError_t e = hipSetDevice(device);   // the array's copy on this device
+        if (e != hipSuccess) return e;
+        uint32_t* p = nullptr;
+        e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
+hipError_t trace_rays_noalloc(const BvhView& b, const float* rays, int64_t n, int any, float4* hits, uint8_t* occ,
+                              uint32_t* scratch_ptr, hipStream_t s) {
+    uint32_t* overflow = scratch_ptr + kNoAllocSpillWords;
+    k_clear_word<<<1, 64, 0, s>>>(overflow);   // a fill kernel, not a memset: callers may capture this
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
+    const int64_t want = (n + kTraceBlock - 1) / kTraceBlock;
+    const int grid = (int)std::min<int64_t>(want, kNoAllocTraceGrid);   // the spill area holds this many blocks
+    if (any) k_trace_rays<true><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, nullptr, scratch_ptr, (int*)overflow);
+    else k_trace_rays<false><<<grid, kTraceBlock, 0, s>>>(b, rays, n, hits, occ, nullptr, scratch_ptr, (int*)overflow);
     return hipGetLastError();
 }
 
