@@ -9,13 +9,18 @@ before the timed region (the build time is reported separately).
 
 Prints one JSON line with Mpaths/s, the traversal roofline of the trace
 kernel (algorithmic bytes from an untimed counting pass of the same frames,
-kernel time from HIP events on the library's stream during the timed steps)
-and the CPU oracle timed on a bounded sample on this host.
+kernel time from HIP events on the library's stream during the timed steps,
+HBM bytes from the committed counter summary) and the CPU oracle timed on
+this host's cores.  Legs beside the headline (in `config`): the viewer loop,
+the denoiser, serving, and the batched configuration -- the 20 M-triangle
+heightfield with the medium box at 1080p, 32 spp, depth 32, autotuned and
+replayed from captured graphs, with its own roofline and CPU oracle sample.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -42,6 +47,10 @@ def parse():
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-heightfield", action="store_true",
+                    help="skip the 20 M-triangle heightfield leg (1080p, 32 spp, depth 32, autotuned graphs)")
+    ap.add_argument("--workload", choices=["sphere_field", "heightfield"], default="sphere_field",
+                    help="with --profile-steps: which workload's steps to run")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg (p50/p99 latency at equal RMSE)")
     ap.add_argument("--no-viewer", action="store_true", help="skip the viewer leg (1-spp frames with readback)")
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser leg (KPCNN time per 1080p frame)")
@@ -58,40 +67,111 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(scene, args):
-    """The CPU oracle on the same workload: full frames, as many spp as fit
-    about 12 s (the per-frame cost and the fixed cost of the oracle's own LBVH
-    build and env cache are measured first, with 1 and 3 spp)."""
+def host_cpu():
+    """The host's CPU as the OS reports it: logical CPUs, the ones this
+    process may run on, and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "model": model}
+
+
+def oracle_rate(scene, W, H, depth, threads, spp, row_step=1):
+    """Mpaths/s of the CPU oracle over W x H (every row_step-th row) at spp
+    frames: paths traced over the time spent tracing them, after the oracle's
+    own LBVH build and environment cache (reported separately)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib  # checker only: the baseline leg, never the measured path
 
-    threads = min(os.cpu_count() or 1, 16)
-    W, H, depth = args.width, args.height, args.depth
-    def timed(n):
-        t = time.perf_counter()
-        oracle_lib.render_stats(scene, W, H, n, max_depth=depth, threads=threads)
-        return time.perf_counter() - t
-
-    t1, t3 = timed(1), timed(3)
-    per_spp = max((t3 - t1) / 2.0, 1e-3)
-    fixed = max(t1 - per_spp, 0.0)
-    spp = int(max(1, min(args.spp, (12.0 - fixed) / per_spp)))
     t0 = time.perf_counter()
-    stats = oracle_lib.render_stats(scene, W, H, spp, max_depth=depth, threads=threads)
-    dt = time.perf_counter() - t0
+    st = oracle_lib.render_stats(scene, W, H, spp, max_depth=depth, threads=threads, row_step=row_step)
+    wall = time.perf_counter() - t0
+    return st["paths"] / st["trace_s"] / 1e6, st, wall - st["trace_s"]
+
+
+def cpu_baseline(scene, W, H, depth, spp_total, label, budget_s=12.0, single_s=4.0):
+    """The CPU oracle on this host: every usable core, and one core, each on
+    a bounded sample of the same workload sized from a quick probe (whole
+    frames when they fit the budget, else every k-th row of one frame; the
+    camera and per-pixel RNG are those of the full workload)."""
+    cores = len(os.sched_getaffinity(0))
+    probe_rate, _, setup_s = oracle_rate(scene, W, H, depth, cores, 1, row_step=16)
+    full_frame_s = W * H / (probe_rate * 1e6)
+    if full_frame_s <= budget_s:
+        spp, step = int(max(1, min(spp_total, budget_s // full_frame_s))), 1
+    else:
+        spp, step = 1, int(math.ceil(full_frame_s / budget_s))
+    rate, st, _ = oracle_rate(scene, W, H, depth, cores, spp, row_step=step)
+    one_step = int(max(1, math.ceil(W * H / (probe_rate / cores * 1e6) / single_s)))
+    rate1, st1, _ = oracle_rate(scene, W, H, depth, 1, 1, row_step=one_step)
+    frac = st["paths"] / (W * H * spp_total)
     return {
-        "value": round(W * H * spp / dt / 1e6, 4),
+        "value": round(rate, 4),
         "unit": "Mpaths/s",
-        "cores": threads,
+        "cores": cores,
         "kind": "port",
-        "sample": f"{W}x{H} at {spp} spp (frames 0..{spp - 1}, depth {depth}) of the {spp_total(args)} spp workload; "
-                  f"includes the oracle's CPU LBVH build and env cache; {stats['extension_rays']} extension rays",
-        "seconds": round(dt, 2),
+        "sample": f"{label}: {W}x{H}" + (f", every {step}th row" if step > 1 else "") +
+                  f", {spp} spp (frames 0..{spp - 1}) of the {spp_total}-spp workload: {st['paths']} paths, "
+                  f"{frac:.4f} of the workload (time extrapolates linearly by {1 / frac:.1f}x); "
+                  f"{st['extension_rays']} extension rays",
+        "seconds": round(st["trace_s"], 2),
+        "single_thread_Mpaths_s": round(rate1, 4),
+        "single_thread_sample": f"{W}x{H}" + (f", every {one_step}th row" if one_step > 1 else "") +
+                                f", 1 spp: {st1['paths']} paths in {st1['trace_s']:.2f} s",
+        "setup_s_excluded": round(setup_s, 2),
+        "host": host_cpu(),
     }
 
 
-def spp_total(args):
-    return args.spp
+NODE_BYTES_DOC = "32-B node record per child box fetched, 36-B triangle per test, 48 B per extension ray, 40 per shadow ray"
+
+
+def traversal_bytes(counted):
+    """Algorithmic bytes of one step's k_trace launches from a counting
+    render of the same frames: camera rays the cover mask rules out are never
+    traced, and the tail kernel's rays are its own launch."""
+    ext_bytes = NODE_BYTES * (counted.ext_node_fetches - counted.resolved_node_fetches) + \
+        TRI_BYTES * (counted.ext_tri_tests - counted.resolved_tri_tests) + \
+        RAY_BYTES * (counted.extension_rays - counted.camera_rays_resolved)
+    sh_nodes = counted.node_fetches - counted.ext_node_fetches
+    sh_tris = counted.tri_tests - counted.ext_tri_tests
+    sh_bytes = NODE_BYTES * sh_nodes + TRI_BYTES * sh_tris + SHADOW_RAY_BYTES * counted.shadow_rays
+    tail_bytes = NODE_BYTES * counted.tail_node_fetches + TRI_BYTES * counted.tail_tri_tests + \
+        RAY_BYTES * counted.tail_extension_rays + SHADOW_RAY_BYTES * counted.tail_shadow_rays
+    ext_rays = max(counted.extension_rays - counted.camera_rays_resolved, 1)
+    return ext_bytes + sh_bytes - tail_bytes, ext_bytes / ext_rays, sh_bytes / max(counted.shadow_rays, 1)
+
+
+def roofline(counted, trace_ms, launches, traffic_file, workload, tuning, strict=True):
+    """The roofline object of k_trace: `achieved` / `frac` from algorithmic
+    bytes (every node and triangle fetch at full size), `traffic` the HBM
+    bytes per launch the counters measured (FETCH_SIZE + WRITE_SIZE, scaled
+    by a copy of known size), and `dram_frac` those bytes over the same
+    kernel time: how much of the algorithmic traffic reaches DRAM."""
+    alg, per_ext, per_sh = traversal_bytes(counted)
+    achieved = alg / (trace_ms / 1e3) / 1e9
+    measured = measured_traffic(traffic_file, workload, tuning, strict)
+    out = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured["per_launch"] if measured else None,
+        "kernel": "k_trace (all launches of one step: extension and shadow rays)",
+        "launches_per_step": int(launches), "bytes_per_step": int(alg),
+        "bytes_basis": f"algorithmic: {NODE_BYTES_DOC}, node and triangle counts from a counting render",
+        "per_ray_bytes": {"extension": round(per_ext, 1), "shadow": round(per_sh, 1)},
+    }
+    if measured is not None:
+        step_bytes = measured["per_step"] or measured["per_launch"] * launches
+        dram = step_bytes / (trace_ms / 1e3) / 1e9
+        out["dram_achieved"] = round(dram, 1)
+        out["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
+        out["traffic_source"] = f"{traffic_file.relative_to(ROOT)} (tuning {measured['tuning']})"
+    return out
 
 
 def serving(ctx, args):
@@ -141,28 +221,33 @@ def copy_calibration():
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_k_trace_sphere_field_1080p_64spp.md"
+TRAFFIC_FILE_HF = ROOT / "profiles" / "pmc_k_trace_heightfield_1080p_32spp.md"
 
 
 def tuning_key(t) -> str:
     return json.dumps(t.as_dict(), sort_keys=True)
 
 
-def measured_traffic(workload: str, tuning: str):
-    """HBM bytes per k_trace launch from the committed counter summary, if
-    it was measured on this workload with this tuning (lines `workload: ...`,
-    `tuning: ...`, `traffic_bytes_per_launch: N`), else None."""
-    if not TRAFFIC_FILE.exists():
+def measured_traffic(path: Path, workload: str, tuning: str, strict: bool = True):
+    """HBM bytes of k_trace from a committed counter summary (lines
+    `workload: ...`, `tuning: ...`, `traffic_bytes_per_launch: N` and
+    optionally `traffic_bytes_per_step: N`) measured on this workload: a dict
+    (per_launch, per_step, tuning), or None.  strict: only when its tuning is
+    this run's too (per-launch bytes depend on the launch shape)."""
+    if not path.exists():
         return None
     fields = {}
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if ":" in line and not line.startswith(("|", "#", " ", "`")):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
-    if fields.get("workload") != workload or fields.get("tuning") != tuning:
+    if fields.get("workload") != workload or (strict and fields.get("tuning") != tuning):
         return None
     if "traffic_bytes_per_launch" not in fields:
         return None
-    return int(float(fields["traffic_bytes_per_launch"]))
+    per_step = fields.get("traffic_bytes_per_step")
+    return {"per_launch": int(float(fields["traffic_bytes_per_launch"])),
+            "per_step": int(float(per_step)) if per_step else None, "tuning": fields.get("tuning")}
 
 
 def viewer(ctx, args, scene, frames=32):
@@ -222,6 +307,96 @@ def denoiser(ctx, args, scene, frames=8):
     return out
 
 
+HF_W, HF_H, HF_SPP, HF_DEPTH = 1920, 1080, 32, 32
+
+
+def heightfield_leg(ctx, args, steps=3):
+    """The batched configuration: 1920x1080, 32 spp, depth 32 over the
+    20,016,980-triangle fBm heightfield with the homogeneous-medium box and
+    the dielectric sphere, per-triangle interleaved materials.  The launch
+    tuning comes from oxr_autotune on a 4-spp probe; the timed steps replay
+    the captured iteration graphs.  Its own counting render gives the
+    traversal roofline, and the CPU oracle is timed on a sample of it."""
+    from optixrender_amd import abi, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.heightfield_medium()
+    gen_s = time.perf_counter() - t0
+    sid = ctx.upload(sc)
+    build_ms = ctx.build_bvh(sid)
+    base = ctx.get_tuning()
+    try:
+        ctx.select(sid)
+        ctx.resize(HF_W, HF_H)
+        ctx.set_camera(sc.camera)
+        print(f"[bench] heightfield: {sc.n_tris} tris generated in {gen_s:.1f}s, BVH build {build_ms:.1f} ms",
+              file=sys.stderr, flush=True)
+        tuned, probe_ms = ctx.autotune(spp=4, max_depth=HF_DEPTH, rr_start=0)
+        ctx.set_camera(sc.camera)
+        print(f"[bench] heightfield: autotune {tuned.as_dict()} ({probe_ms:.2f} ms per 4-spp probe)",
+              file=sys.stderr, flush=True)
+
+        def step(flags):
+            return ctx.render(spp=HF_SPP, max_depth=HF_DEPTH, rr_start=0, progressive=False, flags=flags)
+
+        counted = step(abi.OXR_FLAG_COUNT_TRAVERSAL)
+        staged = [step(abi.OXR_FLAG_STAGE_TIMING) for _ in range(2)][-1]
+        step(0)                                     # warm the graphs
+        tg = time.perf_counter()
+        for _ in range(steps):
+            step(0)
+        ms = (time.perf_counter() - tg) * 1e3 / steps
+        paths = HF_W * HF_H * HF_SPP
+        workload = f"heightfield_medium {sc.n_tris} tris, {HF_W}x{HF_H}, {HF_SPP} spp, depth {HF_DEPTH} (seed 1)"
+        out = {
+            "workload": workload, "metric": "Mpaths/s", "value": round(paths / (ms / 1e3) / 1e6, 3),
+            "ms_per_step": round(ms, 3), "steps": steps, "launch": "captured iteration graphs, autotuned",
+            "bvh_build_ms": round(build_ms, 3), "tuning": tuned.as_dict(), "autotune_probe_ms": round(probe_ms, 3),
+            "stage_ms": {"trace": round(staged.trace_ms, 3), "shade": round(staged.shade_ms, 3),
+                         "regenerate": round(staged.generate_ms, 3), "accumulate": round(staged.accumulate_ms, 3),
+                         "eager_total": round(staged.render_ms, 3)},
+            "iterations_per_step": int(staged.iterations), "extension_rays": int(counted.extension_rays),
+            "shadow_rays": int(counted.shadow_rays),
+            "roofline": roofline(counted, staged.trace_ms, staged.iterations, TRAFFIC_FILE_HF, workload,
+                                 tuning_key(ctx.get_tuning()), strict=False),
+        }
+        print(f"[bench] heightfield: {ms:.1f} ms/step, {out['value']} Mpaths/s", file=sys.stderr, flush=True)
+        if not args.no_cpu_baseline:
+            print("[bench] heightfield: timing the CPU oracle sample", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(sc, HF_W, HF_H, HF_DEPTH, HF_SPP, "heightfield_medium")
+        return out
+    finally:
+        ctx.set_tuning(base)
+        ctx.release(sid)
+
+
+def profile_heightfield(args):
+    """Only the heightfield leg's graph-replayed steps (for rocprofv3): the
+    tuning is --tuning's, else the default one."""
+    from optixrender_amd import scenes
+    from optixrender_amd.renderer import Context
+
+    sc = scenes.heightfield_medium()
+    ctx = Context(0)
+    sid = ctx.upload(sc)
+    ctx.build_bvh(sid)
+    ctx.select(sid)
+    ctx.resize(HF_W, HF_H)
+    ctx.set_camera(sc.camera)
+    if args.tuning:
+        t = ctx.get_tuning()
+        for kv in args.tuning.split(","):
+            k, v = kv.split("=")
+            setattr(t, k.strip(), int(v))
+        ctx.set_tuning(t)
+    print(f"[bench] heightfield profile: tuning {tuning_key(ctx.get_tuning())}", file=sys.stderr, flush=True)
+    for i in range(args.warmup + args.steps):
+        st = ctx.render(spp=HF_SPP, max_depth=HF_DEPTH, rr_start=0, progressive=False, flags=0)
+        print(f"[bench] heightfield profile step {i + 1}: {st.render_ms:.1f} ms, {st.iterations} iterations",
+              file=sys.stderr, flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -233,6 +408,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+
+    if args.profile_steps and args.workload == "heightfield":
+        profile_heightfield(args)
+        return
 
     t0 = time.perf_counter()
     scene = scenes.sphere_field(seed=1, subdiv=args.subdiv)
@@ -325,30 +504,8 @@ def main():
         out["config"]["graph_replay_ms_per_step"] = round(graph_ms, 3)
         out["config"]["graph_replay_Mpaths_s"] = round(paths / (graph_ms / 1e3) / 1e6, 3)
     if counted is not None:
-        # the counting pass traces every camera ray; the timed steps resolve the
-        # ones the cover mask rules out without a trace, so their bytes are not moved
-        ext_bytes = NODE_BYTES * (counted.ext_node_fetches - counted.resolved_node_fetches) + \
-            TRI_BYTES * (counted.ext_tri_tests - counted.resolved_tri_tests) + \
-            RAY_BYTES * (counted.extension_rays - counted.camera_rays_resolved)
-        sh_nodes = counted.node_fetches - counted.ext_node_fetches
-        sh_tris = counted.tri_tests - counted.ext_tri_tests
-        sh_bytes = NODE_BYTES * sh_nodes + TRI_BYTES * sh_tris + SHADOW_RAY_BYTES * counted.shadow_rays
-        # the tail kernel traces the last bounces of the window's longest paths
-        # in its own launch: its share of the counted work is not k_trace's
-        tail_bytes = NODE_BYTES * counted.tail_node_fetches + TRI_BYTES * counted.tail_tri_tests + \
-            RAY_BYTES * counted.tail_extension_rays + SHADOW_RAY_BYTES * counted.tail_shadow_rays
-        alg = ext_bytes + sh_bytes - tail_bytes
-        achieved = alg / (float(np.mean(trace_ms)) / 1e3) / 1e9
-        out["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(out["config"]["workload"], tuning_key(ctx.get_tuning())),
-            "kernel": "k_trace (all launches of one step: extension and shadow rays)",
-            "launches_per_step": int(st.iterations),
-            "bytes_per_step": int(alg),
-            "per_ray_bytes": {"extension": round(ext_bytes / max(counted.extension_rays - counted.camera_rays_resolved, 1), 1),
-                              "shadow": round(sh_bytes / max(counted.shadow_rays, 1), 1)},
-        }
+        out["roofline"] = roofline(counted, float(np.mean(trace_ms)), st.iterations, TRAFFIC_FILE,
+                                   out["config"]["workload"], tuning_key(ctx.get_tuning()))
         out["config"].update({"extension_rays": int(counted.extension_rays), "shadow_rays": int(counted.shadow_rays),
                               "camera_rays_resolved_without_trace": int(counted.camera_rays_resolved),
                               "tail_kernel_rays": int(counted.tail_extension_rays + counted.tail_shadow_rays),
@@ -361,9 +518,11 @@ def main():
         out["config"]["denoiser"] = denoiser(ctx, args, scene)
     if not args.no_serving and not args.profile_steps:
         out["config"]["serving"] = serving(ctx, args)
+    if not args.no_heightfield and not args.profile_steps:
+        out["config"]["heightfield"] = heightfield_leg(ctx, args)
     if not args.no_cpu_baseline and not args.profile_steps:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(scene, args)
+        out["cpu_baseline"] = cpu_baseline(scene, W, H, depth, spp, "sphere_field")
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -59,6 +59,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -2093,10 +2094,15 @@ int orc_render_stats(const orc_scene* sc, const orc_camera* cam_in, const orc_re
     int threads = p->threads <= 0 ? (int)std::thread::hardware_concurrency() : p->threads;
     if (threads < 1) threads = 1;
     std::vector<Stats> per(threads);
-    parallel_rows(p->height, threads, [&](int iy, int t) {
+    const int step = p->row_step > 1 ? p->row_step : 1;
+    const int rows = (p->height + step - 1) / step;
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_rows(rows, threads, [&](int r, int t) {
+        const int iy = r * step;
         for (int ix = 0; ix < p->width; ix++)
             for (int s = 0; s < p->spp; s++) trace_path(S, cam, *p, ix, iy, p->frame_base + s, &per[t]);
     });
+    auto t1 = std::chrono::steady_clock::now();
     Stats tot;
     for (auto& s : per) {
         tot.ext += s.ext; tot.shadow += s.shadow; tot.nodes += s.nodes; tot.tris += s.tris;
@@ -2104,6 +2110,8 @@ int orc_render_stats(const orc_scene* sc, const orc_camera* cam_in, const orc_re
     }
     out4[0] = tot.ext; out4[1] = tot.shadow; out4[2] = tot.nodes; out4[3] = tot.tris;
     out4[4] = tot.ext_nodes; out4[5] = tot.ext_tris;
+    out4[6] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    out4[7] = (uint64_t)rows * p->width * p->spp;
     return 0;
 }
 

@@ -109,7 +109,7 @@ typedef struct orc_render_params {
     int32_t frame_base;     /* frameID of the first frame */
     int32_t threads;        /* host threads, 0 = all */
     int32_t fp64_accum;     /* 1: accumulate the mean in double */
-    int32_t pad;
+    int32_t row_step;       /* orc_render_stats: trace rows iy % row_step == 0 only (0 or 1: all) */
 } orc_render_params;
 
 /* ---- RNG (random.h:55-89) ---- */
@@ -177,9 +177,11 @@ int orc_render(const orc_scene* scene, const orc_camera* cam, const orc_render_p
                float* color, float* albedo, float* normal, float* depth, float* path_radiance);
 /* counts of extension/shadow rays and traversal work, for measurement:
    out[0] extension rays, out[1] shadow rays, out[2] node records fetched,
-   out[3] triangle tests, out[4] / out[5] the same two for extension rays. */
+   out[3] triangle tests, out[4] / out[5] the same two for extension rays,
+   out[6] nanoseconds spent tracing paths (after the scene's LBVH build and
+   environment cache), out[7] paths traced. */
 int orc_render_stats(const orc_scene* scene, const orc_camera* cam, const orc_render_params* p,
-                     uint64_t* out4);
+                     uint64_t* out8);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,7 @@ class OrcRenderParams(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
         ("rr_start", C.c_int32), ("filter", C.c_int32), ("frame_base", C.c_int32), ("threads", C.c_int32),
-        ("fp64_accum", C.c_int32), ("pad", C.c_int32),
+        ("fp64_accum", C.c_int32), ("row_step", C.c_int32),
     ]
 
 
@@ -345,11 +345,15 @@ def render(scene: SceneData, width, height, spp, max_depth=25, rr_start=0, filt=
 
 
 def render_stats(scene: SceneData, width, height, spp, max_depth=25, rr_start=0, filt=3, frame_base=0, threads=0,
-                 camera=None):
+                 camera=None, row_step=1):
+    """Ray and traversal counts of a render; with row_step > 1 only every
+    row_step-th row is traced.  trace_s / paths: the time spent tracing (the
+    oracle's LBVH build and environment cache excluded) and the paths traced."""
     s, keep = _scene_struct(scene)
-    prm = OrcRenderParams(width, height, spp, max_depth, rr_start, filt, frame_base, threads, 0, 0)
-    out = np.zeros(6, np.uint64)
+    prm = OrcRenderParams(width, height, spp, max_depth, rr_start, filt, frame_base, threads, 0, row_step)
+    out = np.zeros(8, np.uint64)
     cam = camera or scene.camera
     assert lib().orc_render_stats(C.byref(s), C.byref(cam), C.byref(prm), p(out)) == 0
     return dict(extension_rays=int(out[0]), shadow_rays=int(out[1]), node_fetches=int(out[2]), tri_tests=int(out[3]),
-                ext_node_fetches=int(out[4]), ext_tri_tests=int(out[5]))
+                ext_node_fetches=int(out[4]), ext_tri_tests=int(out[5]), trace_s=float(out[6]) * 1e-9,
+                paths=int(out[7]))

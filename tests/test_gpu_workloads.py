@@ -98,8 +98,8 @@ def test_sphere_field_1080p_ray_counts(ctx, sphere_field):
     gpu = dict(extension_rays=st.extension_rays, shadow_rays=st.shadow_rays, node_fetches=st.node_fetches,
                tri_tests=st.tri_tests, ext_node_fetches=st.ext_node_fetches, ext_tri_tests=st.ext_tri_tests)
     print(f"[counts] gpu {gpu}\n[counts] oracle {ref}", flush=True)
-    for k, v in ref.items():
-        assert abs(gpu[k] - v) <= 1e-4 * v, k
+    for k, v in gpu.items():
+        assert abs(v - ref[k]) <= 1e-4 * ref[k], k
     assert st.extension_rays >= w * h * spp      # every camera ray is traced when counting
 
 
