@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser leg (KPCNN time per 1080p frame)")
     ap.add_argument("--profile-denoiser", action="store_true",
                     help="only the denoiser leg: 4-spp renders denoised at the bench resolution (for rocprofv3)")
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests per serving arm (10,000 in optixrender_amd.serve_bench's own run)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
     ap.add_argument("--tuning", default="",
@@ -179,27 +180,22 @@ def serving(ctx, args):
     scenes (Cornell box, this sphere field, the 20 M-triangle heightfield
     with the medium box, a 5 M-triangle sphere field) served by the
     continuous-batching server, plain accumulation against the KPCNN
-    denoiser at equal RMSE of the tone-mapped image
-    (optixrender_amd.serve_bench)."""
+    denoiser at equal display RMSE per scene (optixrender_amd.serve_bench:
+    each request's denoised sample count comes from its scene's measured
+    RMSE curves, plain accumulation where the network does not win)."""
     from optixrender_amd import denoise, serve_bench
 
     print("[bench] serving: uploading the resident scenes", file=sys.stderr, flush=True)
     ctx.load_denoiser(denoise.flat_weights(denoise.load_weights()))
     resident = serve_bench.upload_scenes(ctx)
-    _, sid, cam = resident[1]
-    eq = serve_bench.equal_rmse_factor(ctx, sid, cam)
-    print(f"[bench] serving: equal RMSE {json.dumps(eq)}", file=sys.stderr, flush=True)
-    stream = serve_bench.request_stream(args.serve_requests, args.serve_rate, len(resident))
-    plain = serve_bench.run_arm(ctx, resident, stream, False, 1.0)
-    print(f"[bench] serving: plain {json.dumps(plain)}", file=sys.stderr, flush=True)
-    dn = serve_bench.run_arm(ctx, resident, stream, True, eq["factor"])
-    print(f"[bench] serving: denoised {json.dumps(dn)}", file=sys.stderr, flush=True)
-    for _, s, _ in resident:
-        ctx.release(s)
-    return {"requests": args.serve_requests, "rate_per_s": args.serve_rate,
-            "scenes": [r[0] for r in resident], "equal_rmse": eq,
-            "plain_p50_ms": plain["p50_ms"], "plain_p99_ms": plain["p99_ms"],
-            "denoised_p50_ms": dn["p50_ms"], "denoised_p99_ms": dn["p99_ms"], "plain": plain, "denoised": dn}
+    try:
+        res = serve_bench.measure(ctx, resident, args.serve_requests, args.serve_rate)
+    finally:
+        for _, s, _ in resident:
+            ctx.release(s)
+    res.update({"plain_p50_ms": res["plain"]["p50_ms"], "plain_p99_ms": res["plain"]["p99_ms"],
+                "denoised_p50_ms": res["denoised"]["p50_ms"], "denoised_p99_ms": res["denoised"]["p99_ms"]})
+    return res
 
 
 COPY_BYTES = 1 << 30

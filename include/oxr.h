@@ -165,6 +165,11 @@ typedef struct oxr_render_stats {
        triangle tests, included in node_fetches / tri_tests. */
     uint64_t tail_extension_rays, tail_shadow_rays;
     uint64_t tail_node_fetches, tail_tri_tests;
+    /* With OXR_FLAG_COUNT_TRAVERSAL, the SIMD use of the trace launches'
+       traversal loop: internal-node visits made by the lanes, and 64 x the
+       visits of the busiest lane of each 64-ray chunk (the loop iterations
+       the waves ran).  Their ratio is the fraction of lanes doing work. */
+    uint64_t trace_lane_visits, trace_wave_slots;
 } oxr_render_stats;
 
 /* ---- context and memory (CUDABuffer-style API, common/CUDABuffer.h:26-81) ---- */
@@ -287,6 +292,9 @@ int oxr_result_color(oxr_server* s, uint64_t id, float* out_rgba, size_t count);
 /* frees a finished request's buffers and forgets it */
 int oxr_release_request(oxr_server* s, uint64_t id);
 int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out);
+/* A finished request's submit -> admission time (waiting for a slot and HBM
+   budget) and submit -> done latency; OXR_ERR_STATE until it is done. */
+int oxr_request_times(oxr_server* s, uint64_t id, double* queued_ms, double* total_ms);
 
 /* ---- launch tuning (no reference counterpart: OptiX chose these internally) ---- */
 typedef struct oxr_tuning {

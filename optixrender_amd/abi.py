@@ -222,6 +222,8 @@ class RenderStats(C.Structure):
         ("tail_shadow_rays", C.c_uint64),
         ("tail_node_fetches", C.c_uint64),
         ("tail_tri_tests", C.c_uint64),
+        ("trace_lane_visits", C.c_uint64),
+        ("trace_wave_slots", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -327,6 +329,7 @@ SIGNATURES = {
     "oxr_result_color": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t]),
     "oxr_release_request": (C.c_int, [_P, C.c_uint64]),
     "oxr_server_get_stats": (C.c_int, [_P, _P]),
+    "oxr_request_times": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oxr_get_tuning": (C.c_int, [_P, _P]),
     "oxr_set_tuning": (C.c_int, [_P, _P]),
     "oxr_autotune": (C.c_int, [_P, _P, _P, C.POINTER(C.c_double)]),

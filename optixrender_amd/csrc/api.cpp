@@ -174,6 +174,17 @@ uint16_t to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// Output channel of MFMA row `row` of layer l (-1: a padding row).  Rows are
+// channels except in the last layer, whose 25 taps are split over the two
+// halves of a wave: row 8 g + 4 h + i holds tap j = 4 g + i (h = 0, j < 13)
+// or 13 + j (h = 1, j < 12), so that a lane's accumulator register j holds
+// one tap of its pixel.
+static int out_channel(int l, int row) {
+    if (l < kDnLayers - 1) return row;
+    const int g = row / 8, h = (row / 4) % 2, i = row % 4, j = 4 * g + i;
+    return h == 0 ? (j < 13 ? j : -1) : (j < 12 ? 13 + j : -1);
+}
+
 // Flat KPCNN weights (per layer w[co][ci][3][3], b[co]) -> MFMA lane order
 // (the layout DenoiserDev documents) and padded biases.
 void arrange_denoiser(const float* w, std::vector<uint16_t>& lanes, std::vector<float>& bias, int64_t* offset) {
@@ -193,13 +204,17 @@ void arrange_denoiser(const float* w, std::vector<uint16_t>& lanes, std::vector<
             for (int s = 0; s < steps; s++)
                 for (int q = 0; q < 64; q++)
                     for (int e = 0; e < 8; e++) {
-                        const int co = q % 32, ci = 16 * s + 8 * (q / 32) + e;
-                        const float v = (co < co_n && ci < ci_n) ? W[((size_t)(co * ci_n + ci) * 3 + t / 3) * 3 + t % 3]
-                                                                 : 0.0f;
+                        const int co = out_channel(l, q % 32), ci = 16 * s + 8 * (q / 32) + e;
+                        const float v = (co >= 0 && co < co_n && ci < ci_n)
+                                            ? W[((size_t)(co * ci_n + ci) * 3 + t / 3) * 3 + t % 3]
+                                            : 0.0f;
                         lanes.push_back(to_bf16(v));
                     }
         groups += 9 * steps * 64;
-        for (int co = 0; co < co_n; co++) bias[l * 32 + co] = B[co];
+        for (int row = 0; row < 32; row++) {
+            const int co = out_channel(l, row);
+            if (co >= 0 && co < co_n) bias[l * 32 + row] = B[co];
+        }
     }
 }
 
@@ -209,15 +224,12 @@ int64_t denoiser_params() {
     return n;
 }
 
-// Output and scratch of the denoiser for w*h pixels.
+// Output of the denoiser for w*h pixels.
 hipError_t ensure_denoise_buffers(oxr_ctx* c, size_t pixels) {
     if (pixels <= c->dn_pixels) return hipSuccess;
     dfree(c->denoised);
-    if (c->dn_scratch) (void)hipFree(c->dn_scratch);
-    c->dn_scratch = nullptr;
     c->dn_pixels = 0;
     hipError_t e = dalloc(&c->denoised, pixels);
-    if (e == hipSuccess) e = hipMalloc(&c->dn_scratch, pixels * 160);
     if (e == hipSuccess) c->dn_pixels = pixels;
     return e;
 }
@@ -353,7 +365,6 @@ int oxr_destroy(oxr_ctx* c) {
     dfree(c->depth);
     dfree(c->path_radiance); dfree(c->reqs); dfree(c->cover);
     dfree(c->dn_lanes); dfree(c->dn_bias); dfree(c->denoised);
-    if (c->dn_scratch) (void)hipFree(c->dn_scratch);
     free_present(c);
     for (hipEvent_t ev : {c->t0, c->t1, c->td0, c->td1, c->present_done[0], c->present_done[1]})
         if (ev) (void)hipEventDestroy(ev);
@@ -836,7 +847,7 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     if (prm->denoise) {
         OXR_HIP(c, ensure_denoise_buffers(c, (size_t)npix));
         OXR_HIP(c, hipEventRecord(c->td0, c->stream));
-        OXR_HIP(c, denoise_run(c->dn, c->color, c->albedo, c->normal, c->depth, W, H, c->dn_scratch, c->denoised,
+        OXR_HIP(c, denoise_run(c->dn, c->color, c->albedo, c->normal, c->depth, W, H, c->num_cu, c->denoised,
                                c->stream));
         OXR_HIP(c, hipEventRecord(c->td1, c->stream));
         OXR_HIP(c, tonemap(c->denoised, c->rgba, npix, c->stream));
@@ -892,6 +903,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     st.tail_tri_tests = tv[10];
     st.tail_extension_rays = tv[11];
     st.tail_shadow_rays = tv[12];
+    st.trace_lane_visits = tv[13];
+    st.trace_wave_slots = tv[14];
     st.paths = (uint64_t)prm->spp * npix;
     st.kernel_launches = launches;
     st.iterations = iterations;
@@ -1066,20 +1079,17 @@ int oxr_denoise_host(oxr_ctx* c, int w, int h, const float* color, const float* 
     const size_t n = (size_t)w * h;
     float4 *dc = nullptr, *da = nullptr, *dn = nullptr, *dout = nullptr;
     float* dd = nullptr;
-    void* scratch = nullptr;
     hipError_t e;
     if ((e = dalloc(&dc, n)) == hipSuccess && (e = dalloc(&da, n)) == hipSuccess && (e = dalloc(&dn, n)) == hipSuccess &&
         (e = dalloc(&dd, n)) == hipSuccess && (e = dalloc(&dout, n)) == hipSuccess &&
-        (e = hipMalloc(&scratch, denoise_scratch_bytes(w, h))) == hipSuccess &&
         (e = hipMemcpy(dc, color, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
         (e = hipMemcpy(da, albedo, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
         (e = hipMemcpy(dn, normal, 16 * n, hipMemcpyHostToDevice)) == hipSuccess &&
         (e = hipMemcpy(dd, depth, 4 * n, hipMemcpyHostToDevice)) == hipSuccess &&
-        (e = denoise_run(c->dn, dc, da, dn, dd, w, h, scratch, dout, c->stream)) == hipSuccess &&
+        (e = denoise_run(c->dn, dc, da, dn, dd, w, h, c->num_cu, dout, c->stream)) == hipSuccess &&
         (e = hipStreamSynchronize(c->stream)) == hipSuccess)
         e = hipMemcpy(out, dout, 16 * n, hipMemcpyDeviceToHost);
     dfree(dc); dfree(da); dfree(dn); dfree(dd); dfree(dout);
-    if (scratch) (void)hipFree(scratch);
     if (e != hipSuccess) return fail(c, OXR_ERR_HIP, std::string("denoise: ") + hipGetErrorString(e));
     return OXR_OK;
 }

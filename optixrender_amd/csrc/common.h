@@ -295,8 +295,11 @@ hipError_t wave_accumulate(const WaveParams& p, int64_t npix, hipStream_t s);
 // ---- denoiser (denoise.hip) ----
 // KPCNN 10 -> 32 -> 32 -> 32 -> 32 -> 25, 3x3 convolutions.  lanes: BF16
 // weights in MFMA lane order, layer l at lane_offset[l] (in 8-element groups):
-// for tap t, K step s, lane q, element e: W[co = q % 32][ci = 16 s + 8 (q / 32)
-// + e][t / 3][t % 3], zero where co or ci is padding.  bias: 5 x 32 floats.
+// for tap t, K step s, lane q, element e: W[co][ci = 16 s + 8 (q / 32) + e]
+// [t / 3][t % 3] with co the output channel of MFMA row q % 32 (the row
+// itself, except in the last layer: row 8 g + 4 h + i holds channel 4 g + i
+// for h = 0 and 13 + 4 g + i for h = 1), zero where co or ci is padding.
+// bias: 5 x 32 floats, by MFMA row.
 constexpr int kDnLayers = 5;
 constexpr int kDnChannels[kDnLayers + 1] = {10, 32, 32, 32, 32, 25};
 struct DenoiserDev {
@@ -304,9 +307,10 @@ struct DenoiserDev {
     int64_t lane_offset[kDnLayers];
     const float* bias;
 };
-size_t denoise_scratch_bytes(int w, int h);
+// The network in one kernel (activations in LDS, no scratch); cus: compute
+// units of the device (one block each at a time).
 hipError_t denoise_run(const DenoiserDev& d, const float4* color, const float4* albedo, const float4* normal,
-                       const float* depth, int w, int h, void* scratch, float4* out, hipStream_t s);
+                       const float* depth, int w, int h, int cus, float4* out, hipStream_t s);
 
 // Largest resident workgroups per CU of the trace instance for stack_lds,
 // and of shade / regen.

@@ -143,14 +143,13 @@ struct oxr_ctx {
     uint32_t* noalloc_scratch = nullptr;  // this device's copy of the fixed scratch (oxr_trace_rays_noalloc)
     uint8_t* cover = nullptr;      // camera-ray cover mask of single renders
     size_t cover_tiles = 0;        // its capacity
-    // denoiser: weights (oxr_denoiser_load), output and scratch of the last denoised render
+    // denoiser: weights (oxr_denoiser_load) and output of the last denoised render
     uint16_t* dn_lanes = nullptr;
     float* dn_bias = nullptr;
     oxr::DenoiserDev dn{};
     bool dn_loaded = false;
     float4* denoised = nullptr;
-    void* dn_scratch = nullptr;
-    size_t dn_pixels = 0;  // capacity of denoised / dn_scratch
+    size_t dn_pixels = 0;  // capacity of denoised
     bool denoised_valid = false;
     // streamed readback (OXR_FLAG_PRESENT): two page-locked frames written in
     // turn on copy_stream; present_done[k] marks frame k's copy complete

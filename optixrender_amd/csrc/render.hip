@@ -44,6 +44,18 @@ OXR_D void wave_count_add(unsigned long long* dst, uint32_t v) {
     if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
 
+OXR_D void wave_count_add64(unsigned long long* dst, unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane_id() == 0 && v) atomicAdd(dst, v);
+}
+
+OXR_D uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
 // ------------------------------------------------------------ ray queries
 template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(BvhView b, const float* __restrict__ rays, int64_t n,
@@ -518,23 +530,35 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
     Stack<S> st(lds, gstack, (int64_t)gridDim.x * blockDim.x);
     uint32_t nn = 0, nt = 0, sn = 0, stt = 0;
     uint32_t cn = 0, cnn = 0, cnt = 0;  // counting: rays the cover mask would resolve, their fetches / tests
+    // counting: SIMD use of the traversal loop -- node visits of active lanes,
+    // and 64 x the most visits of any lane per 64-ray chunk (the iterations
+    // the wave runs)
+    unsigned long long lane_visits = 0, wave_slots = 0;
     while (true) {
         const int j = wave_next(fetch);
         if (j >= chunks) break;
         if (j < c_shadow) {
             const bool light = j >= c_env;
             const int slot = ((light ? j - c_env : j) << 6) + lane_id();
+            uint32_t vis = 0;
             if (slot < (light ? n_light : n_env)) {
                 const uint32_t pid = shl[(light ? (size_t)p.pool : 0) + slot];
                 float4* rec = (light ? p.sh_light : p.sh_env) + 3 * (size_t)pid;
                 const float4 a = rec[0], b = rec[1];
+                const uint32_t sn0 = sn;
                 Hit h = traverse<true, COUNT>(p.bvh, xyz(a), xyz(b), kEps, a.w, st, sn, stt, overflow);
                 if (h.leaf != kNoHit) reinterpret_cast<float*>(rec + 2)[3] = 0.0f;
+                vis = (sn - sn0) >> 1;
+            }
+            if (COUNT) {
+                lane_visits += vis;
+                wave_slots += 64ull * wave_max_u32(vis);
             }
             continue;
         }
         const int64_t qp = sq.position(j - c_shadow, k, p.region);
         int cls = -1;
+        uint32_t vis = 0;
         if (qp >= 0) {
             f3 org, dir;
             const bool fresh = is_fresh(p, qp);
@@ -556,8 +580,13 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
                 cnn += nn - nn0;
                 cnt += nt - nt0;
             }
+            if (COUNT) vis = (nn - nn0) >> 1;
             p.hits[qp] = make_float4(h.t, __uint_as_float(h.leaf), h.u, h.v);
             cls = h.leaf != kNoHit ? 1 + p.mats[p.mat_ids[h.leaf]].type : fresh ? kClassCameraMiss : 0;
+        }
+        if (COUNT) {
+            lane_visits += vis;
+            wave_slots += 64ull * wave_max_u32(vis);
         }
         // list the queue position under its shade class: one append per
         // class present in the wave
@@ -578,6 +607,8 @@ __global__ __launch_bounds__(kTraceBlock, 8) void k_trace(WaveParams p, int y) {
         wave_count_add(&p.traversal[6], cn);
         wave_count_add(&p.traversal[7], cnn);
         wave_count_add(&p.traversal[8], cnt);
+        if (lane_id() == 0) atomicAdd(&p.traversal[14], wave_slots);  // wave-uniform
+        wave_count_add64(&p.traversal[13], lane_visits);
     }
 }
 

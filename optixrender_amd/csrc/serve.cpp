@@ -46,7 +46,7 @@ struct Request {
     float* depth = nullptr;
     uint32_t* rgba = nullptr;
     uint8_t* cover = nullptr;  // camera-ray cover mask (one byte per 8x8-pixel tile)
-    Clock::time_point submitted, finished;
+    Clock::time_point submitted, admitted, finished;
     std::string error;
     // The worker holds a request from submit until it has dropped it from the
     // pending queue, the active list and any tick (done, failed or cancelled
@@ -74,7 +74,6 @@ struct oxr_server {
     int trace_grid = 0, shade_grid = 0, sky_grid = 0, stack_lds = 32, iters_per_poll = 2;
     int64_t tail_paths = 0;
     WaveEngine eng;  // shared path pool; a tick is one sample window
-    void* dn_scratch = nullptr;  // denoiser activations for one frame of max_frame_pixels
     ReqDesc* reqs_dev = nullptr;
     ReqDesc* reqs_host = nullptr;  // pinned staging
     int max_reqs = 0;
@@ -169,6 +168,7 @@ void admit(oxr_server* s) {
         q->bytes = need;
         s->in_use += need;
         q->state = kActive;
+        q->admitted = Clock::now();
         s->pending.pop_front();
         s->active.push_back(q);
     }
@@ -232,8 +232,8 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
         if (q->frames_done + frames[i] < q->r.spp) continue;
         const int64_t np = (int64_t)q->r.width * q->r.height;
         if (q->r.denoise) {
-            if ((e = denoise_run(c->dn, q->color, q->albedo, q->normal, q->depth, q->r.width, q->r.height,
-                                 s->dn_scratch, q->denoised, s->stream)) != hipSuccess ||
+            if ((e = denoise_run(c->dn, q->color, q->albedo, q->normal, q->depth, q->r.width, q->r.height, c->num_cu,
+                                 q->denoised, s->stream)) != hipSuccess ||
                 (e = tonemap(q->denoised, q->rgba, np, s->stream)) != hipSuccess)
                 return e;
         } else if ((e = tonemap(q->color, q->rgba, np, s->stream)) != hipSuccess) {
@@ -359,7 +359,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->tail_paths = c->tuning.tail_paths;
     s->iters_per_poll = c->tuning.iters_per_poll;
     s->pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, s->cap));
-    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid) + denoise_scratch_bytes((int)cfg->max_frame_pixels, 1);
+    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid);
     auto bail = [&](int code, const char* m) {
         c->err = m;
         oxr_server_destroy(s);
@@ -368,8 +368,7 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     if (s->wave_bytes >= s->budget) return bail(OXR_ERR_INVALID, "server: budget smaller than the wavefront buffers");
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP, "stream");
     if (s->cap > ((int64_t)1 << 31) - 1) return bail(OXR_ERR_INVALID, "server: tick larger than 2^31 samples");
-    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs) ||
-        hipMalloc(&s->dn_scratch, std::max<size_t>(denoise_scratch_bytes((int)cfg->max_frame_pixels, 1), 256)))
+    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs))
         return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
     if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
         return bail(OXR_ERR_OOM, "server: pinned staging");
@@ -396,7 +395,6 @@ int oxr_server_destroy(oxr_server* s) {
     }
     wave_release(s->eng);
     sfree(s->reqs_dev);
-    if (s->dn_scratch) (void)hipFree(s->dn_scratch);
     if (s->reqs_host) (void)hipHostFree(s->reqs_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -519,6 +517,17 @@ int oxr_release_request(oxr_server* s, uint64_t id) {
     s->all.erase(q->id);
     delete q;
     s->cv.notify_all();
+    return OXR_OK;
+}
+
+int oxr_request_times(oxr_server* s, uint64_t id, double* queued_ms, double* total_ms) {
+    if (!s || !queued_ms || !total_ms) return OXR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* q = find_request(s, id);
+    if (!q) return OXR_ERR_INVALID;
+    if (q->state != kDone) return OXR_ERR_STATE;
+    *queued_ms = std::chrono::duration<double, std::milli>(q->admitted - q->submitted).count();
+    *total_ms = std::chrono::duration<double, std::milli>(q->finished - q->submitted).count();
     return OXR_OK;
 }
 

@@ -371,6 +371,12 @@ class Server:
         self.check(self._lib.oxr_release_request(self.h, rid), "oxr_release_request")
         self.sizes.pop(rid, None)
 
+    def times(self, rid):
+        """(queued ms, total ms) of a finished request."""
+        q, t = C.c_double(), C.c_double()
+        self.check(self._lib.oxr_request_times(self.h, rid, C.byref(q), C.byref(t)), "oxr_request_times")
+        return q.value, t.value
+
     def stats(self) -> abi.ServerStats:
         s = abi.ServerStats()
         self.check(self._lib.oxr_server_get_stats(self.h, C.byref(s)), "oxr_server_get_stats")

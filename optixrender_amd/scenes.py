@@ -27,6 +27,7 @@ PCG64 seeded per scene.
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -38,7 +39,13 @@ _PHI = (1.0 + math.sqrt(5.0)) / 2.0
 
 
 def icosphere(subdiv: int) -> np.ndarray:
-    """Unit icosphere as a triangle soup (20*4^s, 3, 3) float64, CCW outward."""
+    """Unit icosphere as a triangle soup (20*4^s, 3, 3) float64, CCW outward
+    (a fresh copy; the subdivision is computed once per level)."""
+    return _icosphere(subdiv).copy()
+
+
+@functools.lru_cache(maxsize=8)
+def _icosphere(subdiv: int) -> np.ndarray:
     v = np.array([[-1, _PHI, 0], [1, _PHI, 0], [-1, -_PHI, 0], [1, -_PHI, 0], [0, -1, _PHI], [0, 1, _PHI],
                   [0, -1, -_PHI], [0, 1, -_PHI], [_PHI, 0, -1], [_PHI, 0, 1], [-_PHI, 0, -1], [-_PHI, 0, 1]],
                  dtype=np.float64)
@@ -245,7 +252,13 @@ def sky_env(w: int = 2048, h: int = 1024, sun_peak: float = 5e4, sun_sigma_deg: 
     """Procedural equirect sky: gradient plus a Gaussian sun, (h, w, 4) float32.
 
     Texel (i, j) is centred at uv = ((j+0.5)/w, (i+0.5)/h) and the direction
-    follows PlaneToSphere (core/cuda/Sampling.h:12-23)."""
+    follows PlaneToSphere (core/cuda/Sampling.h:12-23).  A fresh copy of a
+    cached image."""
+    return _sky_env(w, h, sun_peak, sun_sigma_deg, tuple(sun_dir)).copy()
+
+
+@functools.lru_cache(maxsize=4)
+def _sky_env(w, h, sun_peak, sun_sigma_deg, sun_dir) -> np.ndarray:
     u = (np.arange(w) + 0.5) / w
     v = (np.arange(h) + 0.5) / h
     phi = 2.0 * math.pi * (u - 0.5)

@@ -1,14 +1,19 @@
 """Train the KPCNN denoiser on the path tracer's own renders.
 
-Data: seeded scenes from the generators (sphere fields under the HDR sky seen
-from random cameras, Cornell boxes with random sphere materials), each
-rendered by the HIP library twice -- a noisy input at one of `--spp-in`
-samples (colour, albedo, normal, depth AOVs) and a converged target at
-`--spp-ref` samples.  Training: PyTorch on the GPU, float32, L1 loss on log(1 + colour)
-over random crops, Adam.  Evaluation: held-out scenes, RMSE of log(1 + colour)
-of the plain input, of the denoised input, and of plain renders at more
-samples, to find the sample count plain accumulation needs to match the
-denoiser ("equal RMSE").
+Data: seeded scenes of the headline generator (`--generator sphere_field`:
+12 icospheres at subdivision 6 with seeded centres, radii and materials under
+the 2048x1024 HDR sky, seen from random cameras; `mixed` adds Cornell boxes
+with random sphere materials), each rendered by the HIP library twice -- a
+noisy input at one of `--spp-in` samples (colour, albedo, normal, depth AOVs)
+and a converged target at `--spp-ref` samples.  Training: PyTorch on the
+GPU, float32, L1 loss on log(1 + colour) over random crops, Adam.
+
+Evaluation: `--eval-scenes` held-out seeds at `--eval-res`, the input at
+`--eval-spp` samples denoised by the library's own BF16 kernel, against a
+`--spp-ref` plain render: RMSE of the displayed image (the tone map in float)
+and of log(1 + colour), beside plain accumulation at the input's and at
+more samples, and the plain sample count that matches the denoiser's
+display RMSE (log-log interpolation).
 
     python -m optixrender_amd.train_denoiser --out optixrender_amd/denoiser_kpcnn.npz
 
@@ -62,10 +67,10 @@ def cornell_variant(rng) -> SceneData:
     return sc
 
 
-def make_scene(seed: int, rng) -> SceneData:
-    if seed % 4 == 3:
+def make_scene(seed: int, rng, generator: str = "sphere_field") -> SceneData:
+    if generator == "mixed" and seed % 4 == 3:
         return cornell_variant(rng)
-    sc = scenes.sphere_field(seed=seed + 100, subdiv=4, env_w=1024, env_h=512)
+    sc = scenes.sphere_field(seed=seed + 100, subdiv=6)
     sc.camera = random_camera(rng)
     return sc
 
@@ -79,14 +84,14 @@ def render_aovs(ctx: Context, sid: int, sc: SceneData, w: int, h: int, spp: int,
             ctx.download_depth(w, h))
 
 
-def dataset(ctx, seeds, w, h, spp_in, spp_ref, depth):
+def dataset(ctx, seeds, w, h, spp_in, spp_ref, depth, generator):
     """Per scene: features (10, h, w), noisy colour (3, h, w), target (3, h, w).
     spp_in: the input sample counts, used in turn over the scenes."""
     out = []
     for i, seed in enumerate(seeds):
         spp = spp_in[i % len(spp_in)]
         rng = np.random.default_rng(seed)
-        sc = make_scene(seed, rng)
+        sc = make_scene(seed, rng, generator)
         sid = ctx.upload(sc)
         try:
             ctx.build_bvh(sid)
@@ -106,12 +111,56 @@ def lrmse(a, b):
     return denoise.log_rmse(a, b)
 
 
+def evaluate(ctx, layers, seeds, w, h, spp, spp_ref, depth, generator, plain_spp=(4, 16, 64, 256)):
+    """Held-out scenes at w x h: the library's denoiser on spp-sample inputs
+    against spp_ref plain renders, beside plain accumulation."""
+    from .serve_bench import display_rmse
+
+    ctx.load_denoiser(denoise.flat_weights(layers))
+    rows = []
+    for seed in seeds:
+        rng = np.random.default_rng(seed)
+        sc = make_scene(seed, rng, generator)
+        sid = ctx.upload(sc)
+        try:
+            ctx.build_bvh(sid)
+            ref = render_aovs(ctx, sid, sc, w, h, spp_ref, depth)[0]
+            c, a, n, d = render_aovs(ctx, sid, sc, w, h, spp, depth)
+            dn = ctx.denoise(c, a, n, d)
+            plain = {s: display_rmse(render_aovs(ctx, sid, sc, w, h, s, depth)[0], ref) for s in plain_spp}
+        finally:
+            ctx.release(sid)
+        rows.append({"seed": seed, "display_rmse_denoised": display_rmse(dn, ref),
+                     "display_rmse_plain": plain, "log_rmse_denoised": lrmse(dn, ref),
+                     "log_rmse_plain": lrmse(c, ref)})
+        log(f"eval seed {seed}: display RMSE denoised {rows[-1]['display_rmse_denoised']:.4f}, plain "
+            f"{ {k: round(v, 4) for k, v in plain.items()} }")
+    dn = float(np.mean([r["display_rmse_denoised"] for r in rows]))
+    pl = {s: float(np.mean([r["display_rmse_plain"][s] for r in rows])) for s in plain_spp}
+    xs, ys = np.log(np.array(plain_spp, float)), np.log(np.array([pl[s] for s in plain_spp]))
+    if math.log(dn) <= ys[-1]:
+        equal = f">{plain_spp[-1]}"
+    elif math.log(dn) >= ys[0]:
+        equal = f"<={plain_spp[0]}"
+    else:
+        i = int(np.searchsorted(-ys, -math.log(dn)))
+        f = (math.log(dn) - ys[i - 1]) / (ys[i] - ys[i - 1])
+        equal = round(float(math.exp(xs[i - 1] + f * (xs[i] - xs[i - 1]))), 1)
+    return {"eval_res": [w, h], "eval_seeds": list(seeds), "eval_spp": spp,
+            "eval_display_rmse_denoised": dn, "eval_display_rmse_plain": {str(k): v for k, v in pl.items()},
+            "eval_plain_spp_at_equal_display_rmse": equal,
+            "eval_log_rmse_plain": float(np.mean([r["log_rmse_plain"] for r in rows])),
+            "eval_log_rmse_denoised": float(np.mean([r["log_rmse_denoised"] for r in rows]))}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(denoise.WEIGHTS))
-    ap.add_argument("--scenes", type=int, default=128)
-    ap.add_argument("--eval-scenes", type=int, default=6)
-    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--generator", choices=["sphere_field", "mixed"], default="sphere_field")
+    ap.add_argument("--scenes", type=int, default=200)
+    ap.add_argument("--eval-scenes", type=int, default=20)
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--eval-res", default="1920x1080")
     ap.add_argument("--spp-in", default="2,4,8", help="input sample counts, used in turn over the training scenes")
     ap.add_argument("--eval-spp", type=int, default=4)
     ap.add_argument("--spp-ref", type=int, default=4096)
@@ -134,13 +183,13 @@ def main(argv=None):
     train_seeds = list(range(args.scenes))
     eval_seeds = list(range(1000, 1000 + args.eval_scenes))
     spp_in = [int(x) for x in args.spp_in.split(",")]
-    train = dataset(ctx, train_seeds, args.res, args.res, spp_in, args.spp_ref, args.depth)
-    held = dataset(ctx, eval_seeds, args.res, args.res, [args.eval_spp], args.spp_ref, args.depth)
+    train = dataset(ctx, train_seeds, args.res, args.res, spp_in, args.spp_ref, args.depth, args.generator)
     log(f"data ready in {time.perf_counter() - t0:.1f}s")
 
     feats = torch.from_numpy(np.stack([f for f, _, _, _ in train])).to(dev)
     noisy = torch.from_numpy(np.stack([c for _, c, _, _ in train])).to(dev)
     target = torch.from_numpy(np.stack([r for _, _, r, _ in train])).to(dev)
+    del train
     model = denoise.torch_model().to(dev)
     opt = torch.optim.Adam(model.parameters(), lr=args.lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=args.iters)
@@ -165,19 +214,14 @@ def main(argv=None):
         sched.step()
         if (it + 1) % 1000 == 0 or it == 0:
             log(f"step {it + 1}/{args.iters} loss {loss.item():.5f} ({time.perf_counter() - t1:.1f}s)")
+    del feats, noisy, target
 
     layers = denoise.torch_layers(model)
-    # held-out evaluation with the BF16 reference of the device kernel
-    ev = {"plain": [], "denoised": []}
-    for _, _, _, (c, a, nrm, d, ref) in held:
-        dn = denoise.reference_denoise(layers, c, a, nrm, d, bf16=True)
-        ev["plain"].append(lrmse(c, ref))
-        ev["denoised"].append(lrmse(dn, ref))
+    ew, eh = (int(v) for v in args.eval_res.split("x"))
+    ev = evaluate(ctx, layers, eval_seeds, ew, eh, args.eval_spp, args.spp_ref, args.depth, args.generator)
     meta = {
-        "train_scenes": args.scenes, "eval_scenes": args.eval_scenes, "res": args.res, "spp_in": spp_in,
-        "eval_spp": args.eval_spp,
-        "spp_ref": args.spp_ref, "iters": args.iters, "batch": args.batch, "crop": args.crop,
-        "eval_log_rmse_plain": float(np.mean(ev["plain"])), "eval_log_rmse_denoised": float(np.mean(ev["denoised"])),
+        "generator": args.generator, "train_scenes": args.scenes, "res": args.res, "spp_in": spp_in,
+        "spp_ref": args.spp_ref, "iters": args.iters, "batch": args.batch, "crop": args.crop, **ev,
         "seconds": round(time.perf_counter() - t0, 1),
     }
     log("eval " + json.dumps(meta))

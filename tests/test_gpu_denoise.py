@@ -56,6 +56,30 @@ def test_denoiser_matches_bf16_reference(ctx, aovs, seed, scale):
     assert np.all(got[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("w,h", [(203, 131), (57, 300)])
+def test_denoiser_matches_bf16_reference_across_strips_and_segments(ctx, w, h):
+    # several 56-column strips (the last one partial) and several row
+    # segments per strip, whose walks recompute the rows above them: every
+    # seam must be invisible
+    rng = np.random.default_rng(w)
+    col = np.ones((h, w, 4), np.float32)
+    col[..., :3] = rng.gamma(1.0, 0.6, (h, w, 3))
+    alb = np.ones((h, w, 4), np.float32)
+    alb[..., :3] = rng.uniform(0.05, 0.95, (h, w, 3))
+    nrm = np.zeros((h, w, 4), np.float32)
+    nrm[..., :3] = rng.normal(size=(h, w, 3))
+    nrm[..., :3] /= np.linalg.norm(nrm[..., :3], axis=-1, keepdims=True)
+    dep = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    layers = random_layers(7, 1.5)
+    ctx.load_denoiser(denoise.flat_weights(layers))
+    print(f"[denoise] random AOVs {w}x{h}", flush=True)
+    got = ctx.denoise(col, alb, nrm, dep)
+    ref = denoise.reference_denoise(layers, col, alb, nrm, dep, bf16=True)
+    err = np.abs(got[..., :3] - ref) / (np.abs(ref) + 1e-3)
+    assert np.mean(err <= 1e-2) >= 0.999, f"max rel err {err.max()}"
+    assert err.mean() < 1e-3
+
+
 def test_constant_image_is_a_fixed_point(ctx, aovs):
     # softmax weights sum to one, so any weights keep a constant colour
     _, _, (c, a, n, d) = aovs

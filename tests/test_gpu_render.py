@@ -95,8 +95,9 @@ def test_stage_timing_and_counts(ctx, cornell):
     # not bit-identical (device and host transcendentals differ by an ulp,
     # hence the per-path tolerance), and a last-bit change can move a leaf
     # box test, so whole-render totals agree to 1e-3.
-    for k, v in ref.items():
-        assert abs(got[k] - v) <= 1e-3 * v + 2, k
+    for k, v in got.items():
+        assert abs(v - ref[k]) <= 1e-3 * ref[k] + 2, k
+    assert ref["paths"] == W * H * 2
     assert st.trace_ms > 0 and st.shade_ms > 0
 
 
