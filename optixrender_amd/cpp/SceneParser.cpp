@@ -2,7 +2,10 @@
 
 #include <cctype>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -139,16 +142,122 @@ vec3f rotate_xyz(vec3f p, vec3f deg) {
 
 }  // namespace
 
+namespace oxr_json {
+namespace {
+
+// Little-endian BSON reader; i is the offset of the next byte.
+struct BsonReader {
+    const std::string& s;
+    size_t i = 0;
+    [[noreturn]] void fail(const std::string& what) {
+        throw std::runtime_error("bson: " + what + " at byte " + std::to_string(i));
+    }
+    void need(size_t n) {
+        if (n > s.size() - i) fail("unexpected end");
+    }
+    uint64_t le(int bytes) {
+        need((size_t)bytes);
+        uint64_t v = 0;
+        for (int k = 0; k < bytes; k++) v |= (uint64_t)(uint8_t)s[i + k] << (8 * k);
+        i += (size_t)bytes;
+        return v;
+    }
+    std::string cstring() {
+        const size_t end = s.find('\0', i);
+        if (end == std::string::npos) fail("unterminated element name");
+        std::string r = s.substr(i, end - i);
+        i = end + 1;
+        return r;
+    }
+    Value document(bool array, int depth) {
+        if (depth > 64) fail("nesting too deep");
+        const size_t start = i;
+        const uint64_t size = le(4);
+        if (size < 5 || size > s.size() - start) fail("bad document size");
+        const size_t end = start + (size_t)size;
+        Value v;
+        v.kind = array ? Value::Array : Value::Object;
+        for (;;) {
+            if (i >= end) fail("document runs past its size");
+            const uint8_t type = (uint8_t)le(1);
+            if (type == 0) break;
+            const std::string key = cstring();
+            Value e = element(type, depth);
+            if (i > end) fail("element runs past its document");
+            if (array) v.arr.push_back(std::move(e));
+            else v.obj[key] = std::move(e);
+        }
+        if (i != end) fail("document size does not match its elements");
+        return v;
+    }
+    Value element(uint8_t type, int depth) {
+        Value e;
+        switch (type) {
+        case 0x01: {                             // double
+            const uint64_t bits = le(8);
+            double d;
+            std::memcpy(&d, &bits, 8);
+            e.kind = Value::Number;
+            e.num = d;
+            break;
+        }
+        case 0x02: {                             // string: int32 length with the NUL, bytes, NUL
+            const uint64_t n = le(4);
+            if (n < 1 || n > 0x7fffffffu) fail("bad string length");
+            need((size_t)n);
+            if (s[i + n - 1] != '\0') fail("unterminated string");
+            e.kind = Value::String;
+            e.str = s.substr(i, (size_t)n - 1);
+            i += (size_t)n;
+            break;
+        }
+        case 0x03: e = document(false, depth + 1); break;
+        case 0x04: e = document(true, depth + 1); break;
+        case 0x08: {
+            const uint64_t b = le(1);
+            if (b > 1) fail("bad boolean");
+            e.kind = Value::Bool;
+            e.b = b == 1;
+            break;
+        }
+        case 0x0A: break;                        // null
+        case 0x10: e.kind = Value::Number; e.num = (double)(int32_t)(uint32_t)le(4); break;
+        case 0x11: e.kind = Value::Number; e.num = (double)le(8); break;
+        case 0x12: e.kind = Value::Number; e.num = (double)(int64_t)le(8); break;
+        default: {
+            char hex[8];
+            std::snprintf(hex, sizeof hex, "0x%02x", type);
+            i -= 1;
+            fail(std::string("unsupported element type ") + hex);
+        }
+        }
+        return e;
+    }
+};
+
+}  // namespace
+
+Value parse_bson(const std::string& bytes) {
+    BsonReader r{bytes};
+    Value v = r.document(false, 0);
+    if (r.i != bytes.size()) r.fail("bytes after the document");
+    return v;
+}
+
+}  // namespace oxr_json
+
 namespace {
 
 // The file's JSON, or false when it cannot be opened (SceneParser.h:31-35:
-// a missing file leaves is_succeed false).
+// a missing file leaves is_succeed false).  A name ending in "bson" is read
+// as BSON (SceneParser.h:13-21, 38-39).
 bool read_json(const std::string& fn, oxr_json::Value& out, std::string& baseDir) {
-    std::ifstream in(fn);
+    std::ifstream in(fn, std::ios::binary);
     if (!in.is_open()) return false;
     std::stringstream buf;
     buf << in.rdbuf();
-    out = oxr_json::parse(buf.str());
+    const bool bson = fn.size() >= 4 && fn.compare(fn.size() - 4, 4, "bson") == 0;
+    out = bson ? oxr_json::parse_bson(buf.str()) : oxr_json::parse(buf.str());
     size_t slash = fn.find_last_of('/');
     baseDir = slash == std::string::npos ? std::string() : fn.substr(0, slash + 1);
     return true;

@@ -68,6 +68,13 @@ struct Value {
 };
 // Throws std::runtime_error with the byte offset on malformed input.
 Value parse(const std::string& text);
+// The same from a BSON document (bsonspec.org, version 1.1), as the
+// reference reads a file whose name ends in "bson" (SceneParser.h:38-39,
+// nlohmann::json::from_bson): double, string, document, array, bool, null,
+// int32, uint64 timestamp and int64 elements (numbers become doubles, array
+// keys are ignored); any other element type, a size that does not match or
+// bytes after the document throw std::runtime_error with the byte offset.
+Value parse_bson(const std::string& bytes);
 }  // namespace oxr_json
 
 constexpr float kSmoothRoughness = 0.02f;

@@ -278,3 +278,81 @@ def test_scene_file_renders_like_the_python_mirror(tmp_path, ctx):
     print(f"[scene-json] mean {img.mean():.1f}, differing px {np.count_nonzero((rgb != img).any(-1))}", flush=True)
     assert img.mean() > 5
     assert np.array_equal(rgb, img)
+
+
+# ---- BSON scene files (SceneParser.h:38-39: a name ending in "bson" is read
+# with nlohmann::json::from_bson) ----
+
+def bson_encode(doc, int64=False):
+    """A BSON 1.1 encoder (bsonspec.org) for the JSON subset scene files use."""
+    def elem(key, v):
+        k = key.encode() + b"\0"
+        if isinstance(v, bool):
+            return b"\x08" + k + (b"\x01" if v else b"\x00")
+        if isinstance(v, int):
+            if not int64 and -2 ** 31 <= v < 2 ** 31:
+                return b"\x10" + k + struct.pack("<i", v)
+            return b"\x12" + k + struct.pack("<q", v)
+        if isinstance(v, float):
+            return b"\x01" + k + struct.pack("<d", v)
+        if isinstance(v, str):
+            s = v.encode() + b"\0"
+            return b"\x02" + k + struct.pack("<i", len(s)) + s
+        if v is None:
+            return b"\x0a" + k
+        if isinstance(v, dict):
+            return b"\x03" + k + document(v.items())
+        if isinstance(v, (list, tuple)):
+            return b"\x04" + k + document((str(i), x) for i, x in enumerate(v))
+        raise TypeError(type(v))
+
+    def document(items):
+        body = b"".join(elem(k, v) for k, v in items)
+        return struct.pack("<i", len(body) + 5) + body + b"\0"
+
+    return document(doc.items())
+
+
+def test_bson_encoder_matches_the_specification_examples():
+    # the two example documents of bsonspec.org's FAQ, byte for byte
+    assert bson_encode({"hello": "world"}) == b"\x16\x00\x00\x00\x02hello\x00\x06\x00\x00\x00world\x00\x00"
+    assert bson_encode({"BSON": ["awesome", 5.05, 1986]}) == (
+        b"\x31\x00\x00\x00\x04BSON\x00\x26\x00\x00\x00\x020\x00\x08\x00\x00\x00awesome\x00"
+        b"\x011\x00\x33\x33\x33\x33\x33\x33\x14\x40\x102\x00\xc2\x07\x00\x00\x00\x00")
+
+
+def dump_bson(tmp_path, data, name="scene.bson"):
+    (tmp_path / name).write_bytes(data)
+    out = tmp_path / "dump.json"
+    if out.exists():
+        out.unlink()
+    env = dict(os.environ, OXR_HEADLESS_DUMP=str(out))
+    r = subprocess.run([str(HEADLESS), str(tmp_path / name), str(tmp_path / "x.ppm"), "1"],
+                       capture_output=True, text=True, timeout=60, env=env)
+    return r, (json.loads(out.read_text()) if r.returncode == 0 else None)
+
+
+@pytest.mark.parametrize("int64", [False, True])
+def test_bson_scene_reads_like_its_json(tmp_path, int64):
+    build_files(tmp_path)
+    sc = base_scene()
+    sc["comment"] = None          # null, and an element nothing reads
+    r, want = dump(tmp_path, sc)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r, got = dump_bson(tmp_path, bson_encode(sc, int64=int64))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert got == want
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda b: b[:-1], "bad document size"),                                # truncated file
+    (lambda b: b + b"\0", "bytes after the document"),
+    (lambda b: struct.pack("<i", len(b) + 4) + b[4:], "bad document size"),
+    (lambda b: b[:4] + b"\x05" + b[5:], "unsupported element type 0x05"),   # first element as binary
+    (lambda b: b[:-8] + struct.pack("<i", 100) + b[-4:], "unexpected end"),  # string longer than the file
+])
+def test_bad_bson_is_reported(tmp_path, mutate, msg):
+    build_files(tmp_path)
+    data = bson_encode({"width": 48, "height": 32, "s": "xy"})   # ends with the string element
+    r, _ = dump_bson(tmp_path, mutate(data))
+    assert r.returncode == 1 and "bson: " + msg in r.stdout, r.stdout + r.stderr
