@@ -50,7 +50,15 @@ OXR_D RayPre prep_ray(f3 o, f3 d) {
     return r;
 }
 
-OXR_D bool tri_test(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float& t, float& u, float& v) {
+// The sheared edge functions U, V, W, their sum det and the scaled distance
+// T of the watertight test; the hit distance is T / det, the barycentrics
+// V / det and W / det.
+struct TriEdges {
+    float V, W, det, T;
+    bool inside;
+};
+
+OXR_D TriEdges tri_edges(const RayPre& r, float4 a, float4 b, float4 c) {
     f3 A = xyz(a) - r.o, B = xyz(b) - r.o, C = xyz(c) - r.o;
     const int kx = r.kx(), ky = r.ky(), kz = r.kz();
     float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
@@ -58,15 +66,30 @@ OXR_D bool tri_test(const RayPre& r, float4 a, float4 b, float4 c, float tmin, f
     float Bx = comp(B, kx) - r.Sx * Bkz, By = comp(B, ky) - r.Sy * Bkz;
     float Cx = comp(C, kx) - r.Sx * Ckz, Cy = comp(C, ky) - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-    float det = U + V + W;
-    if (det == 0.0f) return false;
-    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
-    t = T / det;
-    if (!(t > tmin)) return false;
-    u = V / det;
-    v = W / det;
-    return true;
+    TriEdges e;
+    e.V = V;
+    e.W = W;
+    e.inside = !((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f));
+    e.det = U + V + W;
+    e.T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    return e;
+}
+
+// Hit distance only: the traversal compares t and keeps the leaf; the
+// barycentrics of the one hit it returns are derived afterwards by tri_uv
+// from the same arithmetic, so two divisions run once per ray instead of
+// once per candidate triangle (and never for an any-hit ray).
+OXR_D bool tri_hit_t(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float& t) {
+    const TriEdges e = tri_edges(r, a, b, c);
+    if (!e.inside || e.det == 0.0f) return false;
+    t = e.T / e.det;
+    return t > tmin;
+}
+
+OXR_D void tri_uv(const RayPre& r, float4 a, float4 b, float4 c, float& u, float& v) {
+    const TriEdges e = tri_edges(r, a, b, c);
+    u = e.V / e.det;
+    v = e.W / e.det;
 }
 
 OXR_D bool box_test(const RayPre& r, float4 lo, float4 hi, float tmin, float tbest, float& tn) {
@@ -147,15 +170,23 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
         const float4* tr = b.tris + 3 * (size_t)k;
         float4 a = tr[0], bb = tr[1], c = tr[2];
         if (COUNT) nt++;
-        float t, u, v;
-        if (!tri_test(r, a, bb, c, tmin, t, u, v)) return false;
+        float t;
+        if (!tri_hit_t(r, a, bb, c, tmin, t)) return false;
         uint32_t id = __float_as_uint(a.w);
         bool take = ANY ? (t < tmax) : (t < h.t || (t == h.t && h.leaf != kNoHit && id < h.id));
-        if (take) { h.t = t; h.leaf = k; h.id = id; h.u = u; h.v = v; }
+        if (take) { h.t = t; h.leaf = k; h.id = id; }
         return take;
+    };
+    // barycentrics of the closest hit, recomputed from its triangle record
+    auto finish = [&]() {
+        if (!ANY && h.leaf != kNoHit) {
+            const float4* tr = b.tris + 3 * (size_t)h.leaf;
+            tri_uv(r, tr[0], tr[1], tr[2], h.u, h.v);
+        }
     };
     if (b.n == 1) {
         leaf_test(0);
+        finish();
         return h;
     }
     st.sp = 0;
@@ -202,6 +233,7 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
             node = st.pop();
         }
     }
+    finish();
     return h;
 }
 
