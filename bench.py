@@ -194,7 +194,9 @@ def serving(ctx, args):
         for _, s, _ in resident:
             ctx.release(s)
     res.update({"plain_p50_ms": res["plain"]["p50_ms"], "plain_p99_ms": res["plain"]["p99_ms"],
-                "denoised_p50_ms": res["denoised"]["p50_ms"], "denoised_p99_ms": res["denoised"]["p99_ms"]})
+                "denoised_p50_ms": res["denoised"]["p50_ms"], "denoised_p99_ms": res["denoised"]["p99_ms"],
+                "plain_sustained_p50_ms": res["plain_sustained"]["p50_ms"],
+                "plain_sustained_p99_ms": res["plain_sustained"]["p99_ms"]})
     return res
 
 

@@ -246,17 +246,19 @@ int oxr_download_path_radiance(oxr_ctx* ctx, float* out, size_t count);
 
 /* ---- render server: continuous batching over resident scenes ----
    No counterpart in the reference (its viewer renders one frame per Render()
-   call, main.cpp:66-128).  A scheduler thread admits requests FIFO while a
-   slot is free and their buffers fit in the HBM budget, and runs the active
-   requests of one (scene, max_depth, rr_start, camera medium) group through
-   shared wavefronts, handing out frames round-robin up to tick_paths paths
-   per tick.  A request's image equals a single oxr_render of the same
-   request (non-progressive, frames 0..spp-1). */
+   call, main.cpp:66-128).  Runner threads, each with its own path pool and
+   stream, admit requests FIFO while a slot is free and their buffers fit in
+   the HBM budget; each runs the active requests of one (scene, max_depth,
+   rr_start, camera medium) group that no other runner holds through a shared
+   wavefront, handing out frames round-robin up to tick_paths paths per tick,
+   so the ticks of different groups (other scenes, or more requests of a busy
+   one) overlap on the device.  A request's image equals a single oxr_render
+   of the same request (non-progressive, frames 0..spp-1). */
 typedef struct oxr_server oxr_server;
 typedef struct oxr_server_config {
     size_t hbm_budget_bytes;   /* wavefront buffers + all admitted requests' buffers */
     int32_t max_active;        /* admitted requests at a time (<= 256) */
-    int32_t reserved;
+    int32_t runners;           /* ticks in flight at once, each with its own path pool (0..8; 0 = 2) */
     int64_t tick_paths;        /* paths per shared wavefront tick */
     int64_t max_frame_pixels;  /* largest request frame accepted (width*height) */
 } oxr_server_config;
@@ -278,6 +280,8 @@ typedef struct oxr_server_stats {
     double p50_ms, p99_ms, mean_ms;  /* submit -> done latency of completed requests */
     int32_t active, pending;
     size_t bytes_in_use, budget_bytes;
+    uint64_t overlapped_ticks;  /* ticks that began while another runner's tick was running */
+    int32_t runners, max_running;  /* runner threads; most ticks in flight at once */
 } oxr_server_stats;
 int oxr_server_create(oxr_ctx* ctx, const oxr_server_config* cfg, oxr_server** out);
 int oxr_server_destroy(oxr_server* s);

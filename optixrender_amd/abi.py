@@ -234,7 +234,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_size_t),
         ("max_active", C.c_int32),
-        ("reserved", C.c_int32),
+        ("runners", C.c_int32),
         ("tick_paths", C.c_int64),
         ("max_frame_pixels", C.c_int64),
     ]
@@ -262,6 +262,7 @@ class ServerStats(C.Structure):
         ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
         ("active", C.c_int32), ("pending", C.c_int32),
         ("bytes_in_use", C.c_size_t), ("budget_bytes", C.c_size_t),
+        ("overlapped_ticks", C.c_uint64), ("runners", C.c_int32), ("max_running", C.c_int32),
     ]
 
     def as_dict(self):

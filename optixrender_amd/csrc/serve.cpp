@@ -3,18 +3,24 @@
 // The reference renders one frame per Render() call from the viewer loop
 // (main.cpp:66-128); this server is new.  Scenes and their BVHs stay resident
 // in the context; each request gets its own accumulation buffers (colour,
-// albedo, normal, RGBA8).  A scheduler thread repeatedly
-//   1. admits pending requests (FIFO) while a slot is free and their buffers
-//      fit in the HBM budget left over by the shared wavefront buffers,
-//   2. takes the active requests that share (scene, max depth, RR start,
-//      camera medium) with the oldest one and hands out frames round-robin up
-//      to the tick's path budget (every request gets at least one frame),
-//   3. runs one shared sample window over all of them on the path pool
-//      (extend, shade, connect, regenerate until every sample is done, then
-//      accumulate) and tone-maps the requests that finished, recording their
-//      latency.
-// A path is (pixel, frameID) of its own request, so a request's image does
-// not depend on what it shared the device with.
+// albedo, normal, RGBA8).  Several runners -- threads, each with its own path
+// pool and HIP stream -- repeatedly
+//   1. admit pending requests (FIFO) while a slot is free and their buffers
+//      fit in the HBM budget left over by the runners' wavefront buffers,
+//   2. take the oldest active request that no other runner holds, with the
+//      unheld active requests that share its (scene, max depth, RR start,
+//      camera medium), and hand out frames round-robin up to the tick's path
+//      budget (every request gets at least one frame),
+//   3. run one shared sample window over all of them on the runner's path
+//      pool (extend, shade, connect, regenerate until every sample is done,
+//      then accumulate) and tone-map the requests that finished, recording
+//      their latency.
+// So the ticks of different groups -- requests over other scenes, or the
+// next requests of a busy scene -- run at the same time on the device, each
+// on its own stream, and a runner picks up new work as soon as its tick
+// ends, without waiting for the others.  A path is (pixel, frameID) of its
+// own request, so a request's image does not depend on what it shared the
+// device with.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +39,9 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
+constexpr int kMaxRunners = 8;
+constexpr int kDefaultRunners = 2;
+
 enum State { kPending = 0, kActive = 1, kDone = 2, kCancelled = 3, kFailed = 4 };
 
 struct Request {
@@ -46,6 +55,8 @@ struct Request {
     float* depth = nullptr;
     uint32_t* rgba = nullptr;
     uint8_t* cover = nullptr;  // camera-ray cover mask (one byte per 8x8-pixel tile)
+    bool cleared = false;      // accumulation buffers zeroed (by the stream of its first tick)
+    bool in_tick = false;      // a runner's tick is rendering it
     Clock::time_point submitted, admitted, finished;
     std::string error;
     // The worker holds a request from submit until it has dropped it from the
@@ -65,17 +76,24 @@ size_t request_bytes(const oxr_request& r) {
 
 }  // namespace
 
+// A runner: a thread with its own path pool and stream; its tick is one
+// sample window over one group of requests.
+struct Runner {
+    WaveEngine eng;
+    hipStream_t stream = nullptr;
+    ReqDesc* reqs_dev = nullptr;
+    ReqDesc* reqs_host = nullptr;  // pinned staging
+    std::thread thread;
+};
+
 struct oxr_server {
     oxr_ctx* ctx = nullptr;
-    hipStream_t stream = nullptr;
-    size_t budget = 0, wave_bytes = 0, in_use = 0;
+    size_t budget = 0, wave_bytes = 0, in_use = 0;  // wave_bytes: all runners' wavefront buffers
     int max_active = 0;
     int64_t tick_paths = 0, cap = 0, pool = 0;
     int trace_grid = 0, shade_grid = 0, sky_grid = 0, stack_lds = 32, iters_per_poll = 2;
     int64_t tail_paths = 0;
-    WaveEngine eng;  // shared path pool; a tick is one sample window
-    ReqDesc* reqs_dev = nullptr;
-    ReqDesc* reqs_host = nullptr;  // pinned staging
+    std::vector<Runner*> runners;
     int max_reqs = 0;
     // requests
     std::mutex mu;
@@ -85,9 +103,10 @@ struct oxr_server {
     std::unordered_map<uint64_t, Request*> all;
     uint64_t next_id = 1;
     bool stop = false;
-    std::thread worker;
     // statistics
     uint64_t submitted = 0, completed = 0, cancelled = 0, rejected = 0, ticks = 0, paths = 0;
+    uint64_t overlapped = 0;          // ticks that began while another runner's tick ran
+    int running = 0, max_running = 0;  // ticks in progress, and the most at once
     double busy_ms = 0.0;
     std::vector<double> latencies;
     Clock::time_point started;
@@ -152,11 +171,7 @@ void admit(oxr_server* s) {
             (e = salloc(&q->normal, np)) != hipSuccess || (e = salloc(&q->rgba, np)) != hipSuccess ||
             (e = salloc(&q->depth, np)) != hipSuccess ||
             (e = salloc(&q->cover, (size_t)cover_tiles(q->r.width) * cover_tiles(q->r.height))) != hipSuccess ||
-            (q->r.denoise && (e = salloc(&q->denoised, np)) != hipSuccess) ||
-            (e = hipMemsetAsync(q->depth, 0, 4 * np, s->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(q->color, 0, 16 * np, s->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(q->albedo, 0, 16 * np, s->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(q->normal, 0, 16 * np, s->stream)) != hipSuccess) {
+            (q->r.denoise && (e = salloc(&q->denoised, np)) != hipSuccess)) {
             free_request_buffers(s, q);
             q->state = kFailed;
             q->error = std::string("allocation: ") + hipGetErrorString(e);
@@ -174,14 +189,27 @@ void admit(oxr_server* s) {
     }
 }
 
-hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int>& frames) {
+// One tick on runner rn (without the lock; the group's requests are marked
+// in_tick, so nothing else touches them).  Returns the paths it rendered.
+hipError_t run_tick(oxr_server* s, Runner& rn, std::vector<Request*>& group, std::vector<int>& frames,
+                    int64_t& paths) {
     oxr_ctx* c = s->ctx;
     SceneDev* scene = group[0]->scene;
     const oxr_request& r0 = group[0]->r;
     int64_t path = 0, pix = 0;
+    hipError_t e = hipSuccess;
     for (size_t i = 0; i < group.size(); i++) {
         Request* q = group[i];
-        ReqDesc& d = s->reqs_host[i];
+        if (!q->cleared) {  // first tick of the request: its running means start at zero
+            const size_t np = (size_t)q->r.width * q->r.height;
+            if ((e = hipMemsetAsync(q->depth, 0, 4 * np, rn.stream)) != hipSuccess ||
+                (e = hipMemsetAsync(q->color, 0, 16 * np, rn.stream)) != hipSuccess ||
+                (e = hipMemsetAsync(q->albedo, 0, 16 * np, rn.stream)) != hipSuccess ||
+                (e = hipMemsetAsync(q->normal, 0, 16 * np, rn.stream)) != hipSuccess)
+                return e;
+            q->cleared = true;
+        }
+        ReqDesc& d = rn.reqs_host[i];
         std::memset(&d, 0, sizeof(d));
         d.path_begin = path;
         d.pix_begin = pix;
@@ -205,16 +233,15 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
         path += np * frames[i];
         pix += np;
     }
-    hipError_t e = hipMemcpyAsync(s->reqs_dev, s->reqs_host, sizeof(ReqDesc) * group.size(), hipMemcpyHostToDevice,
-                                  s->stream);
+    e = hipMemcpyAsync(rn.reqs_dev, rn.reqs_host, sizeof(ReqDesc) * group.size(), hipMemcpyHostToDevice, rn.stream);
     if (e != hipSuccess) return e;
     WaveParams p;
     fill_scene_params(p, scene, c->lut_bsdf, c->lut_albedo);
-    wave_bind(s->eng, std::min(s->pool, wave_round_pool(path)), p);
+    wave_bind(rn.eng, std::min(s->pool, wave_round_pool(path)), p);
     p.cam_medium = r0.camera.medium;
     p.max_depth = r0.max_depth > 0 ? r0.max_depth : 25;
     p.rr_start = r0.rr_start;
-    p.reqs = s->reqs_dev;
+    p.reqs = rn.reqs_dev;
     p.nreq = (int)group.size();
     p.stack_lds = s->stack_lds;
     p.resolve = wave_resolve_mode(p, false);
@@ -226,32 +253,32 @@ hipError_t run_tick(oxr_server* s, std::vector<Request*>& group, std::vector<int
     run.iters_per_poll = s->iters_per_poll;
     run.total_samples = path;
     run.npix_total = pix;
-    if ((e = wave_run_window(s->eng, p, run, s->stream)) != hipSuccess) return e;
+    if ((e = wave_run_window(rn.eng, p, run, rn.stream)) != hipSuccess) return e;
     for (size_t i = 0; i < group.size(); i++) {
         Request* q = group[i];
         if (q->frames_done + frames[i] < q->r.spp) continue;
         const int64_t np = (int64_t)q->r.width * q->r.height;
         if (q->r.denoise) {
             if ((e = denoise_run(c->dn, q->color, q->albedo, q->normal, q->depth, q->r.width, q->r.height, c->num_cu,
-                                 q->denoised, s->stream)) != hipSuccess ||
-                (e = tonemap(q->denoised, q->rgba, np, s->stream)) != hipSuccess)
+                                 q->denoised, rn.stream)) != hipSuccess ||
+                (e = tonemap(q->denoised, q->rgba, np, rn.stream)) != hipSuccess)
                 return e;
-        } else if ((e = tonemap(q->color, q->rgba, np, s->stream)) != hipSuccess) {
+        } else if ((e = tonemap(q->color, q->rgba, np, rn.stream)) != hipSuccess) {
             return e;
         }
     }
-    s->paths += (uint64_t)path;
-    return hipStreamSynchronize(s->stream);
+    paths = path;
+    return hipStreamSynchronize(rn.stream);
 }
 
-void worker_loop(oxr_server* s) {
+void runner_loop(oxr_server* s, Runner* rn) {
     (void)hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (!s->stop) {
         admit(s);
-        // drop cancelled active requests
+        // drop cancelled active requests that no tick is rendering
         for (auto it = s->active.begin(); it != s->active.end();) {
-            if ((*it)->state == kCancelled) {
+            if ((*it)->state == kCancelled && !(*it)->in_tick) {
                 Request* q = *it;
                 free_request_buffers(s, q);
                 it = s->active.erase(it);
@@ -260,14 +287,18 @@ void worker_loop(oxr_server* s) {
                 ++it;
             }
         }
-        if (s->active.empty()) {
+        // the group of the oldest active request no tick is rendering
+        Request* lead = nullptr;
+        for (Request* q : s->active)
+            if (!q->in_tick && q->state == kActive) { lead = q; break; }
+        if (!lead) {
             s->cv.wait_for(lk, std::chrono::milliseconds(50));
             continue;
         }
-        // the group of the oldest active request, frames handed out round-robin
         std::vector<Request*> group;
         for (Request* q : s->active)
-            if (same_group(q, s->active[0]) && (int)group.size() < s->max_reqs) group.push_back(q);
+            if (!q->in_tick && q->state == kActive && same_group(q, lead) && (int)group.size() < s->max_reqs)
+                group.push_back(q);
         // Frames round-robin within the tick's path budget; only the oldest
         // request may overshoot it (by at most one of its frames, which the
         // wavefront capacity covers).  Requests left without frames sit out.
@@ -305,15 +336,23 @@ void worker_loop(oxr_server* s) {
             s->done_cv.notify_all();
             continue;
         }
+        for (Request* q : group) q->in_tick = true;
+        if (s->running > 0) s->overlapped++;
+        s->running++;
+        s->max_running = std::max(s->max_running, s->running);
         lk.unlock();
         auto t0 = Clock::now();
-        hipError_t e = run_tick(s, group, frames);
+        int64_t paths = 0;
+        hipError_t e = run_tick(s, *rn, group, frames, paths);
         auto t1 = Clock::now();
         lk.lock();
+        s->running--;
         s->ticks++;
+        s->paths += (uint64_t)paths;
         s->busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
         for (size_t i = 0; i < group.size(); i++) {
             Request* q = group[i];
+            q->in_tick = false;
             if (e != hipSuccess) {
                 q->state = kFailed;
                 q->error = std::string("render tick: ") + hipGetErrorString(e);
@@ -329,6 +368,7 @@ void worker_loop(oxr_server* s) {
             let_go(s, q);
         }
         s->done_cv.notify_all();
+        s->cv.notify_all();  // requests of this tick that want more frames are free for any runner
     }
 }
 
@@ -340,8 +380,9 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     if (!c || !cfg || !out) return OXR_ERR_INVALID;
     *out = nullptr;
     if (cfg->max_active < 1 || cfg->tick_paths < 1 || cfg->max_active > 256 || cfg->max_frame_pixels < 1 ||
-        cfg->max_frame_pixels > (1LL << 30)) {
-        c->err = "server: max_active in [1, 256], tick_paths >= 1 and max_frame_pixels in [1, 2^30] required";
+        cfg->max_frame_pixels > (1LL << 30) || cfg->runners < 0 || cfg->runners > kMaxRunners) {
+        c->err = "server: max_active in [1, 256], tick_paths >= 1, max_frame_pixels in [1, 2^30] and runners in "
+                 "[0, 8] required";
         return OXR_ERR_INVALID;
     }
     if (hipSetDevice(c->device) != hipSuccess) return OXR_ERR_HIP;
@@ -359,21 +400,27 @@ int oxr_server_create(oxr_ctx* c, const oxr_server_config* cfg, oxr_server** out
     s->tail_paths = c->tuning.tail_paths;
     s->iters_per_poll = c->tuning.iters_per_poll;
     s->pool = wave_round_pool(std::min<int64_t>(c->tuning.pool_paths, s->cap));
-    s->wave_bytes = wave_bytes(s->pool, s->cap, s->trace_grid);
+    const int nr = cfg->runners > 0 ? cfg->runners : kDefaultRunners;
+    s->wave_bytes = (size_t)nr * wave_bytes(s->pool, s->cap, s->trace_grid);
     auto bail = [&](int code, const char* m) {
         c->err = m;
         oxr_server_destroy(s);
         return code;
     };
     if (s->wave_bytes >= s->budget) return bail(OXR_ERR_INVALID, "server: budget smaller than the wavefront buffers");
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(OXR_ERR_HIP, "stream");
     if (s->cap > ((int64_t)1 << 31) - 1) return bail(OXR_ERR_INVALID, "server: tick larger than 2^31 samples");
-    if (wave_reserve(s->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&s->reqs_dev, s->max_reqs))
-        return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
-    if (hipHostMalloc((void**)&s->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
-        return bail(OXR_ERR_OOM, "server: pinned staging");
+    for (int i = 0; i < nr; i++) {
+        Runner* rn = new Runner();
+        s->runners.push_back(rn);
+        if (hipStreamCreateWithFlags(&rn->stream, hipStreamNonBlocking) != hipSuccess)
+            return bail(OXR_ERR_HIP, "server: stream");
+        if (wave_reserve(rn->eng, s->pool, s->cap, s->trace_grid) != hipSuccess || salloc(&rn->reqs_dev, s->max_reqs))
+            return bail(OXR_ERR_OOM, "server: wavefront allocation failed");
+        if (hipHostMalloc((void**)&rn->reqs_host, sizeof(ReqDesc) * s->max_reqs, 0) != hipSuccess)
+            return bail(OXR_ERR_OOM, "server: pinned staging");
+    }
     s->started = Clock::now();
-    s->worker = std::thread(worker_loop, s);
+    for (Runner* rn : s->runners) rn->thread = std::thread(runner_loop, s, rn);
     *out = s;
     return OXR_OK;
 }
@@ -385,18 +432,23 @@ int oxr_server_destroy(oxr_server* s) {
         s->stop = true;
     }
     s->cv.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (Runner* rn : s->runners)
+        if (rn->thread.joinable()) rn->thread.join();
     (void)hipSetDevice(s->ctx->device);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (Runner* rn : s->runners)
+        if (rn->stream) (void)hipStreamSynchronize(rn->stream);
     for (auto& kv : s->all) {
         if (kv.second->held) kv.second->scene->server_refs.fetch_sub(1);
         free_request_buffers(s, kv.second);
         delete kv.second;
     }
-    wave_release(s->eng);
-    sfree(s->reqs_dev);
-    if (s->reqs_host) (void)hipHostFree(s->reqs_host);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    for (Runner* rn : s->runners) {
+        wave_release(rn->eng);
+        sfree(rn->reqs_dev);
+        if (rn->reqs_host) (void)hipHostFree(rn->reqs_host);
+        if (rn->stream) (void)hipStreamDestroy(rn->stream);
+        delete rn;
+    }
     delete s;
     return OXR_OK;
 }
@@ -547,6 +599,9 @@ int oxr_server_get_stats(oxr_server* s, oxr_server_stats* out) {
     out->pending = (int32_t)s->pending.size();
     out->bytes_in_use = s->in_use + s->wave_bytes;
     out->budget_bytes = s->budget;
+    out->overlapped_ticks = s->overlapped;
+    out->runners = (int32_t)s->runners.size();
+    out->max_running = s->max_running;
     if (!s->latencies.empty()) {
         std::vector<double> l = s->latencies;
         std::sort(l.begin(), l.end());

@@ -311,10 +311,10 @@ class Server:
     """Continuous-batching render server over a context's resident scenes (oxr_server_*)."""
 
     def __init__(self, ctx: Context, budget_bytes: int, max_active: int = 16, tick_paths: int = 1 << 24,
-                 max_frame_pixels: int = 1920 * 1080):
+                 max_frame_pixels: int = 1920 * 1080, runners: int = 0):
         self.ctx = ctx
         self._lib = ctx._lib
-        cfg = abi.ServerConfig(budget_bytes, max_active, 0, tick_paths, max_frame_pixels)
+        cfg = abi.ServerConfig(budget_bytes, max_active, runners, tick_paths, max_frame_pixels)
         h = C.c_void_p()
         ctx.check(self._lib.oxr_server_create(ctx.h, C.byref(cfg), C.byref(h)), "oxr_server_create")
         self.h = h

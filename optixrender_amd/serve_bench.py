@@ -19,7 +19,9 @@ plain accumulation on that scene at that sample count -- it is served plain,
 the denoiser's fallback.
 
 Two arms over the same request stream: plain (every request at its spp) and
-denoised (the plan above).  Latency is the server's own submit-to-done time
+denoised (the plan above); then plain again over a quarter as many requests
+at three quarters of the rate the plain arm actually served, a load it
+sustains, so that its latency is render time rather than queueing.  Latency is the server's own submit-to-done time
 per request; the time spent waiting for admission is reported beside it.
 Each arm's offered load is compared with the scenes' single-render
 throughput: an arm whose offered paths per second exceed what the GPU renders
@@ -174,7 +176,7 @@ def percentile(v, q):
 
 
 def run_arm(ctx: Context, resident, stream, plan_of, budget_gb: float = 160.0, tick_paths: int = 1 << 26,
-            depth: int = DEPTH):
+            depth: int = DEPTH, runners: int = 4):
     """Submit the stream at its arrival times; plan_of(request) -> (denoise,
     spp).  A reaper thread releases every finished request as soon as it is
     done (a finished request holds its buffers, and so its share of the HBM
@@ -183,7 +185,7 @@ def run_arm(ctx: Context, resident, stream, plan_of, budget_gb: float = 160.0, t
     import threading
 
     srv = Server(ctx, budget_bytes=int(budget_gb * (1 << 30)), max_active=64, tick_paths=tick_paths,
-                 max_frame_pixels=1920 * 1080)
+                 max_frame_pixels=1920 * 1080, runners=runners)
     meta = {}            # id -> (scene name, denoised)
     times = {}           # id -> (queued ms, total ms)
     lock = threading.Lock()
@@ -253,6 +255,8 @@ def run_arm(ctx: Context, resident, stream, plan_of, budget_gb: float = 160.0, t
     return {"requests": len(times), "p50_ms": round(percentile(total, 0.5), 2), "p99_ms": round(percentile(total, 0.99), 2),
             "mean_ms": round(float(np.mean(total)), 2), "queued_p50_ms": round(percentile(queued, 0.5), 2),
             "queued_p99_ms": round(percentile(queued, 0.99), 2), "wall_s": round(wall, 2), "ticks": int(st.ticks),
+            "runners": int(st.runners), "overlapped_ticks": int(st.overlapped_ticks),
+            "max_ticks_at_once": int(st.max_running),
             "Mpaths_s": round(paths / wall / 1e6, 1), "paths": int(paths),
             "denoised_requests": int(sum(1 for _, dn in meta.values() if dn)),
             "per_scene_p50_p99_ms": {k: [round(percentile(v, 0.5), 1), round(percentile(v, 0.99), 1)]
@@ -278,6 +282,15 @@ def measure(ctx: Context, resident, requests: int, rate: float, seed: int = 1):
     log(f"plain: {json.dumps(plain)}")
     dn = run_arm(ctx, resident, stream, lambda r: plans[resident[r["scene"]][0]][r["spp"]][:2])
     log(f"denoised: {json.dumps(dn)}")
+    # Plain accumulation at a load it sustains: three quarters of the rate
+    # the overloaded plain arm actually served, over a quarter as many
+    # requests of the same stream law.  Its latency is render time rather
+    # than queueing.
+    sus_rate = 0.75 * rate * plain["Mpaths_s"] / max(load["plain"]["offered_Mpaths_s"], 1e-9)
+    sus_stream = request_stream(max(requests // 4, 1), sus_rate, len(resident), seed + 1)
+    sus = run_arm(ctx, resident, sus_stream, lambda r: (False, r["spp"]))
+    sus["rate_per_s"] = round(sus_rate, 2)
+    log(f"plain at a sustained load: {json.dumps(sus)}")
     served = {name: {s: {"denoise": p[0], "spp": p[1]} for s, p in plan.items() if s in (1, 2, 4, 8, 16, 32, 64)}
               for name, plan in plans.items()}
     return {"requests": requests, "rate_per_s": rate, "scenes": [r[0] for r in resident],
@@ -286,6 +299,7 @@ def measure(ctx: Context, resident, requests: int, rate: float, seed: int = 1):
                            "denoised": {str(s): round(x, 5) for s, x in v["denoised"].items()}}
                        for k, v in curves.items()},
             "plan": served, "single_render_Mpaths_s": rates, "offered": load, "plain": plain, "denoised": dn,
+            "plain_sustained": sus,
             "plain_overloaded": load["plain"]["utilisation"] >= 1.0,
             "denoised_overloaded": load["denoised"]["utilisation"] >= 1.0}
 
@@ -311,6 +325,12 @@ def summary_markdown(res: dict, command: str) -> str:
         a, o = res[arm], res["offered"][arm]
         lines.append(f"| {arm} | {o['offered_Mpaths_s']} | {o['utilisation']} | {a['Mpaths_s']} | {a['p50_ms']} | "
                      f"{a['p99_ms']} | {a['queued_p50_ms']} / {a['queued_p99_ms']} | {a['denoised_requests']} |")
+    a = res["plain_sustained"]
+    lines.append(f"| plain, {a['requests']} requests at {a['rate_per_s']}/s | - | - | {a['Mpaths_s']} | {a['p50_ms']} | "
+                 f"{a['p99_ms']} | {a['queued_p50_ms']} / {a['queued_p99_ms']} | 0 |")
+    lines += ["", f"Runners (ticks in flight at once, each with its own path pool and stream): {res['plain']['runners']}; "
+              f"ticks that began beside another: plain {res['plain']['overlapped_ticks']} of {res['plain']['ticks']}, "
+              f"denoised {res['denoised']['overlapped_ticks']} of {res['denoised']['ticks']}."]
     lines += ["", "Per scene p50 / p99 ms:", "", "| scene | plain | denoised |", "|---|---|---|"]
     for name in res["scenes"]:
         p, d = res["plain"]["per_scene_p50_p99_ms"][name], res["denoised"]["per_scene_p50_p99_ms"][name]

@@ -116,26 +116,31 @@ def test_denoised_request_equals_denoised_render(ctx, cornell):
 
 
 
-def test_requests_over_several_scenes_equal_single_renders(ctx, cornell):
+def test_ticks_of_several_scenes_overlap_and_equal_single_renders(ctx, cornell):
     # requests over different resident scenes in one server: a textured
     # scene, a lit sphere field (sky pass, lights in view) and the Cornell
-    # box.  A tick serves one scene's requests; each result equals its
-    # single render
+    # box.  A tick serves one group (scene, depth, RR start, camera medium);
+    # the runners' ticks of different groups run at the same time, each on
+    # its own stream.  Each result equals its single render.
     _, cid = cornell
     tex = scenes.textured_cornell(subdiv=2)
     lit = scenes.lit_sphere_field(subdiv=1)
     tid, lid = ctx.upload(tex), ctx.upload(lit)
     ctx.build_bvh(tid)
     ctx.build_bvh(lid)
-    reqs = [(tid, 40, 32, 3, tex.camera), (lid, 48, 27, 2, lit.camera), (cid, 32, 32, 2, scenes.cornell_spheres().camera),
-            (tid, 24, 24, 4, tex.camera)]
-    srv = Server(ctx, budget_bytes=1 << 30, max_active=8, tick_paths=4000, max_frame_pixels=64 * 64)
+    reqs = [(tid, 40, 32, 24, tex.camera), (lid, 48, 27, 32, lit.camera),
+            (cid, 32, 32, 32, scenes.cornell_spheres().camera), (tid, 24, 24, 40, tex.camera)]
+    srv = Server(ctx, budget_bytes=1 << 31, max_active=8, tick_paths=2000, max_frame_pixels=64 * 64, runners=3)
     try:
         ids = [srv.submit(s, w, h, spp, cam, max_depth=8) for s, w, h, spp, cam in reqs]
         for rid in ids:
             assert srv.wait(rid, 60000) == 0
         got = [srv.result_color(rid) for rid in ids]
-        assert srv.stats().ticks >= 2
+        st = srv.stats()
+        print(f"[serve] {st.ticks} ticks on {st.runners} runners, {st.overlapped_ticks} began beside another, "
+              f"at most {st.max_running} at once", flush=True)
+        assert st.runners == 3
+        assert st.max_running >= 2 and st.overlapped_ticks >= 1
     finally:
         srv.close()
     try:
@@ -154,7 +159,7 @@ def test_cancel_and_release_during_a_tick(ctx, cornell):
     # scene cannot be released while a request of it is live.
     sc, sid = cornell
     cam = abi.camera((0, 1, 3), (0, 1, 0))
-    srv = Server(ctx, budget_bytes=1 << 31, max_active=1, tick_paths=1 << 22, max_frame_pixels=256 * 256)
+    srv = Server(ctx, budget_bytes=1 << 33, max_active=1, tick_paths=1 << 22, max_frame_pixels=256 * 256)
     try:
         a = srv.submit(sid, 256, 256, 64, cam, max_depth=8)     # 4 M paths: one long tick
         b = srv.submit(sid, 64, 64, 8, cam, max_depth=8)        # pending behind a (max_active 1)

@@ -112,7 +112,8 @@ def test_heightfield_per_path(ctx, heightfield):
 def test_serving_mix_equals_single_renders(ctx, sphere_field, heightfield):
     # 20 requests of the seeded serving stream (Poisson arrivals, resolutions
     # over 256^2 .. 1920x1080, spp log-uniform in [1, 64], depth 16) over the
-    # four resident scenes, submitted at once so that ticks are shared.
+    # four resident scenes, submitted at once so that ticks are shared and
+    # the ticks of different scenes run at the same time on four runners.
     resident = [scenes.cornell_spheres(), sphere_field, heightfield, scenes.sphere_field_5m()]
     sids = []
     try:
@@ -121,7 +122,8 @@ def test_serving_mix_equals_single_renders(ctx, sphere_field, heightfield):
             ctx.build_bvh(sid)
             sids.append(sid)
         stream = serve_bench.request_stream(20, 200.0, len(resident))
-        srv = Server(ctx, budget_bytes=64 << 30, max_active=32, tick_paths=1 << 25, max_frame_pixels=1920 * 1080)
+        srv = Server(ctx, budget_bytes=64 << 30, max_active=32, tick_paths=1 << 25, max_frame_pixels=1920 * 1080,
+                     runners=4)
         try:
             ids = []
             for r in stream:
@@ -132,8 +134,10 @@ def test_serving_mix_equals_single_renders(ctx, sphere_field, heightfield):
             got = [srv.result_rgba8(rid) for rid in ids]
             cols = [srv.result_color(rid) for rid in ids]
             st = srv.stats()
-            print(f"[serve] {len(ids)} requests, {st.ticks} ticks, {st.paths / 1e6:.1f} M paths, "
+            print(f"[serve] {len(ids)} requests, {st.ticks} ticks ({st.overlapped_ticks} beside another, at most "
+                  f"{st.max_running} at once), {st.paths / 1e6:.1f} M paths, "
                   f"p50 {st.p50_ms:.1f} ms p99 {st.p99_ms:.1f} ms", flush=True)
+            assert st.max_running >= 2
         finally:
             srv.close()
         for r, px, col in zip(stream, got, cols):
