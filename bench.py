@@ -32,7 +32,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES = 32, 36, 48   # 32-B node record, 36-B triangle, 32-B ray in + 16-B hit out
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 16, 36, 48   # 32-B record per two child boxes, 36-B triangle, ray in + hit out
+SURVEY_NODE_BYTES = 32   # one uncompressed 32-B box record per child (lo, ref, hi), the per-ray formula's unit
 SHADOW_RAY_BYTES = 40   # 4-B slot id + 32-B ray in + 4-B flag cleared when occluded
 
 
@@ -130,13 +131,15 @@ def cpu_baseline(scene, W, H, depth, spp_total, label, budget_s=12.0, single_s=4
     }
 
 
-NODE_BYTES_DOC = "32-B node record per child box fetched, 36-B triangle per test, 48 B per extension ray, 40 per shadow ray"
+NODE_BYTES_DOC = ("16 B per child box fetched (a 32-B compressed record holds both children of a node), 36-B "
+                  "triangle per test, 48 B per extension ray, 40 per shadow ray")
 
 
-def traversal_bytes(counted):
+def traversal_bytes(counted, node_bytes=NODE_BYTES):
     """Algorithmic bytes of one step's k_trace launches from a counting
     render of the same frames: camera rays the cover mask rules out are never
     traced, and the tail kernel's rays are its own launch."""
+    NODE_BYTES = node_bytes
     ext_bytes = NODE_BYTES * (counted.ext_node_fetches - counted.resolved_node_fetches) + \
         TRI_BYTES * (counted.ext_tri_tests - counted.resolved_tri_tests) + \
         RAY_BYTES * (counted.extension_rays - counted.camera_rays_resolved)
@@ -166,6 +169,12 @@ def roofline(counted, trace_ms, launches, traffic_file, workload, tuning, strict
         "bytes_basis": f"algorithmic: {NODE_BYTES_DOC}, node and triangle counts from a counting render",
         "per_ray_bytes": {"extension": round(per_ext, 1), "shadow": round(per_sh, 1)},
     }
+    # the same counts priced at 32 B per child box, the unit of uncompressed
+    # (lo, ref, hi) records: how fast the traversal covers the tree
+    alg32 = traversal_bytes(counted, SURVEY_NODE_BYTES)[0]
+    out["uncompressed_equivalent"] = {"achieved": round(alg32 / (trace_ms / 1e3) / 1e9, 1),
+                                      "frac": round(alg32 / (trace_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                      "basis": "32 B per child box, 36 B per triangle, 48 / 40 B per ray"}
     if measured is not None:
         step_bytes = measured["per_step"] or measured["per_launch"] * launches
         dram = step_bytes / (trace_ms / 1e3) / 1e9

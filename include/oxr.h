@@ -194,6 +194,13 @@ int oxr_scene_select(oxr_ctx* ctx, uint32_t scene_id);
    (internal nodes then leaves). Any pointer may be NULL. */
 int oxr_scene_download_bvh(oxr_ctx* ctx, uint32_t scene_id, uint32_t* codes, uint32_t* order,
                            uint32_t* left, uint32_t* right, float* lo, float* hi);
+/* The traversal's compressed node records of a built scene: 8 words per
+   internal node (n-1): origin.xyz (floats), gamma | left leaf << 30 | right
+   leaf << 31, three signed per-axis scale exponents, twelve 8-bit child
+   bounds (child 0 lo.xyz hi.xyz, child 1 lo.xyz hi.xyz).  Replaces the
+   acceleration structure OptiX builds and keeps opaque
+   (optixAccelBuild/Compact, Renderer.cpp:247-274). */
+int oxr_scene_download_nodes(oxr_ctx* ctx, uint32_t scene_id, uint32_t* out);
 /* Environment importance cache built on the device (HdrTexture::CalculateHdrCache,
    Scene.cpp:217-313): env_w*env_h RGBA32F. */
 int oxr_scene_download_env_cache(oxr_ctx* ctx, uint32_t scene_id, float* cache_rgba);

@@ -305,6 +305,7 @@ SIGNATURES = {
     "oxr_scene_release": (C.c_int, [_P, C.c_uint32]),
     "oxr_scene_select": (C.c_int, [_P, C.c_uint32]),
     "oxr_scene_download_bvh": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P]),
+    "oxr_scene_download_nodes": (C.c_int, [_P, C.c_uint32, _P]),
     "oxr_scene_download_env_cache": (C.c_int, [_P, C.c_uint32, _P]),
     "oxr_trace_rays": (C.c_int, [_P, _P, _I64, C.c_int, _P, _P, _P]),
     "oxr_resize": (C.c_int, [_P, C.c_int, C.c_int]),

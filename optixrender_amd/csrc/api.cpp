@@ -152,7 +152,7 @@ void dfree(T*& p) {
 // follows a failed one, so a retry after an allocation failure leaks nothing).
 void free_build(SceneDev* s) {
     dfree(s->bvh.codes); dfree(s->bvh.order); dfree(s->bvh.left); dfree(s->bvh.right); dfree(s->bvh.parent);
-    dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.pairs); dfree(s->bvh.tris);
+    dfree(s->bvh.lo); dfree(s->bvh.hi); dfree(s->bvh.qnodes); dfree(s->bvh.tris);
     dfree(s->normals_sorted); dfree(s->mat_sorted); dfree(s->uvs_sorted);
     dfree(s->cover_boxes); dfree(s->cover_count);
 }
@@ -267,7 +267,7 @@ float oxr::env_luminance_sum(const float* rgba, int64_t texels) {
 
 void oxr::fill_scene_params(WaveParams& p, const SceneDev* s, const float* lut_bsdf, const float* lut_albedo) {
     std::memset(&p, 0, sizeof(p));
-    p.bvh = BvhView{s->bvh.pairs, s->bvh.tris, s->n};
+    p.bvh = BvhView{s->bvh.qnodes, s->bvh.tris, s->n};
     p.normals = s->normals_sorted;
     p.mat_ids = s->mat_sorted;
     p.mats = s->mats;
@@ -540,7 +540,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
         OXR_HIP(c, dalloc(&b.parent, nodes));
         OXR_HIP(c, dalloc(&b.lo, 3 * nodes));
         OXR_HIP(c, dalloc(&b.hi, 3 * nodes));
-        OXR_HIP(c, dalloc(&b.pairs, 4 * ni));
+        OXR_HIP(c, dalloc(&b.qnodes, 2 * ni));
         OXR_HIP(c, dalloc(&b.tris, 3 * n));
         OXR_HIP(c, dalloc(&s->normals_sorted, 9 * n));
         OXR_HIP(c, dalloc(&s->mat_sorted, n));
@@ -561,8 +561,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
             e = gather_sorted_attributes(s->normals, s->mat_ids, s->uvs, s->bvh.order, n, s->normals_sorted,
                                          s->mat_sorted, s->uvs_sorted, c->stream);
     }
-    if (e == hipSuccess) e = build_cover_set(BvhView{s->bvh.pairs, s->bvh.tris, n}, s->cover_boxes, s->cover_count,
-                                             c->stream);
+    if (e == hipSuccess) e = build_cover_set(s->bvh, n, s->cover_boxes, s->cover_count, c->stream);
     OXR_HIP(c, hipEventRecord(e1, c->stream));
     hipError_t es = hipStreamSynchronize(c->stream);
     uint32_t bad = 0;
@@ -626,6 +625,18 @@ int oxr_scene_download_bvh(oxr_ctx* c, uint32_t id, uint32_t* codes, uint32_t* o
     return OXR_OK;
 }
 
+int oxr_scene_download_nodes(oxr_ctx* c, uint32_t id, uint32_t* out) {
+    if (!c || !out) return OXR_ERR_INVALID;
+    auto it = c->scenes.find(id);
+    if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
+    SceneDev* s = it->second;
+    if (!s->built) return fail(c, OXR_ERR_STATE, "bvh not built");
+    const int64_t ni = s->n > 1 ? s->n - 1 : 0;
+    OXR_HIP(c, hipStreamSynchronize(c->stream));
+    if (ni) OXR_HIP(c, hipMemcpy(out, s->bvh.qnodes, 32 * ni, hipMemcpyDeviceToHost));
+    return OXR_OK;
+}
+
 int oxr_scene_download_env_cache(oxr_ctx* c, uint32_t id, float* out) {
     if (!c || !out) return OXR_ERR_INVALID;
     auto it = c->scenes.find(id);
@@ -659,7 +670,7 @@ int oxr_trace_rays(oxr_ctx* c, const float* rays, int64_t n, int any_hit, float*
         cleanup();
         return fail(c, OXR_ERR_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
     }
-    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
+    BvhView bv{s->bvh.qnodes, s->bvh.tris, s->n};
     if ((e = wave_reserve(c->eng, 0, 0, c->trace_grid)) == hipSuccess &&
         (e = hipMemsetAsync(c->eng.overflow, 0, sizeof(int), c->stream)) == hipSuccess)
         e = trace_rays(bv, dr, n, any_hit, dh, dOcc, dc, c->eng.gstack, c->trace_grid, c->eng.overflow, c->stream);
@@ -687,7 +698,7 @@ int oxr_trace_rays_noalloc(oxr_ctx* c, const float* rays, int64_t n, int any_hit
     if (!s || !s->built) return fail(c, OXR_ERR_STATE, "no built scene selected");
     if (!c->noalloc_scratch) return fail(c, OXR_ERR_STATE, "trace_rays_noalloc: no scratch on this device");
     OXR_HIP(c, hipSetDevice(c->device));
-    BvhView bv{s->bvh.pairs, s->bvh.tris, s->n};
+    BvhView bv{s->bvh.qnodes, s->bvh.tris, s->n};
     hipError_t e = trace_rays_noalloc(bv, rays, n, any_hit, (float4*)hits, occ, c->noalloc_scratch, c->stream);
     int ov = 0;
     if (e == hipSuccess)

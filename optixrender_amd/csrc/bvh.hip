@@ -232,21 +232,12 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     parent[(R & kLeafBit) ? ni + (R & ~kLeafBit) : R] = (uint32_t)i;
 }
 
-__device__ __forceinline__ void write_record(float4* pairs, const uint32_t* left, uint32_t parent, uint32_t ref,
-                                             const float lo[3], const float hi[3]) {
-    int side = (left[parent] == ref) ? 0 : 1;
-    float4* r = pairs + 4 * (size_t)parent + 2 * side;
-    r[0] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(ref));
-    r[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
-}
-
 // One thread per leaf: leaf box, sorted triangle record, then climb while
 // this thread is the second to reach a node.
 __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, int64_t n,
                                                 const uint32_t* __restrict__ order, const uint32_t* __restrict__ left,
                                                 const uint32_t* __restrict__ right, const uint32_t* __restrict__ parent,
-                                                uint32_t* flags, float* lo, float* hi, float4* pairs, float4* tris,
-                                                uint32_t* error) {
+                                                uint32_t* flags, float* lo, float* hi, float4* tris, uint32_t* error) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int64_t ni = n - 1;
@@ -265,9 +256,7 @@ __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, in
 #pragma unroll
     for (int a = 0; a < 3; a++) { lo[3 * node + a] = bl[a]; hi[3 * node + a] = bh[a]; }
     if (ni == 0) return;
-    uint32_t ref = kLeafBit | (uint32_t)k;
     uint32_t par = parent[node];
-    write_record(pairs, left, par, ref, bl, bh);
     for (int depth = 0;; depth++) {
         if (depth > 256 || par >= (uint32_t)ni) {  // LBVH depth <= 30 + log2(n): a malformed tree
             atomicOr(error, 1u);
@@ -295,10 +284,60 @@ __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, in
             __hip_atomic_store(&hi[3 * (size_t)par + a], nh[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (par == 0) return;
-        uint32_t gp = parent[par];
-        write_record(pairs, left, gp, par, nl, nh);
-        par = gp;
+        par = parent[par];
     }
+}
+
+// exact 2^e (|e| < 1023)
+__device__ __forceinline__ double pow2(int e) { return __longlong_as_double((long long)(e + 1023) << 52); }
+
+// Smallest scale exponent whose 253 steps reach the extent (kQuantEmin for
+// an empty or NaN extent).
+__device__ int quant_exponent(double extent) {
+    if (!(extent > 0.0)) return kQuantEmin;
+    const int k = (int)((__double_as_longlong(extent) >> 52) & 0x7ff) - 1022;  // extent < 2^k when normal
+    int e = max(kQuantEmin, k - 9);
+    while (e < kQuantEmax && extent > 253.0 * pow2(e)) e++;
+    return e;
+}
+
+// One thread per internal node: its compressed record (layout in common.h).
+// Bounds are computed in double, so they round outwards whatever the
+// magnitudes: a child's lower bound lies at least half a step below its box
+// (or at the origin), its upper bound at least half a step above.
+__global__ __launch_bounds__(256) void k_quantize(int64_t ni, const uint32_t* __restrict__ left,
+                                                   const uint32_t* __restrict__ right, const float* __restrict__ lo,
+                                                   const float* __restrict__ hi, uint4* qnodes) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= ni) return;
+    const uint32_t cl = left[v], cr = right[v];
+    const int64_t ch[2] = {(cl & kLeafBit) ? ni + (cl & ~kLeafBit) : cl, (cr & kLeafBit) ? ni + (cr & ~kLeafBit) : cr};
+    int e[3];
+    float org[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        org[a] = lo[3 * v + a];
+        e[a] = quant_exponent((double)hi[3 * v + a] - (double)org[a]);
+    }
+    uint32_t q[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double step_inv = pow2(-e[a]);
+            const double xl = ((double)lo[3 * ch[c] + a] - (double)org[a]) * step_inv;
+            const double xh = ((double)hi[3 * ch[c] + a] - (double)org[a]) * step_inv;
+            const uint32_t ql = (uint32_t)(xl - 0.5 >= 0.0 ? floor(xl - 0.5) : 0.0);
+            const uint32_t qh = (uint32_t)(xh + 0.5 < 254.0 ? floor(xh + 0.5) + 1.0 : 255.0);
+            const int bl = 6 * c + a, bh = 6 * c + 3 + a;  // byte positions
+            q[bl >> 2] |= ql << (8 * (bl & 3));
+            q[bh >> 2] |= qh << (8 * (bh & 3));
+        }
+    const uint32_t w3 = ((cl & ~kLeafBit) & kQGamma) | ((cl & kLeafBit) ? kQLeft : 0u) | ((cr & kLeafBit) ? kQRight : 0u);
+    const uint32_t w4 = (uint32_t)(uint8_t)(int8_t)e[0] | (uint32_t)(uint8_t)(int8_t)e[1] << 8 |
+                        (uint32_t)(uint8_t)(int8_t)e[2] << 16;
+    qnodes[2 * v] = make_uint4(__float_as_uint(org[0]), __float_as_uint(org[1]), __float_as_uint(org[2]), w3);
+    qnodes[2 * v + 1] = make_uint4(w4, q[0], q[1], q[2]);
 }
 
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -350,7 +389,8 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
         if (e != hipSuccess) return e;
     }
     k_refit<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.left, o.right, o.parent, flags, o.lo, o.hi,
-                                               o.pairs, o.tris, bounds + 6);
+                                               o.tris, bounds + 6);
+    if (n > 1) k_quantize<<<blocks_for(n - 1, 256), 256, 0, s>>>(n - 1, o.left, o.right, o.lo, o.hi, o.qnodes);
     return hipGetLastError();
 }
 

@@ -13,12 +13,22 @@ constexpr int kStackTotal = 64;   // traversal stack entries per lane (LBVH dept
 constexpr int kStackLdsMin = 8;   // fewest entries an instance keeps in LDS; the rest spill to HBM
 constexpr int kTraceBlock = 256;
 
-// BVH in traversal layout.  pairs: for internal node i, the two children
-// records at pairs[4i .. 4i+3]: (lo.xyz, ref), (hi.xyz, 0) for the left
-// child then the right child; ref = internal id, or kLeafBit | sorted leaf.
+// BVH in traversal layout.  qnodes: one 32-byte record per internal node i
+// at qnodes[2i], qnodes[2i + 1] (words 0..7):
+//   0..2  origin: the node's lower corner (floats)
+//   3     gamma | left leaf << 30 | right leaf << 31: the children are gamma
+//         and gamma + 1 (Karras split), each an internal id or, flagged, a
+//         sorted leaf
+//   4     e_x, e_y, e_z as signed bytes: per-axis scale 2^e
+//   5..7  12 bytes: child 0 lo.xyz, hi.xyz, child 1 lo.xyz, hi.xyz, each an
+//         8-bit multiple of the scale above the origin, rounded outwards
+// A child's slab bound at q steps lies at ray distance
+// fma(q, 2^e * inv, (origin - o) * inv).
 // tris: 3 float4 per sorted leaf: (v0, triangle id bits), (v1, 0), (v2, 0).
+constexpr uint32_t kQLeft = 1u << 30, kQRight = 1u << 31, kQGamma = kQLeft - 1;
+constexpr int kQuantEmin = -64, kQuantEmax = 63;
 struct BvhView {
-    const float4* pairs;
+    const uint4* qnodes;
     const float4* tris;
     int64_t n;
 };
@@ -32,7 +42,7 @@ struct BvhBuffers {
     uint32_t* parent = nullptr;    // 2n-1 (internal, then leaves)
     float* lo = nullptr;           // 3*(2n-1)
     float* hi = nullptr;           // 3*(2n-1)
-    float4* pairs = nullptr;       // 4*(n-1)
+    uint4* qnodes = nullptr;       // 2*(n-1): compressed records for the traversal
     float4* tris = nullptr;        // 3n
 };
 
@@ -268,7 +278,7 @@ hipError_t wave_fill(const WaveParams& p, int grid, hipStream_t s);
 // Scene cover set: the boxes of the BVH's top levels (breadth first, at most
 // kCoverBoxes; leaves met on the way included), every triangle inside one.
 // boxes: 2 float4 (lo, hi) each; count: one int.
-hipError_t build_cover_set(const BvhView& b, float4* boxes, int* count, hipStream_t s);
+hipError_t build_cover_set(const BvhBuffers& b, int64_t n, float4* boxes, int* count, hipStream_t s);
 // Cover masks of requests r0 .. r0 + nreq - 1 of the device table: a tile is
 // 1 when the camera ray of some sample of its pixels may enter a cover box.
 hipError_t build_cover_masks(const ReqDesc* reqs, int nreq, int max_tiles, const float4* boxes, const int* count,

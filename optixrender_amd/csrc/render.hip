@@ -1188,26 +1188,31 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     q.depth[pix] = dz;
 }
 
-// The scene's cover set (one thread): starting from the root's two child
-// records, replace every internal node by its children's records, level by
-// level, while the list stays within kCoverBoxes.  Expanding from the back
-// writes each entry at or after the one being read.
-__global__ void k_cover_set(BvhView b, float4* boxes, int* count) {
+// The scene's cover set (one thread): starting from the root's two
+// children, replace every internal node by its children, level by level,
+// while the list stays within kCoverBoxes.  Entries are (lo.xyz, ref),
+// (hi.xyz, 0) with the exact boxes of the Karras layout.  Expanding from the
+// back writes each entry at or after the one being read.
+__global__ void k_cover_set(int64_t nt, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
+                            float4* boxes, int* count) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    if (b.n <= 0) {
+    if (nt <= 0) {
         *count = 0;
         return;
     }
-    if (b.n == 1) {
-        float4 v0 = b.tris[0], v1 = b.tris[1], v2 = b.tris[2];
-        boxes[0] = make_float4(fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y),
-                               fminf(fminf(v0.z, v1.z), v2.z), __uint_as_float(kLeafBit));
-        boxes[1] = make_float4(fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y),
-                               fmaxf(fmaxf(v0.z, v1.z), v2.z), 0.0f);
+    const int64_t ni = nt - 1;
+    auto put = [&](uint32_t ref, float4* out) {
+        const int64_t idx = (ref & kLeafBit) ? ni + (ref & ~kLeafBit) : ref;
+        out[0] = make_float4(lo[3 * idx], lo[3 * idx + 1], lo[3 * idx + 2], __uint_as_float(ref));
+        out[1] = make_float4(hi[3 * idx], hi[3 * idx + 1], hi[3 * idx + 2], 0.0f);
+    };
+    if (nt == 1) {
+        put(kLeafBit, boxes);
         *count = 1;
         return;
     }
-    for (int i = 0; i < 4; i++) boxes[i] = b.pairs[i];
+    put(left[0], boxes);
+    put(right[0], boxes + 2);
     int n = 2;
     while (true) {
         int internal = 0;
@@ -1215,19 +1220,16 @@ __global__ void k_cover_set(BvhView b, float4* boxes, int* count) {
         if (internal == 0 || n + internal > kCoverBoxes) break;
         int m = n + internal;
         for (int i = n - 1; i >= 0; i--) {
-            const float4 lo = boxes[2 * i], hi = boxes[2 * i + 1];
-            const uint32_t ref = __float_as_uint(lo.w);
+            const float4 blo = boxes[2 * i], bhi = boxes[2 * i + 1];
+            const uint32_t ref = __float_as_uint(blo.w);
             if (ref & kLeafBit) {
                 m--;
-                boxes[2 * m] = lo;
-                boxes[2 * m + 1] = hi;
+                boxes[2 * m] = blo;
+                boxes[2 * m + 1] = bhi;
             } else {
-                const float4* pr = b.pairs + 4 * (size_t)ref;
                 m -= 2;
-                boxes[2 * m] = pr[0];
-                boxes[2 * m + 1] = pr[1];
-                boxes[2 * m + 2] = pr[2];
-                boxes[2 * m + 3] = pr[3];
+                put(left[ref], boxes + 2 * m);
+                put(right[ref], boxes + 2 * m + 2);
             }
         }
         n += internal;
@@ -1331,8 +1333,8 @@ inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
 
-hipError_t build_cover_set(const BvhView& b, float4* boxes, int* count, hipStream_t s) {
-    k_cover_set<<<1, 64, 0, s>>>(b, boxes, count);
+hipError_t build_cover_set(const BvhBuffers& b, int64_t n, float4* boxes, int* count, hipStream_t s) {
+    k_cover_set<<<1, 64, 0, s>>>(n, b.left, b.right, b.lo, b.hi, boxes, count);
     return hipGetLastError();
 }
 

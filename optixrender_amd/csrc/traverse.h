@@ -2,8 +2,10 @@
 // (core/cuda/devicePrograms.cu:258-269 closest hit, :303-320/:389-405/:437-453
 // any hit with TERMINATE_ON_FIRST_HIT).
 //
-// One ray per lane.  Each visited internal node costs one 64-byte fetch of
-// its two child records (4 x dwordx4), leaves are tested at once in
+// One ray per lane.  Each visited internal node costs one 32-byte fetch of
+// its compressed record (2 x dwordx4: origin, children, per-axis scale and
+// both children's boxes as 8-bit bounds; the vector-memory path, not the
+// ALU, is what bounds this loop), leaves are tested at once in
 // (left, right) order with a watertight ray/triangle test (Woop, Benthin,
 // Wald 2013) on a 48-byte triangle record, then for a closest-hit ray the
 // nearer hit internal child is descended and the farther pushed (any hit:
@@ -92,13 +94,37 @@ OXR_D void tri_uv(const RayPre& r, float4 a, float4 b, float4 c, float& u, float
     v = e.W / e.det;
 }
 
-OXR_D bool box_test(const RayPre& r, float4 lo, float4 hi, float tmin, float tbest, float& tn) {
-    float tx0 = (lo.x - r.o.x) * r.inv.x, tx1 = (hi.x - r.o.x) * r.inv.x;
-    float ty0 = (lo.y - r.o.y) * r.inv.y, ty1 = (hi.y - r.o.y) * r.inv.y;
-    float tz0 = (lo.z - r.o.z) * r.inv.z, tz1 = (hi.z - r.o.z) * r.inv.z;
+// Slab test of a box given by its six slab distances; the far distance is
+// widened by 2 ulp.
+OXR_D bool slab_test(float tx0, float tx1, float ty0, float ty1, float tz0, float tz1, float tmin, float tbest,
+                     float& tn) {
     tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000024f;
     return tn <= tf && tf >= tmin && tn <= tbest;
+}
+
+// 8-bit bound k (0..3) of a word as a float (one v_cvt_f32_ubyte<k>)
+OXR_D float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
+
+// Both children of a compressed node record (layout in common.h): per axis
+// the origin's distance A = (origin - o) * inv and the step's B = 2^e * inv;
+// a bound q steps above the origin lies at fma(q, B, A).  A bound of 0 is the
+// origin itself, so that face is tested exactly as an uncompressed box.
+OXR_D void qnode_test(const RayPre& r, uint4 w0, uint4 w1, float tmin, float tbest, bool& h0, bool& h1, float& tn0,
+                      float& tn1) {
+    const float Ax = (__uint_as_float(w0.x) - r.o.x) * r.inv.x;
+    const float Ay = (__uint_as_float(w0.y) - r.o.y) * r.inv.y;
+    const float Az = (__uint_as_float(w0.z) - r.o.z) * r.inv.z;
+    const float Bx = ldexpf(r.inv.x, (int)(int8_t)(w1.x & 0xffu));
+    const float By = ldexpf(r.inv.y, (int)(int8_t)((w1.x >> 8) & 0xffu));
+    const float Bz = ldexpf(r.inv.z, (int)(int8_t)((w1.x >> 16) & 0xffu));
+    // bytes 0..11 of words y, z, w: child 0 lo.xyz hi.xyz, child 1 lo.xyz hi.xyz
+    h0 = slab_test(fmaf(qbyte(w1.y, 0), Bx, Ax), fmaf(qbyte(w1.y, 3), Bx, Ax), fmaf(qbyte(w1.y, 1), By, Ay),
+                   fmaf(qbyte(w1.z, 0), By, Ay), fmaf(qbyte(w1.y, 2), Bz, Az), fmaf(qbyte(w1.z, 1), Bz, Az), tmin,
+                   tbest, tn0);
+    h1 = slab_test(fmaf(qbyte(w1.z, 2), Bx, Ax), fmaf(qbyte(w1.w, 1), Bx, Ax), fmaf(qbyte(w1.z, 3), By, Ay),
+                   fmaf(qbyte(w1.w, 2), By, Ay), fmaf(qbyte(w1.w, 0), Bz, Az), fmaf(qbyte(w1.w, 3), Bz, Az), tmin,
+                   tbest, tn1);
 }
 
 struct Hit {
@@ -196,13 +222,14 @@ OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT&
     int budget = (int)b.n + 64;
     while (true) {
         if (--budget < 0 || node >= (uint32_t)(b.n - 1)) { *overflow = 2; break; }
-        const float4* pr = b.pairs + 4 * (size_t)node;
-        float4 lo0 = pr[0], hi0 = pr[1], lo1 = pr[2], hi1 = pr[3];
+        const uint4 w0 = b.qnodes[2 * (size_t)node], w1 = b.qnodes[2 * (size_t)node + 1];
         if (COUNT) nn += 2;
-        uint32_t c0 = __float_as_uint(lo0.w), c1 = __float_as_uint(lo1.w);
+        const uint32_t g = w0.w & kQGamma;
+        const uint32_t c0 = (w0.w & kQLeft) ? (kLeafBit | g) : g;
+        const uint32_t c1 = (w0.w & kQRight) ? (kLeafBit | (g + 1)) : g + 1;
         float tn0, tn1;
-        bool h0 = box_test(r, lo0, hi0, tmin, h.t, tn0);
-        bool h1 = box_test(r, lo1, hi1, tmin, h.t, tn1);
+        bool h0, h1;
+        qnode_test(r, w0, w1, tmin, h.t, h0, h1, tn0, tn1);
         // Leaf children: child 0's triangle, then child 1's.  One block
         // serves every lane with a leaf on either side and a second only the
         // lanes with two, so a wave whose lanes reach leaves on different

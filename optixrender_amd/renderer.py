@@ -140,6 +140,12 @@ class Context:
                                                     abi.ptr(right), abi.ptr(lo), abi.ptr(hi)), "download_bvh")
         return codes, order, left[:ni], right[:ni], lo[: 2 * n - 1], hi[: 2 * n - 1]
 
+    def download_nodes(self, sid: int, n: int) -> np.ndarray:
+        """The traversal's compressed node records: (n-1, 8) words."""
+        out = np.zeros((max(n - 1, 1), 8), np.uint32)
+        self.check(self._lib.oxr_scene_download_nodes(self.h, sid, abi.ptr(out)), "download_nodes")
+        return out[: max(n - 1, 0)]
+
     def download_env_cache(self, sid: int, w: int, h: int) -> np.ndarray:
         out = np.zeros((h, w, 4), np.float32)
         self.check(self._lib.oxr_scene_download_env_cache(self.h, sid, abi.ptr(out)), "download_env_cache")

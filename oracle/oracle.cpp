@@ -1254,7 +1254,70 @@ struct Bvh {
     std::vector<float> lo, hi;          // (n-1) internal then n leaves, 3 floats each
     std::vector<float> tri;             // sorted triangles, 9 floats
     std::vector<uint32_t> prim;         // sorted position -> triangle id
+    std::vector<uint32_t> qnode;        // kQWords per internal node: the traversal's compressed records
 };
+
+// Compressed node records.  The traversal reads, per internal node, 32
+// bytes: the node's own lower corner (the origin, 3 floats), its children
+// (the Karras split gamma: left child gamma, right child gamma + 1, with a
+// leaf flag each), one power-of-two scale per axis, and each child's box as
+// 8-bit multiples of that scale above the origin, rounded outwards by at
+// least half a step (a lower bound of 0 is the origin itself):
+//   word 0..2  origin.xyz
+//   word 3     gamma | left leaf << 30 | right leaf << 31
+//   word 4     e_x, e_y, e_z as signed bytes (scale 2^e)
+//   word 5..7  child 0 lo.xyz, hi.xyz, child 1 lo.xyz, hi.xyz (one byte each)
+// A slab bound at q steps is at distance fma(q, 2^e * inv, (origin - o) * inv).
+constexpr int kQWords = 8;
+constexpr int kQuantEmin = -64, kQuantEmax = 63;
+constexpr uint32_t kQLeft = 1u << 30, kQRight = 1u << 31, kQGamma = kQLeft - 1;
+
+double pow2(int e) {                     // exact 2^e for e in [-1022, 1023]
+    uint64_t bits = (uint64_t)(e + 1023) << 52;
+    double x;
+    std::memcpy(&x, &bits, 8);
+    return x;
+}
+
+// Smallest scale exponent whose 253 steps reach the extent.
+int quant_exponent(double extent) {
+    if (!(extent > 0.0)) return kQuantEmin;  // also NaN
+    uint64_t bits;
+    std::memcpy(&bits, &extent, 8);
+    const int k = (int)((bits >> 52) & 0x7ff) - 1022;  // extent < 2^k for a normal extent
+    int e = std::max(kQuantEmin, k - 9);
+    while (e < kQuantEmax && extent > 253.0 * pow2(e)) e++;
+    return e;
+}
+
+// The node record of internal node v (Karras layout: lo/hi of internal v at
+// v, of leaf k at n - 1 + k), computed in double so that the rounding is
+// outwards whatever the magnitudes.
+void quantize_node(int64_t ni, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
+                   int64_t v, uint32_t* out) {
+    const uint32_t cl = left[v], cr = right[v];
+    const int64_t il = (cl & kLeaf) ? ni + (cl & ~kLeaf) : cl;
+    const int64_t ir = (cr & kLeaf) ? ni + (cr & ~kLeaf) : cr;
+    int e[3];
+    for (int a = 0; a < 3; a++) {
+        std::memcpy(&out[a], &lo[3 * v + a], 4);
+        e[a] = quant_exponent((double)hi[3 * v + a] - (double)lo[3 * v + a]);
+    }
+    out[3] = ((cl & ~kLeaf) & kQGamma) | ((cl & kLeaf) ? kQLeft : 0u) | ((cr & kLeaf) ? kQRight : 0u);
+    out[4] = (uint32_t)(uint8_t)(int8_t)e[0] | (uint32_t)(uint8_t)(int8_t)e[1] << 8 |
+             (uint32_t)(uint8_t)(int8_t)e[2] << 16;
+    uint8_t q[12];
+    const int64_t ch[2] = {il, ir};
+    for (int c = 0; c < 2; c++)
+        for (int a = 0; a < 3; a++) {
+            const double inv_step = pow2(-e[a]);
+            const double xl = ((double)lo[3 * ch[c] + a] - (double)lo[3 * v + a]) * inv_step;
+            const double xh = ((double)hi[3 * ch[c] + a] - (double)lo[3 * v + a]) * inv_step;
+            q[6 * c + a] = (uint8_t)(xl - 0.5 >= 0.0 ? std::floor(xl - 0.5) : 0.0);
+            q[6 * c + 3 + a] = (uint8_t)(xh + 0.5 < 254.0 ? std::floor(xh + 0.5) + 1.0 : 255.0);
+        }
+    std::memcpy(&out[5], q, 12);
+}
 void build_bvh(Bvh& b, int64_t n, const float* pos) {
     b.n = n;
     b.codes.resize(n);
@@ -1337,6 +1400,9 @@ void build_bvh(Bvh& b, int64_t n, const float* pos) {
             done[v] = 1;
         }
     }
+    b.qnode.assign((size_t)kQWords * (ni > 0 ? ni : 0), 0u);
+    for (int64_t v = 0; v < ni; v++)
+        quantize_node(ni, b.left.data(), b.right.data(), b.lo.data(), b.hi.data(), v, &b.qnode[kQWords * v]);
 }
 
 struct RayPre {
@@ -1382,14 +1448,37 @@ inline bool tri_test(const RayPre& r, const float* p, float tmin, float& t, floa
     bv = W / det;
     return true;
 }
-inline bool box_test(const RayPre& r, const float* lo, const float* hi, float tmin, float tbest, float& tnear) {
-    float tx0 = (lo[0] - r.o.x) * r.inv.x, tx1 = (hi[0] - r.o.x) * r.inv.x;
-    float ty0 = (lo[1] - r.o.y) * r.inv.y, ty1 = (hi[1] - r.o.y) * r.inv.y;
-    float tz0 = (lo[2] - r.o.z) * r.inv.z, tz1 = (hi[2] - r.o.z) * r.inv.z;
+// Slab test of a box given as its six slab distances (x lo/hi, y, z); the
+// far distance is widened by 2 ulp.
+inline bool slab_test(float tx0, float tx1, float ty0, float ty1, float tz0, float tz1, float tmin, float tbest,
+                      float& tnear) {
     float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000024f;
     tnear = tn;
     return tn <= tf && tf >= tmin && tn <= tbest;
+}
+// The two child boxes of a compressed node record against the ray: per axis
+// the origin's distance A = (origin - o) * inv and the step's B = 2^e * inv,
+// a bound q steps above the origin at fma(q, B, A).
+inline void qnode_test(const RayPre& r, const uint32_t* w, float tmin, float tbest, bool hit[2], float tn[2]) {
+    float org[3];
+    std::memcpy(org, w, 12);
+    const float oc[3] = {r.o.x, r.o.y, r.o.z}, ic[3] = {r.inv.x, r.inv.y, r.inv.z};
+    float A[3], B[3];
+    for (int a = 0; a < 3; a++) {
+        A[a] = (org[a] - oc[a]) * ic[a];
+        B[a] = std::ldexp(ic[a], (int)(int8_t)((w[4] >> (8 * a)) & 0xffu));
+    }
+    uint8_t q[12];
+    std::memcpy(q, &w[5], 12);
+    for (int c = 0; c < 2; c++) {
+        float t[6];
+        for (int a = 0; a < 3; a++) {
+            t[2 * a] = std::fma((float)q[6 * c + a], B[a], A[a]);
+            t[2 * a + 1] = std::fma((float)q[6 * c + 3 + a], B[a], A[a]);
+        }
+        hit[c] = slab_test(t[0], t[1], t[2], t[3], t[4], t[5], tmin, tbest, tn[c]);
+    }
 }
 struct Hit { float t; uint32_t prim; float u, v; };
 constexpr uint32_t kNoHit = 0xffffffffu;
@@ -1424,14 +1513,13 @@ Hit traverse(const Bvh& b, V3 o, V3 d, float tmin, float tmax, uint32_t* cnt) {
     int sp = 0;
     uint32_t node = 0;
     for (;;) {
-        uint32_t c[2] = {b.left[node], b.right[node]};
+        const uint32_t* w = &b.qnode[kQWords * (size_t)node];
+        const uint32_t g = w[3] & kQGamma;
+        uint32_t c[2] = {(w[3] & kQLeft) ? (kLeaf | g) : g, (w[3] & kQRight) ? (kLeaf | (g + 1)) : g + 1};
         nn += 2;
         float tn[2];
         bool hit[2];
-        for (int k = 0; k < 2; k++) {
-            int64_t idx = (c[k] & kLeaf) ? ni + (c[k] & ~kLeaf) : c[k];
-            hit[k] = box_test(r, &b.lo[3 * idx], &b.hi[3 * idx], tmin, h.t, tn[k]);
-        }
+        qnode_test(r, w, tmin, h.t, hit, tn);
         bool done = false;
         for (int k = 0; k < 2 && !done; k++) {   // any hit stops at the first occluder
             if (hit[k] && (c[k] & kLeaf)) {
@@ -1832,7 +1920,7 @@ void orc_morton_codes(int64_t n, const float* positions, uint32_t* codes, float*
 }
 
 int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t* order, uint32_t* left,
-                   uint32_t* right, uint32_t* parent, float* lo, float* hi) {
+                   uint32_t* right, uint32_t* parent, float* lo, float* hi, uint32_t* qnode) {
     Bvh b;
     build_bvh(b, n, positions);
     if (n == 0) return 0;
@@ -1845,6 +1933,7 @@ int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t*
     if (parent) std::memcpy(parent, b.parent.data(), (2 * n - 1) * 4);
     std::memcpy(lo, b.lo.data(), (2 * n - 1) * 12);
     std::memcpy(hi, b.hi.data(), (2 * n - 1) * 12);
+    if (qnode && n > 1) std::memcpy(qnode, b.qnode.data(), b.qnode.size() * 4);
     return 0;
 }
 

@@ -65,6 +65,8 @@ def test_lbvh_bit_exact(ctx, name):
         assert np.array_equal(right, ref["right"])
         assert np.array_equal(lo.view(np.uint32), ref["lo"].view(np.uint32))
         assert np.array_equal(hi.view(np.uint32), ref["hi"].view(np.uint32))
+        # the traversal's compressed records, computed on the device in double
+        assert np.array_equal(ctx.download_nodes(sid, pos.shape[0]), ref["qnode"])
     finally:
         ctx.release(sid)
 

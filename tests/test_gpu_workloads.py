@@ -45,6 +45,7 @@ def _lbvh_equal(ctx, sc, label):
     try:
         ctx.build_bvh(sid)
         codes, order, left, right, lo, hi = ctx.download_bvh(sid, n)
+        qnode = ctx.download_nodes(sid, n)
     finally:
         ctx.release(sid)
     ref = oracle_lib.lbvh(sc.positions)
@@ -54,6 +55,7 @@ def _lbvh_equal(ctx, sc, label):
     assert np.array_equal(right, ref["right"])
     assert np.array_equal(lo.view(np.uint32), ref["lo"].view(np.uint32))
     assert np.array_equal(hi.view(np.uint32), ref["hi"].view(np.uint32))
+    assert np.array_equal(qnode, ref["qnode"])
     shared = 1.0 - np.unique(codes).size / n
     print(f"[lbvh] {label}: bit-exact; {shared:.1%} of triangles share a Morton code", flush=True)
 

@@ -136,6 +136,52 @@ def test_lbvh_boxes_contain_children_and_triangles():
         assert np.array_equal(hi[i], np.maximum(hi[js[0]], hi[js[1]]))
 
 
+@pytest.mark.parametrize("seed,spread,size", [(11, 10.0, 0.5), (12, 1000.0, 0.01), (13, 0.01, 0.001)])
+def test_compressed_nodes_bound_their_children(seed, spread, size):
+    # The traversal's 32-byte node records, decoded here in float64 from
+    # their documented layout: each child's box widened outwards by at least
+    # half a step (or starting at the origin, the node's own corner), within
+    # 255 steps, at the smallest power-of-two step that spans the node in
+    # 253; gamma and the leaf flags name the Karras children.
+    pos = soup(2000, seed, spread, size)
+    pos[::7, :, 1] = pos[::7, :1, 1]                        # some flat triangles: zero extents
+    r = oracle_lib.lbvh(pos)
+    n = pos.shape[0]
+    lo, hi, q = r["lo"].astype(np.float64), r["hi"].astype(np.float64), r["qnode"]
+    assert q.shape == (n - 1, 8)
+    origin = q[:, 0:3].view(np.float32).astype(np.float64)
+    assert np.array_equal(origin, lo[: n - 1])
+    gamma = q[:, 3] & 0x3FFFFFFF
+    left = np.where(q[:, 3] & (1 << 30), LEAF | gamma, gamma).astype(np.uint32)
+    right = np.where(q[:, 3] & (1 << 31), LEAF | (gamma + 1), gamma + 1).astype(np.uint32)
+    assert np.array_equal(left, r["left"]) and np.array_equal(right, r["right"])
+    e = np.stack([((q[:, 4] >> (8 * a)) & 0xFF).astype(np.uint8).view(np.int8) for a in range(3)], 1).astype(int)
+    step = np.ldexp(1.0, e)
+    ext = hi[: n - 1] - lo[: n - 1]
+    assert np.all(ext <= 253 * step)
+    assert np.all((e == -64) | (ext > 253 * step / 2))      # the smallest such step
+    qb = q[:, 5:8].copy().view(np.uint8).reshape(n - 1, 12).astype(np.float64)
+    for c, refs in ((0, r["left"]), (1, r["right"])):
+        idx = np.where(refs & LEAF, n - 1 + (refs & ~np.uint32(LEAF)), refs).astype(np.int64)
+        qlo, qhi = qb[:, 6 * c: 6 * c + 3], qb[:, 6 * c + 3: 6 * c + 6]
+        blo, bhi = origin + qlo * step, origin + qhi * step
+        assert np.all(blo <= lo[idx]) and np.all(bhi >= hi[idx])
+        assert np.all((qlo == 0) | (lo[idx] - blo >= 0.5 * step))
+        assert np.all(bhi - hi[idx] >= 0.5 * step)
+        assert np.all(qhi <= 255)
+
+
+def test_compressed_traversal_finds_the_exact_boxes_hits():
+    # the traversal over the compressed records returns the hits of brute
+    # force over every triangle on rays grazing the soup in every direction
+    pos = soup(400, 21, spread=5.0, size=0.3)
+    rays = rays_for(pos, 4000, 22)
+    hits, counts = oracle_lib.Bvh(pos).closest(rays)
+    ref = oracle_lib.brute_closest(pos, rays)
+    assert np.array_equal(hits[:, :2].view(np.uint32), ref[:, :2].view(np.uint32))
+    assert np.all(counts[:, 0] % 2 == 0)                       # two child boxes per record fetched
+
+
 def rays_for(pos, n, seed):
     rng = np.random.default_rng(seed)
     lo, hi = pos.reshape(-1, 3).min(0), pos.reshape(-1, 3).max(0)
