@@ -457,31 +457,37 @@ def main():
         counted = step(abi.OXR_FLAG_COUNT_TRAVERSAL)   # untimed: algorithmic bytes of the same frames
         print(f"[bench] counting pass done: {counted.extension_rays} ext rays, {counted.shadow_rays} shadow rays",
               file=sys.stderr, flush=True)
+    # The timed steps are the product path: each render replays its captured
+    # iteration graphs.  (--profile-steps: eager launches, one kernel at a
+    # time, for the profiler.)
+    timed_flags = abi.OXR_FLAG_STAGE_TIMING if args.profile_steps else 0
     for i in range(args.warmup):
-        step()
+        step(timed_flags)
         print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    ctx_ms, trace_ms, shade_ms, regen_ms = [], [], [], []
+    ctx_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = step()
+        st = step(timed_flags)
         ctx_ms.append(st.render_ms)
-        trace_ms.append(st.trace_ms)
-        shade_ms.append(st.shade_ms)
-        regen_ms.append(st.generate_ms)
-        print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms (trace {st.trace_ms:.1f}, shade "
-              f"{st.shade_ms:.1f}, regenerate {st.generate_ms:.1f}, accumulate "
-              f"{st.accumulate_ms:.1f}; {st.iterations} iterations, {st.kernel_launches} launches)",
+        print(f"[bench] step {i + 1}/{args.steps}: {st.render_ms:.1f} ms ({st.iterations} iterations)",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    graph_ms = None
-    if not args.profile_steps:   # the same steps replayed as captured hipGraphs, for comparison
-        step(0)
-        tg = time.perf_counter()
-        for i in range(args.steps):
-            step(0)
-        graph_ms = (time.perf_counter() - tg) * 1e3 / args.steps
-        print(f"[bench] graph replay: {graph_ms:.1f} ms/step", file=sys.stderr, flush=True)
+    # Stage times from eager launches with HIP events between the stages
+    # (the roofline's kernel time): a few more steps, untimed by the wall clock.
+    trace_ms, shade_ms, regen_ms, eager_ms = [], [], [], []
+    for i in range(0 if args.profile_steps else 3):
+        sst = step()
+        eager_ms.append(sst.render_ms)
+        trace_ms.append(sst.trace_ms)
+        shade_ms.append(sst.shade_ms)
+        regen_ms.append(sst.generate_ms)
+        print(f"[bench] staged step {i + 1}: {sst.render_ms:.1f} ms (trace {sst.trace_ms:.1f}, shade "
+              f"{sst.shade_ms:.1f}, regenerate {sst.generate_ms:.1f}, accumulate "
+              f"{sst.accumulate_ms:.1f}; {sst.iterations} iterations, {sst.kernel_launches} launches)",
+              file=sys.stderr, flush=True)
+    if not trace_ms:
+        trace_ms, shade_ms, regen_ms, eager_ms = [st.trace_ms], [st.shade_ms], [st.generate_ms], [st.render_ms]
     paths = W * H * spp
     value = paths / (wall / args.steps) / 1e6
     out = {
@@ -505,11 +511,11 @@ def main():
                    "regenerate_ms_per_step": round(float(np.mean(regen_ms)), 3),
                    "iterations_per_step": int(st.iterations), "launches_per_step": int(st.kernel_launches),
                    "tuning": ctx.get_tuning().as_dict(),
-                   "launch": "eager launches with HIP events between stages"},
+                   "launch": ("eager launches with HIP events between stages" if args.profile_steps else
+                              "captured iteration graphs (the product path); stage times from eager steps with "
+                              "HIP events between stages"),
+                   "eager_ms_per_step": round(float(np.mean(eager_ms)), 3)},
     }
-    if graph_ms is not None:
-        out["config"]["graph_replay_ms_per_step"] = round(graph_ms, 3)
-        out["config"]["graph_replay_Mpaths_s"] = round(paths / (graph_ms / 1e3) / 1e6, 3)
     if counted is not None:
         out["roofline"] = roofline(counted, float(np.mean(trace_ms)), st.iterations, TRAFFIC_FILE,
                                    out["config"]["workload"], tuning_key(ctx.get_tuning()))
