@@ -1467,7 +1467,11 @@ inline void qnode_test(const RayPre& r, const uint32_t* w, float tmin, float tbe
     float A[3], B[3];
     for (int a = 0; a < 3; a++) {
         A[a] = (org[a] - oc[a]) * ic[a];
-        B[a] = std::ldexp(ic[a], (int)(int8_t)((w[4] >> (8 * a)) & 0xffu));
+        // ic * 2^e, correctly rounded like ldexpf (2^e, |e| <= 64, is a normal float)
+        const uint32_t pbits = (uint32_t)((int)(int8_t)((w[4] >> (8 * a)) & 0xffu) + 127) << 23;
+        float p2;
+        std::memcpy(&p2, &pbits, 4);
+        B[a] = ic[a] * p2;
     }
     uint8_t q[12];
     std::memcpy(q, &w[5], 12);
