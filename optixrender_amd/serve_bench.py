@@ -176,7 +176,7 @@ def percentile(v, q):
 
 
 def run_arm(ctx: Context, resident, stream, plan_of, budget_gb: float = 160.0, tick_paths: int = 1 << 26,
-            depth: int = DEPTH, runners: int = 4):
+            depth: int = DEPTH, runners: int = 2):
     """Submit the stream at its arrival times; plan_of(request) -> (denoise,
     spp).  A reaper thread releases every finished request as soon as it is
     done (a finished request holds its buffers, and so its share of the HBM
