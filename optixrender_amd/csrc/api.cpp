@@ -194,23 +194,29 @@ void arrange_denoiser(const float* w, std::vector<uint16_t>& lanes, std::vector<
     int64_t groups = 0;
     for (int l = 0; l < kDnLayers; l++) {
         const int ci_n = kDnChannels[l], co_n = kDnChannels[l + 1];
-        const int steps = ci_n <= 16 ? 1 : 2;
         const float* W = w + pos;
         pos += (size_t)co_n * ci_n * 9;
         const float* B = w + pos;
         pos += co_n;
         offset[l] = groups;
-        for (int t = 0; t < 9; t++)
-            for (int s = 0; s < steps; s++)
+        // Layer 1 reads features packed three pixels wide (the pixel to the
+        // left, the pixel, the pixel to the right: 3 x 10 channels and 2 of
+        // zero), so it runs as a 3x1 convolution over 32 channels: its k at
+        // tap row t is (tap column k / 10, channel k % 10).  The others run
+        // 3x3 over 32 channels: k at tap t is the channel.
+        const bool packed = l == 0;
+        const int taps = packed ? 3 : 9;
+        for (int t = 0; t < taps; t++)
+            for (int s = 0; s < 2; s++)
                 for (int q = 0; q < 64; q++)
                     for (int e = 0; e < 8; e++) {
-                        const int co = out_channel(l, q % 32), ci = 16 * s + 8 * (q / 32) + e;
-                        const float v = (co >= 0 && co < co_n && ci < ci_n)
-                                            ? W[((size_t)(co * ci_n + ci) * 3 + t / 3) * 3 + t % 3]
-                                            : 0.0f;
+                        const int co = out_channel(l, q % 32), k = 16 * s + 8 * (q / 32) + e;
+                        const int ty = packed ? t : t / 3, tx = packed ? k / 10 : t % 3, ci = packed ? k % 10 : k;
+                        const bool in = co >= 0 && co < co_n && (packed ? k < 3 * ci_n : ci < ci_n);
+                        const float v = in ? W[((size_t)(co * ci_n + ci) * 3 + ty) * 3 + tx] : 0.0f;
                         lanes.push_back(to_bf16(v));
                     }
-        groups += 9 * steps * 64;
+        groups += taps * 2 * 64;
         for (int row = 0; row < 32; row++) {
             const int co = out_channel(l, row);
             if (co >= 0 && co < co_n) bias[l * 32 + row] = B[co];

@@ -6,7 +6,10 @@
 // reference's network is closed, so the network is our own:
 //
 //   features (10 per pixel): log(1 + colour) (3), albedo (3), normal (3),
-//       depth d / (1 + d) (1), stored as BF16 NHWC padded to 16 channels;
+//       depth d / (1 + d) (1), stored as BF16 packed three pixels wide (the
+//       left neighbour's, the pixel's, the right neighbour's: 30 channels and
+//       2 of zero), so that layer 1 runs as a 3x1 convolution over 32
+//       channels (12 MFMAs per row of 64 pixels instead of 18 over 16);
 //   five 3x3 convolutions 10 -> 32 -> 32 -> 32 -> 32 -> 25 (zero padding),
 //       ReLU after the first four, activations BF16 NHWC with 32 channels;
 //   softmax over the 25 outputs = a 5x5 kernel per pixel, applied to the
@@ -92,7 +95,11 @@ constexpr int kRingC = 32;               // channels of an activation ring (feat
 // or 48 bytes, odd multiples of 16): the 16-byte operand reads of a wave's
 // consecutive pixels then spread over the LDS banks.
 template <int C> constexpr int ring_stride() { return C + 8; }
-constexpr int kFusedWaves = 7;
+// Wave roles.  Waves go to SIMD wave % 4, and the two waves of a SIMD share
+// its matrix core and vector ALU, so the table spreads MFMAs and VALU work.
+enum DnRole { kLoader = 0, kLayer1, kLayer2, kLayer3, kLayer4, kLastTile0, kLastTile1, kIdle };
+constexpr int kRoles[] = {kLoader, kLastTile0, kLayer2, kLayer3, kLayer4, kLastTile1, kLayer1};
+constexpr int kFusedWaves = sizeof(kRoles) / sizeof(kRoles[0]);
 constexpr int kRingRows = 4;
 // colour ring rows: in step t the last layer filters row ys + t - 15 with the
 // colour of rows ys + t - 17 .. ys + t - 13 while the loader stores row ys + t
@@ -114,12 +121,12 @@ __device__ __forceinline__ void step_barrier() {
 // One layer's output row at ring pixels p0 .. p0 + 32 NT - 1: NT 32-pixel
 // tiles, NT independent MFMA accumulation chains (taps in the order tap row,
 // tap column, channel step), the B operands read from the input ring.
-template <int CIN, int NT>
+template <int CIN, int NT, int TX = 3>
 __device__ __forceinline__ void dn_layer_row(const uint16_t* __restrict__ ring, const bf16x8* a, int y, int p0,
                                              int lane, const f32x16& bias, f32x16 (&acc)[NT]) {
     constexpr int STEPS = CIN / 16;
     constexpr int PS = ring_stride<CIN>();
-    constexpr int G = 3 * STEPS * NT;            // operands (and MFMAs) of one tap row
+    constexpr int G = TX * STEPS * NT;           // operands (and MFMAs) of one tap row
     const int px = lane & 31, half = lane >> 5;
 #pragma unroll
     for (int tile = 0; tile < NT; tile++) acc[tile] = bias;  // the sum starts from the bias
@@ -128,9 +135,9 @@ __device__ __forceinline__ void dn_layer_row(const uint16_t* __restrict__ ring, 
     // behind the matrix core rather than stalling every MFMA.
     bf16x8 b[2][G];
     auto load = [&](int ty, bf16x8* dst) {
-        const uint16_t* row = ring + ring_slot(y - 1 + ty) * kRingW * PS + (p0 + px - 1) * PS + 8 * half;
+        const uint16_t* row = ring + ring_slot(y - 1 + ty) * kRingW * PS + (p0 + px - (TX == 3 ? 1 : 0)) * PS + 8 * half;
 #pragma unroll
-        for (int tx = 0; tx < 3; tx++)
+        for (int tx = 0; tx < TX; tx++)
 #pragma unroll
             for (int s = 0; s < STEPS; s++)
 #pragma unroll
@@ -143,8 +150,8 @@ __device__ __forceinline__ void dn_layer_row(const uint16_t* __restrict__ ring, 
     for (int ty = 0; ty < 3; ty++) {
         if (ty < 2) load(ty + 1, b[(ty + 1) & 1]);
 #pragma unroll
-        for (int j = 0; j < 3 * STEPS; j++) {
-            const int k = ty * 3 * STEPS + j;
+        for (int j = 0; j < TX * STEPS; j++) {
+            const int k = ty * TX * STEPS + j;
 #pragma unroll
             for (int tile = 0; tile < NT; tile++)
                 acc[tile] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k], b[ty & 1][NT * j + tile], acc[tile], 0, 0, 0);
@@ -256,7 +263,7 @@ __device__ __forceinline__ void finish_row(int p0, int lane, int y, const f32x16
 // step t is y_l(t) = ys + t + 1 - 3 l: three steps behind layer l - 1, so
 // that the row computed in step t is finished (stored) in step t + 1 while
 // the MFMAs of the next row run, and its readers take it in step t + 2.
-template <int CIN, bool LAST, int NT>
+template <int CIN, bool LAST, int NT, int TX = 3>
 __device__ __forceinline__ void layer_walk(int l, int tile, int lane, const uint16_t* feat,
                                            uint16_t (*act)[kRingRows * kRingW * ring_stride<kRingC>()],
                                            const float4* crow, const bf16x8* __restrict__ wl, const DenoiserDev& d,
@@ -264,7 +271,7 @@ __device__ __forceinline__ void layer_walk(int l, int tile, int lane, const uint
                                            float4* __restrict__ out) {
     const int half = lane >> 5;
     const int p0 = l + 32 * tile;                // its first ring pixel
-    constexpr int NA = 9 * CIN / 16;
+    constexpr int NA = 3 * TX * CIN / 16;
     bf16x8 a[NA];
     const bf16x8* src = wl + d.lane_offset[l - 1];
 #pragma unroll
@@ -290,11 +297,11 @@ __device__ __forceinline__ void layer_walk(int l, int tile, int lane, const uint
 #pragma unroll
         for (int i = 0; i < NT; i++) acc[i] = bias;
         if (compute && prev_needed) {            // the steady state: one block for the scheduler
-            dn_layer_row<CIN, NT>(in_ring, a, y, p0, lane, bias, acc);
+            dn_layer_row<CIN, NT, TX>(in_ring, a, y, p0, lane, bias, acc);
             finish_row<LAST, NT>(p0, lane, prev_y, prev, out_ring, ct, x0, y0, w, h, seg_h, out);
         } else {
             if (prev_needed) finish_row<LAST, NT>(p0, lane, prev_y, prev, out_ring, ct, x0, y0, w, h, seg_h, out);
-            if (compute) dn_layer_row<CIN, NT>(in_ring, a, y, p0, lane, bias, acc);
+            if (compute) dn_layer_row<CIN, NT, TX>(in_ring, a, y, p0, lane, bias, acc);
         }
         // row y's colour taps for the next step: its rows y - 2 .. y + 2 were
         // stored by the loader at least 12 steps ago, and the ring keeps 32
@@ -311,7 +318,8 @@ __global__ __launch_bounds__(64 * kFusedWaves, 2) void k_dn_fused(
     const float4* __restrict__ color, const float4* __restrict__ albedo, const float4* __restrict__ normal,
     const float* __restrict__ depth, const bf16x8* __restrict__ wl, DenoiserDev d, int w, int h, int seg_h,
     float4* __restrict__ out) {
-    __shared__ __align__(16) uint16_t feat[kRingRows * kRingW * ring_stride<16>()];
+    // features packed three pixels wide for layer 1 (see the loader)
+    __shared__ __align__(16) uint16_t feat[kRingRows * kRingW * ring_stride<32>()];
     __shared__ __align__(16) uint16_t act[4][kRingRows * kRingW * ring_stride<kRingC>()];
     // linear colour of the last 16 feature rows at clamped coordinates (the
     // 5x5 kernel's clamped borders), for the last layer
@@ -326,7 +334,8 @@ __global__ __launch_bounds__(64 * kFusedWaves, 2) void k_dn_fused(
     // Two roles, one loop each: every wave passes the same barrier once per
     // step (waves are wave-uniform in their role), and the roles' registers
     // never overlap.
-    if (wave == 0) {
+    const int role = kRoles[wave];
+    if (role == kLoader) {
         // Loader: the inputs of a row (colour at clamped coordinates, albedo,
         // normal, depth; zero albedo/normal/depth outside the image) are
         // fetched three steps before the row is stored, into one of two
@@ -371,15 +380,37 @@ __global__ __launch_bounds__(64 * kFusedWaves, 2) void k_dn_fused(
         // step t: store row ys + t (converted last step), convert row
         // ys + t + 1 from `next`, then refill `next` with row ys + t + 3
         auto step = [&](int t, Raw& next) {
+            // Layer 1 reads ring pixels 1 .. 64, each holding the 10 features
+            // of the pixel to its left, its own and the pixel to its right
+            // (32 channels, the last two zero): this lane's pixel `lane` and,
+            // in lane 0, pixel 64.  The neighbours' come from the lanes that
+            // converted them.
+            // Whole-wave DPP shifts (v_mov_b32 wave_shr:1 / wave_shl:1) and
+            // lane reads, no LDS traffic.
+            uint32_t lf[5], rt[5], lf64[5], rt64[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const uint32_t up = (uint32_t)__builtin_amdgcn_mov_dpp((int)fw[0][i], 0x138, 0xf, 0xf, false);
+                const uint32_t down = (uint32_t)__builtin_amdgcn_mov_dpp((int)fw[0][i], 0x130, 0xf, 0xf, false);
+                const uint32_t p64 = (uint32_t)__builtin_amdgcn_readlane((int)fw[1][i], 0);
+                lf[i] = up;                                  // pixel lane - 1
+                rt[i] = lane == 63 ? p64 : down;             // pixel lane + 1
+                lf64[i] = (uint32_t)__builtin_amdgcn_readlane((int)fw[0][i], 63);
+                rt64[i] = (uint32_t)__builtin_amdgcn_readlane((int)fw[1][i], 1);
+            }
+            auto put = [&](int p, const uint32_t* l, const uint32_t* c, const uint32_t* r) {
+                uint4* dst = reinterpret_cast<uint4*>(feat + (ring_slot(ys + t) * kRingW + p) * ring_stride<32>());
+                dst[0] = make_uint4(l[0], l[1], l[2], l[3]);
+                dst[1] = make_uint4(l[4], c[0], c[1], c[2]);
+                dst[2] = make_uint4(c[3], c[4], r[0], r[1]);
+                dst[3] = make_uint4(r[2], r[3], r[4], 0u);
+            };
+            put(lane, lf, fw[0], rt);
+            if (lane == 0) put(64, lf64, fw[1], rt64);
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 const int p = lane + 64 * k;
-                if (p < kRingW) {
-                    uint4* dst = reinterpret_cast<uint4*>(feat + (ring_slot(ys + t) * kRingW + p) * ring_stride<16>());
-                    dst[0] = make_uint4(fw[k][0], fw[k][1], fw[k][2], fw[k][3]);
-                    dst[1] = make_uint4(fw[k][4], fw[k][5], fw[k][6], fw[k][7]);
-                    crow[(t & (kColourRows - 1)) * kRingW + p] = cn[k];
-                }
+                if (p < kRingW) crow[(t & (kColourRows - 1)) * kRingW + p] = cn[k];
             }
             if (t + 1 < steps) {
                 convert(ys + t + 1, next);
@@ -398,17 +429,15 @@ __global__ __launch_bounds__(64 * kFusedWaves, 2) void k_dn_fused(
             step(t + 1, r0);
         }
         if (t < steps) step(t, r1);
+    } else if (role == kIdle) {
+        for (int t = 0; t < steps; t++) step_barrier();
+    } else if (role == kLayer1) {
+        layer_walk<32, false, 2, 1>(1, 0, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps, out);
+    } else if (role == kLastTile0 || role == kLastTile1) {
+        layer_walk<32, true, 1>(5, role == kLastTile1, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps,
+                                out);
     } else {
-        // Waves go to SIMD wave % 4.  MFMAs and VALU work per step: loader +
-        // layer 4 on SIMD 0 (36 MFMA), the last layer's two tiles on SIMD 1
-        // (36 MFMA, the most VALU), layers 2 and 1 on SIMD 2 (54 MFMA, little
-        // VALU), layer 3 on SIMD 3 (36 MFMA).  (Splitting layer 1 into two
-        // single-tile waves on SIMDs 2 and 3 measured slower.)
-        if (wave == 6)
-            layer_walk<16, false, 2>(1, 0, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps, out);
-        else if (wave == 1 || wave == 5)
-            layer_walk<32, true, 1>(5, wave == 5, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps, out);
-        else layer_walk<32, false, 2>(wave, 0, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps, out);
+        layer_walk<32, false, 2>(role, 0, lane, feat, act, crow, wl, d, x0, y0, ys, w, h, seg_h, steps, out);
     }
 }
 
