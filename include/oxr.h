@@ -185,8 +185,15 @@ int oxr_synchronize(oxr_ctx* ctx);
 
 /* ---- scenes (Scene / BuildAccel / CreateTextures / BuildSBT) ---- */
 int oxr_scene_upload(oxr_ctx* ctx, const oxr_scene_desc* desc, uint32_t* scene_id);
-/* LBVH build on the device (replaces optixAccelBuild/Compact, Renderer.cpp:247-274). */
+/* BVH build on the device (replaces optixAccelBuild/Compact, Renderer.cpp:247-274):
+   an LBVH (Morton codes, radix sort, Karras hierarchy, refit), one round of
+   treelet restructuring (Karras & Aila 2013) relabelled as Karras's layout of
+   its own leaf order, and the traversal's compressed node records. */
 int oxr_scene_build_bvh(oxr_ctx* ctx, uint32_t scene_id, float* build_ms);
+/* The same with `rounds` treelet rounds (0: the plain LBVH; -1: the default
+   of oxr_scene_build_bvh; at most 4).  No reference counterpart: OptiX's
+   build quality is not exposed. */
+int oxr_scene_build_bvh_rounds(oxr_ctx* ctx, uint32_t scene_id, int32_t rounds, float* build_ms);
 int oxr_scene_release(oxr_ctx* ctx, uint32_t scene_id);
 int oxr_scene_select(oxr_ctx* ctx, uint32_t scene_id);
 /* BVH of a built scene in Karras layout: codes/order: n; left/right: n-1

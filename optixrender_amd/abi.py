@@ -302,6 +302,7 @@ SIGNATURES = {
     "oxr_synchronize": (C.c_int, [_P]),
     "oxr_scene_upload": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
     "oxr_scene_build_bvh": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float)]),
+    "oxr_scene_build_bvh_rounds": (C.c_int, [_P, C.c_uint32, C.c_int32, C.POINTER(C.c_float)]),
     "oxr_scene_release": (C.c_int, [_P, C.c_uint32]),
     "oxr_scene_select": (C.c_int, [_P, C.c_uint32]),
     "oxr_scene_download_bvh": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P]),

@@ -528,7 +528,12 @@ int oxr_scene_upload(oxr_ctx* c, const oxr_scene_desc* d, uint32_t* scene_id) {
 }
 
 int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
-    if (!c) return OXR_ERR_INVALID;
+    return oxr_scene_build_bvh_rounds(c, id, -1, build_ms);
+}
+
+int oxr_scene_build_bvh_rounds(oxr_ctx* c, uint32_t id, int32_t rounds, float* build_ms) {
+    if (!c || rounds > 4) return OXR_ERR_INVALID;
+    if (rounds < 0) rounds = kTreeletRoundsDefault;
     auto it = c->scenes.find(id);
     if (it == c->scenes.end()) return fail(c, OXR_ERR_INVALID, "unknown scene id");
     SceneDev* s = it->second;
@@ -562,7 +567,7 @@ int oxr_scene_build_bvh(oxr_ctx* c, uint32_t id, float* build_ms) {
     OXR_HIP(c, hipEventRecord(e0, c->stream));
     hipError_t e = hipSuccess;
     if (n > 0) {
-        e = bvh_build(s->positions, n, s->bvh, scratch, c->stream);
+        e = bvh_build(s->positions, n, s->bvh, scratch, c->stream, rounds);
         if (e == hipSuccess)
             e = gather_sorted_attributes(s->normals, s->mat_ids, s->uvs, s->bvh.order, n, s->normals_sorted,
                                          s->mat_sorted, s->uvs_sorted, c->stream);

@@ -340,7 +340,266 @@ __global__ __launch_bounds__(256) void k_quantize(int64_t ni, const uint32_t* __
     qnodes[2 * v + 1] = make_uint4(w4, q[0], q[1], q[2]);
 }
 
+// ---- treelet restructuring (Karras & Aila 2013), then Karras relabelling ----
+// Each round climbs the tree bottom up (the second thread to reach a node
+// goes on, as in the refit); at a node it recomputes box, leaf count and SAH
+// cost from the children and, if at least kTreeletLeaves << round leaves lie
+// below it, optimises its treelet: the treelet leaf (internal) of largest
+// area is replaced by its two children until there are seven, a dynamic
+// program over the subsets of those finds the binary tree of least SAH cost,
+// and if it is cheaper the treelet's internal nodes are rewired to it (the
+// root keeps its id; the others are reused in the order they were expanded,
+// depth first, left part first).  The tree is then relabelled as Karras's
+// layout of its own leaf order: every node's leaves are consecutive, internal
+// ids follow the endpoint rule (a left child takes the last position of its
+// range, a right child the first), so a node's children are again gamma and
+// gamma + 1.  Reads and writes of the tree during the climb are relaxed
+// device-scope atomics, so a thread sees what the threads below it wrote.
+constexpr int kTreeletLeaves = 7;
+constexpr float kCostNode = 1.2f, kCostTri = 1.0f;
+
+template <typename T>
+__device__ __forceinline__ T ald(const T* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T>
+__device__ __forceinline__ void ast(T* p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ __forceinline__ float box_area(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * ((dx * dy + dy * dz) + dz * dx);
+}
+
+struct TreeDev {
+    int64_t ni;
+    uint32_t *left, *right, *parent, *nl;
+    float *lo, *hi, *cost;
+    __device__ int64_t index(uint32_t ref) const { return (ref & kLeafBit) ? ni + (ref & ~kLeafBit) : ref; }
+    __device__ void box(int64_t x, float l[3], float h[3]) const {
+#pragma unroll
+        for (int k = 0; k < 3; k++) { l[k] = ald(&lo[3 * x + k]); h[k] = ald(&hi[3 * x + k]); }
+    }
+    // internal node v from its two children: box, leaf count and SAH cost
+    __device__ void join(int64_t v) const {
+        const int64_t a = index(ald(&left[v])), c = index(ald(&right[v]));
+        float la[3], ha[3], lc[3], hc[3], l[3], h[3];
+        box(a, la, ha);
+        box(c, lc, hc);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l[k] = fminf(la[k], lc[k]);
+            h[k] = fmaxf(ha[k], hc[k]);
+            ast(&lo[3 * v + k], l[k]);
+            ast(&hi[3 * v + k], h[k]);
+        }
+        ast(&nl[v], ald(&nl[a]) + ald(&nl[c]));
+        ast(&cost[v], kCostNode * box_area(l, h) + ald(&cost[a]) + ald(&cost[c]));
+    }
+};
+
+__device__ void treelet(const TreeDev& T, int64_t v) {
+    uint32_t ref[kTreeletLeaves], inner[kTreeletLeaves];
+    int m = 2, ninner = 0;
+    ref[0] = ald(&T.left[v]);
+    ref[1] = ald(&T.right[v]);
+    while (m < kTreeletLeaves) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int i = 0; i < m; i++) {
+            if (ref[i] & kLeafBit) continue;
+            float l[3], h[3];
+            T.box(ref[i], l, h);
+            const float a = box_area(l, h);
+            if (a > best_area) { best_area = a; best = i; }
+        }
+        if (best < 0) break;
+        const uint32_t c = ref[best];
+        inner[ninner++] = c;
+        ref[best] = ald(&T.left[c]);
+        ref[m++] = ald(&T.right[c]);
+    }
+    if (m < 3) return;
+    const int full = (1 << m) - 1;
+    float slo[1 << kTreeletLeaves][3], shi[1 << kTreeletLeaves][3], scost[1 << kTreeletLeaves];
+    uint8_t part[1 << kTreeletLeaves];
+    for (int s = 1; s <= full; s++) {
+        if ((s & (s - 1)) == 0) {
+            const int64_t x = T.index(ref[__builtin_ctz(s)]);
+            T.box(x, slo[s], shi[s]);
+            scost[s] = ald(&T.cost[x]);
+            continue;
+        }
+        const int top = 31 - __builtin_clz((unsigned)s), rest = s & ~(1 << top);
+        float l[3], h[3];
+        T.box(T.index(ref[top]), l, h);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            slo[s][k] = fminf(slo[rest][k], l[k]);
+            shi[s][k] = fmaxf(shi[rest][k], h[k]);
+        }
+        const int low = s & -s;
+        float best = __builtin_inff();
+        int bp = 0;
+        for (int q = (s - 1) & s; q > 0; q = (q - 1) & s) {
+            if (!(q & low)) continue;
+            const float c = scost[q] + scost[s ^ q];
+            if (c < best) { best = c; bp = q; }
+        }
+        scost[s] = kCostNode * box_area(slo[s], shi[s]) + best;
+        part[s] = (uint8_t)bp;
+    }
+    if (!(scost[full] < ald(&T.cost[v]))) return;
+    int next = 0;
+    int64_t st_node[kTreeletLeaves], order_nodes[kTreeletLeaves];
+    int st_set[kTreeletLeaves];
+    int sp = 0, nn = 0;
+    st_node[sp] = v;
+    st_set[sp++] = full;
+    while (sp > 0) {
+        --sp;
+        const int64_t node = st_node[sp];
+        const int set = st_set[sp];
+        const int pl = part[set], pr = set ^ pl;
+        const uint32_t rl = (pl & (pl - 1)) == 0 ? ref[__builtin_ctz(pl)] : inner[next++];
+        const uint32_t rr = (pr & (pr - 1)) == 0 ? ref[__builtin_ctz(pr)] : inner[next++];
+        ast(&T.left[node], rl);
+        ast(&T.right[node], rr);
+        ast(&T.parent[T.index(rl)], (uint32_t)node);
+        ast(&T.parent[T.index(rr)], (uint32_t)node);
+        order_nodes[nn++] = node;
+        if ((pr & (pr - 1)) != 0) { st_node[sp] = rr; st_set[sp++] = pr; }
+        if ((pl & (pl - 1)) != 0) { st_node[sp] = rl; st_set[sp++] = pl; }
+    }
+    for (int i = nn - 1; i >= 0; i--) T.join(order_nodes[i]);
+}
+
+// leaves: leaf count 1 and SAH cost kCostTri * area
+__global__ void k_tree_leaves(int64_t n, TreeDev T) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t x = T.ni + k;
+    float l[3], h[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) { l[a] = T.lo[3 * x + a]; h[a] = T.hi[3 * x + a]; }
+    T.nl[x] = 1;
+    T.cost[x] = kCostTri * box_area(l, h);
+}
+
+// one round: thread per leaf climbs; the second arrival joins the node and,
+// with gamma leaves or more below it (gamma 0: join only), optimises its treelet
+__global__ __launch_bounds__(256) void k_treelet_round(int64_t n, TreeDev T, uint32_t gamma, uint32_t* flags,
+                                                       uint32_t* error) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t node = ald(&T.parent[T.ni + k]);
+    for (int depth = 0;; depth++) {
+        if (depth > 4096 || node >= (uint32_t)T.ni) {
+            atomicOr(error, 1u);
+            return;
+        }
+        __threadfence();
+        if (atomicAdd(&flags[node], 1u) == 0u) return;  // the other child's thread is not done yet
+        __threadfence();
+        T.join(node);
+        if (gamma > 0 && ald(&T.nl[node]) >= gamma) treelet(T, node);
+        __threadfence();
+        if (node == 0) return;
+        node = ald(&T.parent[node]);
+    }
+}
+
+// Relabelling: the leaf position where every node's range starts (the
+// leaves of the left siblings on its path from the root), then ids.
+__global__ void k_first(int64_t n, const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                        const uint32_t* __restrict__ parent, const uint32_t* __restrict__ nl, uint32_t* first,
+                        uint32_t* error) {
+    const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = n - 1;
+    if (x >= 2 * n - 1) return;
+    uint32_t f = 0;
+    int64_t c = x;
+    for (int depth = 0; c != 0; depth++) {
+        const uint32_t p = parent[c];
+        if (depth > 4096 || p >= (uint32_t)ni) {  // a malformed hierarchy
+            atomicOr(error, 1u);
+            break;
+        }
+        const uint32_t lc = left[p];
+        const int64_t li = (lc & kLeafBit) ? ni + (lc & ~kLeafBit) : lc;
+        if (li != c) f += nl[li];  // c is the right child
+        c = p;
+    }
+    first[x] = f;
+}
+
+__global__ void k_relabel(int64_t n, const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                          const uint32_t* __restrict__ parent, const uint32_t* __restrict__ nl,
+                          const uint32_t* __restrict__ first, uint32_t* newid) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = n - 1;
+    if (v >= ni) return;
+    if (v == 0) { newid[0] = 0; return; }
+    const uint32_t p = parent[v];
+    newid[v] = left[p] == (uint32_t)v ? first[v] + nl[v] - 1 : first[v];
+}
+
+__global__ void k_scatter_nodes(int64_t n, const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                                const float* __restrict__ lo, const float* __restrict__ hi,
+                                const uint32_t* __restrict__ first, const uint32_t* __restrict__ newid, uint32_t* L2,
+                                uint32_t* R2, uint32_t* P2, float* lo2, float* hi2) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = n - 1;
+    if (v >= ni) return;
+    auto remap = [&](uint32_t ref) -> uint32_t {
+        return (ref & kLeafBit) ? (kLeafBit | first[ni + (ref & ~kLeafBit)]) : newid[ref];
+    };
+    const uint32_t w = newid[v];
+    const uint32_t a = remap(left[v]), c = remap(right[v]);
+    L2[w] = a;
+    R2[w] = c;
+    P2[(a & kLeafBit) ? ni + (a & ~kLeafBit) : a] = w;
+    P2[(c & kLeafBit) ? ni + (c & ~kLeafBit) : c] = w;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { lo2[3 * w + k] = lo[3 * v + k]; hi2[3 * w + k] = hi[3 * v + k]; }
+}
+
+__global__ void k_scatter_leaves(int64_t n, const float* __restrict__ lo, const float* __restrict__ hi,
+                                 const uint32_t* __restrict__ order, const float4* __restrict__ tris,
+                                 const uint32_t* __restrict__ first, float* lo2, float* hi2, uint32_t* order2,
+                                 float4* tris2) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = n - 1;
+    if (k >= n) return;
+    const int64_t q = first[ni + k];
+    order2[q] = order[k];
+#pragma unroll
+    for (int i = 0; i < 3; i++) tris2[3 * q + i] = tris[3 * k + i];
+#pragma unroll
+    for (int a = 0; a < 3; a++) { lo2[3 * (ni + q) + a] = lo[3 * (ni + k) + a]; hi2[3 * (ni + q) + a] = hi[3 * (ni + k) + a]; }
+}
+
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// Word offsets of the restructuring's arrays in the build scratch, after the
+// sort's (each aligned to 64 words).
+struct TreeScratch {
+    size_t cost, nl, first, newid, L2, R2, P2, lo2, hi2, order2, tris2, end;
+    explicit TreeScratch(int64_t n) {
+        const int64_t nb = (n + kSortTile - 1) / kSortTile;
+        auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
+        size_t o = al((size_t)(4 * n + (int64_t)kRadix * nb + 8 + n + 64));
+        cost = o; o = al(o + 2 * n);
+        nl = o; o = al(o + 2 * n);
+        first = o; o = al(o + 2 * n);
+        newid = o; o = al(o + n);
+        L2 = o; o = al(o + n);
+        R2 = o; o = al(o + n);
+        P2 = o; o = al(o + 2 * n);
+        lo2 = o; o = al(o + 6 * n);
+        hi2 = o; o = al(o + 6 * n);
+        order2 = o; o = al(o + n);
+        tris2 = o; o = al(o + 12 * n);
+        end = o;
+    }
+};
 
 }  // namespace
 
@@ -350,12 +609,12 @@ uint32_t* bvh_error_word(void* scratch, int64_t n) {
 }
 
 size_t bvh_scratch_bytes(int64_t n) {
-    int64_t nb = (n + kSortTile - 1) / kSortTile;
-    // keys/vals ping-pong (4n words) + histogram + bounds + flags
-    return (size_t)(4 * n + (int64_t)kRadix * nb + 8 + n + 64) * sizeof(uint32_t);
+    // keys/vals ping-pong (4n words) + histogram + bounds + flags, then the
+    // restructuring's costs, leaf counts and relabelled copies
+    return TreeScratch(n).end * sizeof(uint32_t);
 }
 
-hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, hipStream_t s) {
+hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, hipStream_t s, int rounds) {
     if (n <= 0) return hipSuccess;
     uint32_t* w = static_cast<uint32_t*>(scratch);
     uint32_t* k0 = w;
@@ -390,6 +649,36 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
     }
     k_refit<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.left, o.right, o.parent, flags, o.lo, o.hi,
                                                o.tris, bounds + 6);
+    if (n > 2 && rounds > 0) {
+        const TreeScratch ts(n);
+        float* cost = reinterpret_cast<float*>(w + ts.cost);
+        uint32_t* nl = w + ts.nl;
+        TreeDev T{n - 1, o.left, o.right, o.parent, nl, o.lo, o.hi, cost};
+        k_tree_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, T);
+        for (int r = 0; r < rounds; r++) {
+            if ((e = hipMemsetAsync(flags, 0, (size_t)(n - 1) * sizeof(uint32_t), s)) != hipSuccess) return e;
+            k_treelet_round<<<blocks_for(n, 256), 256, 0, s>>>(n, T, (uint32_t)kTreeletLeaves << r, flags, bounds + 6);
+        }
+        uint32_t *first = w + ts.first, *newid = w + ts.newid, *L2 = w + ts.L2, *R2 = w + ts.R2, *P2 = w + ts.P2,
+                 *order2 = w + ts.order2;
+        float *lo2 = reinterpret_cast<float*>(w + ts.lo2), *hi2 = reinterpret_cast<float*>(w + ts.hi2);
+        float4* tris2 = reinterpret_cast<float4*>(w + ts.tris2);
+        k_first<<<blocks_for(2 * n - 1, 256), 256, 0, s>>>(n, o.left, o.right, o.parent, nl, first, bounds + 6);
+        k_relabel<<<blocks_for(n - 1, 256), 256, 0, s>>>(n, o.left, o.right, o.parent, nl, first, newid);
+        k_scatter_nodes<<<blocks_for(n - 1, 256), 256, 0, s>>>(n, o.left, o.right, o.lo, o.hi, first, newid, L2, R2,
+                                                                P2, lo2, hi2);
+        k_scatter_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, o.lo, o.hi, o.order, o.tris, first, lo2, hi2, order2,
+                                                             tris2);
+        const size_t ni = (size_t)(n - 1);
+        if ((e = hipMemcpyAsync(o.left, L2, 4 * ni, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.right, R2, 4 * ni, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.parent + 1, P2 + 1, 4 * (2 * n - 2), hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.lo, lo2, 12 * (2 * n - 1), hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.hi, hi2, 12 * (2 * n - 1), hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.order, order2, 4 * n, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(o.tris, tris2, 48 * n, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+            return e;
+    }
     if (n > 1) k_quantize<<<blocks_for(n - 1, 256), 256, 0, s>>>(n - 1, o.left, o.right, o.lo, o.hi, o.qnodes);
     return hipGetLastError();
 }

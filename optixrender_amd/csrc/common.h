@@ -56,7 +56,10 @@ struct Counters64 {
 size_t bvh_scratch_bytes(int64_t n);
 // Device word set non-zero by the build if the hierarchy came out malformed.
 uint32_t* bvh_error_word(void* d_scratch, int64_t n);
-hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void* d_scratch, hipStream_t s);
+// rounds: treelet restructuring rounds after the LBVH (0: the plain LBVH).
+constexpr int kTreeletRoundsDefault = 1;
+hipError_t bvh_build(const float* d_positions, int64_t n, BvhBuffers& out, void* d_scratch, hipStream_t s,
+                     int rounds);
 
 // ---- launchers (render.hip) ----
 // gstack: (kStackTotal - kStackLdsMin) * grid * kTraceBlock words; overflow:

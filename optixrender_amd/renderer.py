@@ -117,9 +117,11 @@ class Context:
         self.check(self._lib.oxr_scene_upload(self.h, C.byref(d), C.byref(sid)), "oxr_scene_upload")
         return sid.value
 
-    def build_bvh(self, sid: int) -> float:
+    def build_bvh(self, sid: int, rounds: int = -1) -> float:
+        """Device BVH build; rounds of treelet restructuring after the LBVH
+        (0: the plain LBVH, -1: the library default)."""
         ms = C.c_float()
-        self.check(self._lib.oxr_scene_build_bvh(self.h, sid, C.byref(ms)), "oxr_scene_build_bvh")
+        self.check(self._lib.oxr_scene_build_bvh_rounds(self.h, sid, rounds, C.byref(ms)), "oxr_scene_build_bvh")
         return ms.value
 
     def select(self, sid: int):

@@ -1318,7 +1318,212 @@ void quantize_node(int64_t ni, const uint32_t* left, const uint32_t* right, cons
         }
     std::memcpy(&out[5], q, 12);
 }
-void build_bvh(Bvh& b, int64_t n, const float* pos) {
+// ---- treelet restructuring (Karras & Aila 2013, "Fast Parallel Construction
+// of High-Quality Bounding Volume Hierarchies"), then Karras relabelling ----
+// Each round visits the internal nodes bottom up; a node with at least
+// 7 << round leaves below it roots a treelet: starting from its two children,
+// the treelet leaf (internal) of largest surface area is replaced by its two
+// children until there are 7.  Over the subsets of those treelet leaves a
+// dynamic program finds the binary tree of least SAH cost
+// (kCostNode * area per internal node, plus the treelet leaves' own subtree
+// costs); if it is cheaper than the current one, the treelet's internal
+// nodes are rewired to it (the root keeps its id, the others are reused in
+// the order they were expanded).  The hierarchy is then relabelled as
+// Karras's layout of its own leaf order (left to right): every node's leaves
+// are consecutive, and internal node ids are the same endpoint rule (the
+// left child of a node takes the last leaf position of its range, the right
+// child the first), so that the two children of a node are again gamma and
+// gamma + 1.
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletRounds = 1;
+constexpr float kCostNode = 1.2f, kCostTri = 1.0f;
+
+inline float box_area(const float* lo, const float* hi) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * ((dx * dy + dy * dz) + dz * dx);
+}
+
+struct TreeletState {
+    int64_t ni;
+    uint32_t *left, *right, *parent;
+    float *lo, *hi, *cost;
+    uint32_t* nleaves;
+    int64_t index(uint32_t ref) const { return (ref & kLeaf) ? ni + (ref & ~kLeaf) : ref; }
+    // internal node v from its two children: box, leaf count and SAH cost
+    void join(int64_t v) {
+        const int64_t a = index(left[v]), c = index(right[v]);
+        for (int k = 0; k < 3; k++) {
+            lo[3 * v + k] = fminf(lo[3 * a + k], lo[3 * c + k]);
+            hi[3 * v + k] = fmaxf(hi[3 * a + k], hi[3 * c + k]);
+        }
+        nleaves[v] = nleaves[a] + nleaves[c];
+        cost[v] = kCostNode * box_area(&lo[3 * v], &hi[3 * v]) + cost[a] + cost[c];
+    }
+};
+
+// Optimises the treelet rooted at internal node v (its subtrees are final).
+void treelet(TreeletState& T, int64_t v) {
+    uint32_t ref[kTreeletLeaves];
+    uint32_t inner[kTreeletLeaves];
+    int m = 2, ninner = 0;
+    ref[0] = T.left[v];
+    ref[1] = T.right[v];
+    while (m < kTreeletLeaves) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int i = 0; i < m; i++) {
+            if (ref[i] & kLeaf) continue;
+            const float a = box_area(&T.lo[3 * (size_t)ref[i]], &T.hi[3 * (size_t)ref[i]]);
+            if (a > best_area) { best_area = a; best = i; }
+        }
+        if (best < 0) break;
+        const uint32_t c = ref[best];
+        inner[ninner++] = c;
+        ref[best] = T.left[c];
+        ref[m++] = T.right[c];
+    }
+    if (m < 3) return;
+    const int full = (1 << m) - 1;
+    float slo[1 << kTreeletLeaves][3], shi[1 << kTreeletLeaves][3], scost[1 << kTreeletLeaves];
+    int part[1 << kTreeletLeaves];
+    for (int s = 1; s <= full; s++) {
+        if ((s & (s - 1)) == 0) {
+            const int64_t x = T.index(ref[__builtin_ctz(s)]);
+            for (int k = 0; k < 3; k++) { slo[s][k] = T.lo[3 * x + k]; shi[s][k] = T.hi[3 * x + k]; }
+            scost[s] = T.cost[x];
+            continue;
+        }
+        const int top = 31 - __builtin_clz((unsigned)s), rest = s & ~(1 << top);
+        const int64_t x = T.index(ref[top]);
+        for (int k = 0; k < 3; k++) {
+            slo[s][k] = fminf(slo[rest][k], T.lo[3 * x + k]);
+            shi[s][k] = fmaxf(shi[rest][k], T.hi[3 * x + k]);
+        }
+        // best split: p holds the lowest leaf of s; the first strictly cheapest from p = s - 1 down
+        const int low = s & -s;
+        float best = INFINITY;
+        int bp = 0;
+        for (int q = (s - 1) & s; q > 0; q = (q - 1) & s) {
+            if (!(q & low)) continue;
+            const float c = scost[q] + scost[s ^ q];
+            if (c < best) { best = c; bp = q; }
+        }
+        scost[s] = kCostNode * box_area(slo[s], shi[s]) + best;
+        part[s] = bp;
+    }
+    if (!(scost[full] < T.cost[v])) return;
+    // rewire: subsets of two or more leaves become internal nodes, v first,
+    // then the expanded nodes in order, depth first, left part first
+    int next = 0;
+    struct Frame { int s; int64_t node; int stage; };
+    Frame st[kTreeletLeaves];
+    int sp = 0;
+    st[sp++] = {full, v, 0};
+    auto child_ref = [&](int sub) -> uint32_t {
+        if ((sub & (sub - 1)) == 0) return ref[__builtin_ctz(sub)];
+        return inner[next++];
+    };
+    // Pre-order assignment of node ids, then joins in post-order.
+    int64_t order_nodes[kTreeletLeaves];
+    int order_subsets[kTreeletLeaves];
+    int nn = 0;
+    while (sp > 0) {
+        Frame f = st[--sp];
+        const int pl = part[f.s], pr = f.s ^ pl;
+        const uint32_t rl = child_ref(pl), rr = child_ref(pr);
+        T.left[f.node] = rl;
+        T.right[f.node] = rr;
+        T.parent[T.index(rl)] = (uint32_t)f.node;
+        T.parent[T.index(rr)] = (uint32_t)f.node;
+        order_nodes[nn] = f.node;
+        order_subsets[nn++] = f.s;
+        if ((pr & (pr - 1)) != 0) st[sp++] = {pr, (int64_t)rr, 0};
+        if ((pl & (pl - 1)) != 0) st[sp++] = {pl, (int64_t)rl, 0};
+    }
+    (void)order_subsets;
+    for (int i = nn - 1; i >= 0; i--) T.join(order_nodes[i]);
+}
+
+void restructure(Bvh& b, int rounds) {
+    const int64_t n = b.n, ni = n - 1;
+    if (ni < 2) return;
+    std::vector<float> cost(2 * n - 1);
+    std::vector<uint32_t> nl(2 * n - 1);
+    TreeletState T{ni, b.left.data(), b.right.data(), b.parent.data(), b.lo.data(), b.hi.data(), cost.data(), nl.data()};
+    for (int64_t k = 0; k < n; k++) {
+        nl[ni + k] = 1;
+        cost[ni + k] = kCostTri * box_area(&b.lo[3 * (ni + k)], &b.hi[3 * (ni + k)]);
+    }
+    auto postorder = [&](auto&& visit) {
+        std::vector<std::pair<int64_t, int>> st = {{0, 0}};
+        while (!st.empty()) {
+            auto [v, state] = st.back();
+            st.pop_back();
+            if (state == 0) {
+                st.push_back({v, 1});
+                for (uint32_t c : {b.left[v], b.right[v]})
+                    if (!(c & kLeaf)) st.push_back({(int64_t)c, 0});
+            } else {
+                visit(v);
+            }
+        }
+    };
+    // each round recomputes a node from its (already optimised) children
+    // before its own treelet is considered
+    for (int r = 0; r < rounds; r++) {
+        const uint32_t gamma = (uint32_t)kTreeletLeaves << r;
+        postorder([&](int64_t v) {
+            T.join(v);
+            if (nl[v] >= gamma) treelet(T, v);
+        });
+    }
+    // relabel: leaf order left to right, Karras ids by the endpoint rule
+    std::vector<int64_t> first(2 * n - 1, 0), newid(ni, 0);
+    {
+        std::vector<int64_t> st = {0};
+        while (!st.empty()) {
+            const int64_t v = st.back();
+            st.pop_back();
+            const int64_t a = T.index(b.left[v]), c = T.index(b.right[v]);
+            first[a] = first[v];
+            first[c] = first[v] + nl[a];
+            if (!(b.left[v] & kLeaf)) { newid[a] = first[a] + nl[a] - 1; st.push_back(a); }
+            if (!(b.right[v] & kLeaf)) { newid[c] = first[c]; st.push_back(c); }
+        }
+    }
+    auto remap = [&](uint32_t ref) -> uint32_t {
+        return (ref & kLeaf) ? (kLeaf | (uint32_t)first[ni + (ref & ~kLeaf)]) : (uint32_t)newid[ref];
+    };
+    std::vector<uint32_t> L(ni), R(ni), P(2 * n - 1, 0xffffffffu), order(n), prim(n);
+    std::vector<float> lo(3 * (2 * n - 1)), hi(3 * (2 * n - 1)), tri(9 * n);
+    for (int64_t v = 0; v < ni; v++) {
+        const int64_t w = newid[v];
+        L[w] = remap(b.left[v]);
+        R[w] = remap(b.right[v]);
+        std::memcpy(&lo[3 * w], &b.lo[3 * v], 12);
+        std::memcpy(&hi[3 * w], &b.hi[3 * v], 12);
+    }
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t q = first[ni + k];
+        order[q] = b.order[k];
+        prim[q] = b.prim[k];
+        std::memcpy(&tri[9 * q], &b.tri[9 * k], 36);
+        std::memcpy(&lo[3 * (ni + q)], &b.lo[3 * (ni + k)], 12);
+        std::memcpy(&hi[3 * (ni + q)], &b.hi[3 * (ni + k)], 12);
+    }
+    for (int64_t w = 0; w < ni; w++)
+        for (uint32_t c : {L[w], R[w]}) P[(c & kLeaf) ? ni + (c & ~kLeaf) : c] = (uint32_t)w;
+    b.left.swap(L);
+    b.right.swap(R);
+    b.parent.swap(P);
+    b.lo.swap(lo);
+    b.hi.swap(hi);
+    b.order.swap(order);
+    b.prim.swap(prim);
+    b.tri.swap(tri);
+}
+
+void build_bvh(Bvh& b, int64_t n, const float* pos, int rounds = kTreeletRounds) {
     b.n = n;
     b.codes.resize(n);
     b.order.resize(n);
@@ -1400,6 +1605,7 @@ void build_bvh(Bvh& b, int64_t n, const float* pos) {
             done[v] = 1;
         }
     }
+    if (rounds > 0) restructure(b, rounds);
     b.qnode.assign((size_t)kQWords * (ni > 0 ? ni : 0), 0u);
     for (int64_t v = 0; v < ni; v++)
         quantize_node(ni, b.left.data(), b.right.data(), b.lo.data(), b.hi.data(), v, &b.qnode[kQWords * v]);
@@ -1923,10 +2129,10 @@ void orc_morton_codes(int64_t n, const float* positions, uint32_t* codes, float*
     morton(n, positions, codes, cbounds6);
 }
 
-int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t* order, uint32_t* left,
-                   uint32_t* right, uint32_t* parent, float* lo, float* hi, uint32_t* qnode) {
+int orc_lbvh_build(int64_t n, const float* positions, int32_t rounds, uint32_t* codes, uint32_t* order,
+                   uint32_t* left, uint32_t* right, uint32_t* parent, float* lo, float* hi, uint32_t* qnode) {
     Bvh b;
-    build_bvh(b, n, positions);
+    build_bvh(b, n, positions, rounds < 0 ? kTreeletRounds : rounds);
     if (n == 0) return 0;
     std::memcpy(codes, b.codes.data(), n * 4);
     std::memcpy(order, b.order.data(), n * 4);

@@ -118,13 +118,16 @@ float orc_lcg_next(uint32_t* state);
 
 /* ---- BVH ---- */
 void orc_morton_codes(int64_t n, const float* positions, uint32_t* codes, float* cbounds6);
-/* order: sorted triangle ids; left/right: child refs of internal nodes
+/* rounds: treelet restructuring rounds after the LBVH (0: the plain LBVH,
+   < 0: the build's default, as oxr_scene_build_bvh does);
+   codes: Morton codes in the radix sort's order;
+   order: triangle id of each leaf position; left/right: child refs of internal nodes
    (n-1 each; bit 31 set = leaf, low bits = sorted leaf position);
    lo/hi: 3 floats per internal node then per leaf (2n-1 boxes);
    qnode (optional): the traversal's compressed record of each internal node,
    8 words: origin.xyz, gamma | left leaf << 30 | right leaf << 31, three
    signed scale exponents, twelve 8-bit child bounds. */
-int orc_lbvh_build(int64_t n, const float* positions, uint32_t* codes, uint32_t* order,
+int orc_lbvh_build(int64_t n, const float* positions, int32_t rounds, uint32_t* codes, uint32_t* order,
                    uint32_t* left, uint32_t* right, uint32_t* parent, float* lo, float* hi, uint32_t* qnode);
 
 /* Ray queries against an LBVH built internally over scene positions.

@@ -52,7 +52,7 @@ def lib():
         L.orc_lcg_next.restype = C.c_float
         L.orc_lcg_next.argtypes = [C.POINTER(C.c_uint32)]
         L.orc_morton_codes.argtypes = [C.c_int64, _P, _P, _P]
-        L.orc_lbvh_build.argtypes = [C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+        L.orc_lbvh_build.argtypes = [C.c_int64, _P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P]
         L.orc_bvh_create.restype = _P
         L.orc_bvh_create.argtypes = [C.c_int64, _P]
         L.orc_bvh_destroy.argtypes = [_P]
@@ -117,7 +117,9 @@ def morton(pos):
     return codes, cb
 
 
-def lbvh(pos):
+def lbvh(pos, rounds=-1):
+    """The oracle's hierarchy over the positions: rounds = 0 the plain LBVH,
+    < 0 the build default (the LBVH restructured by treelets, relabelled)."""
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
     n = pos.size // 9
     codes = np.zeros(n, np.uint32)
@@ -128,7 +130,7 @@ def lbvh(pos):
     lo = np.zeros((max(2 * n - 1, 1), 3), np.float32)
     hi = np.zeros((max(2 * n - 1, 1), 3), np.float32)
     qnode = np.zeros((max(n - 1, 1), 8), np.uint32)
-    lib().orc_lbvh_build(n, p(pos), p(codes), p(order), p(left), p(right), p(parent), p(lo), p(hi), p(qnode))
+    lib().orc_lbvh_build(n, p(pos), rounds, p(codes), p(order), p(left), p(right), p(parent), p(lo), p(hi), p(qnode))
     ni = max(n - 1, 0)
     return dict(codes=codes, order=order, left=left[:ni], right=right[:ni], parent=parent[: 2 * n - 1],
                 lo=lo[: 2 * n - 1], hi=hi[: 2 * n - 1], qnode=qnode[:ni])

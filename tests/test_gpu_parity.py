@@ -50,15 +50,18 @@ def bvh_cases():
     }
 
 
+@pytest.mark.parametrize("rounds", [0, -1])
 @pytest.mark.parametrize("name", list(bvh_cases().keys()))
-def test_lbvh_bit_exact(ctx, name):
+def test_lbvh_bit_exact(ctx, name, rounds):
+    # rounds 0: the plain LBVH; -1: the default build (treelet restructuring
+    # and relabelling on top of it)
     pos = bvh_cases()[name]
-    print(f"[lbvh] case={name} n={pos.shape[0]}", flush=True)
+    print(f"[lbvh] case={name} n={pos.shape[0]} rounds={rounds}", flush=True)
     sid = ctx.upload(soup_scene(pos))
     try:
-        ctx.build_bvh(sid)
+        ctx.build_bvh(sid, rounds)
         codes, order, left, right, lo, hi = ctx.download_bvh(sid, pos.shape[0])
-        ref = oracle_lib.lbvh(pos)
+        ref = oracle_lib.lbvh(pos, rounds)
         assert np.array_equal(codes, ref["codes"])
         assert np.array_equal(order, ref["order"])
         assert np.array_equal(left, ref["left"])

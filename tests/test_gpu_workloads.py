@@ -38,17 +38,17 @@ def heightfield():
     return scenes.heightfield_medium()
 
 
-def _lbvh_equal(ctx, sc, label):
+def _lbvh_equal(ctx, sc, label, rounds=-1):
     n = sc.n_tris
-    print(f"[lbvh] {label} n={n}", flush=True)
+    print(f"[lbvh] {label} n={n} rounds={rounds}", flush=True)
     sid = ctx.upload(sc)
     try:
-        ctx.build_bvh(sid)
+        ctx.build_bvh(sid, rounds)
         codes, order, left, right, lo, hi = ctx.download_bvh(sid, n)
         qnode = ctx.download_nodes(sid, n)
     finally:
         ctx.release(sid)
-    ref = oracle_lib.lbvh(sc.positions)
+    ref = oracle_lib.lbvh(sc.positions, rounds)
     assert np.array_equal(codes, ref["codes"])
     assert np.array_equal(order, ref["order"])
     assert np.array_equal(left, ref["left"])
@@ -60,8 +60,9 @@ def _lbvh_equal(ctx, sc, label):
     print(f"[lbvh] {label}: bit-exact; {shared:.1%} of triangles share a Morton code", flush=True)
 
 
-def test_lbvh_bit_exact_sphere_field(ctx, sphere_field):
-    _lbvh_equal(ctx, sphere_field, "sphere field 983,040 tris")
+@pytest.mark.parametrize("rounds", [0, -1])
+def test_lbvh_bit_exact_sphere_field(ctx, sphere_field, rounds):
+    _lbvh_equal(ctx, sphere_field, "sphere field 983,040 tris", rounds)
 
 
 def test_lbvh_bit_exact_heightfield(ctx, heightfield):

@@ -94,7 +94,7 @@ def karras_to_ranges(left, right, n):
 def test_lbvh_equals_binary_radix_tree(n, seed, dup):
     pos = np.repeat(soup(n // dup, seed), dup, axis=0) if dup > 1 else soup(n, seed)
     n = pos.shape[0]
-    r = oracle_lib.lbvh(pos)
+    r = oracle_lib.lbvh(pos, rounds=0)
     codes, order = r["codes"], r["order"]
     raw = morton_numpy(pos)
     # stable sort by code, ties by triangle index
@@ -180,6 +180,86 @@ def test_compressed_traversal_finds_the_exact_boxes_hits():
     ref = oracle_lib.brute_closest(pos, rays)
     assert np.array_equal(hits[:, :2].view(np.uint32), ref[:, :2].view(np.uint32))
     assert np.all(counts[:, 0] % 2 == 0)                       # two child boxes per record fetched
+
+
+def sah(r, n):
+    """SAH cost (1.2 per internal node box area, 1 per leaf box area) of an
+    oracle hierarchy, over the root's area, in float64."""
+    lo, hi = r["lo"].astype(np.float64), r["hi"].astype(np.float64)
+    d = hi - lo
+    area = 2 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+    return (1.2 * area[: n - 1].sum() + area[n - 1:].sum()) / area[0]
+
+
+@pytest.mark.parametrize("n,seed", [(3, 31), (50, 32), (2000, 33)])
+def test_restructured_hierarchy_is_a_karras_tree(n, seed):
+    # The default build: the LBVH restructured by treelets, then relabelled
+    # as Karras's layout of its own leaf order.  Every internal node covers
+    # consecutive leaf positions, its id is an endpoint of that range by the
+    # Karras rule, its children are gamma and gamma + 1, the boxes are the
+    # unions of the children's, the leaves are a permutation of the
+    # triangles -- and the SAH cost is lower than the plain LBVH's.
+    pos = soup(n, seed)
+    r, r0 = oracle_lib.lbvh(pos), oracle_lib.lbvh(pos, rounds=0)
+    assert np.array_equal(np.sort(r["order"]), np.arange(n))
+    tri = pos[r["order"]]
+    assert np.array_equal(r["lo"][n - 1:], tri.min(1)) and np.array_equal(r["hi"][n - 1:], tri.max(1))
+    ranges = karras_to_ranges(r["left"], r["right"], n)
+    assert ranges[0] == (0, n - 1)
+    for i in range(n - 1):
+        a, b = ranges[i]
+        assert i in (a, b) or i == 0
+        L, R = int(r["left"][i]), int(r["right"][i])
+        g = L & ~LEAF
+        assert (R & ~LEAF) == g + 1 and a <= g < b
+        for c, want in ((L, (a, g)), (R, (g + 1, b))):
+            if c & LEAF:
+                assert want[0] == want[1] == (c & ~LEAF)
+            else:
+                assert ranges[c] == want
+                j = c
+                kids = [((n - 1 + (x & ~LEAF)) if x & LEAF else x) for x in (int(r["left"][j]), int(r["right"][j]))]
+                assert np.array_equal(r["lo"][j], np.minimum(r["lo"][kids[0]], r["lo"][kids[1]]))
+    if n >= 50:
+        assert sah(r, n) < sah(r0, n)
+
+
+def all_tree_costs(boxes):
+    """The least SAH cost (1.2 x area per internal node, leaf areas) over
+    every binary tree of the given leaf boxes, by enumeration of splits."""
+    import functools
+
+    m = len(boxes)
+
+    @functools.lru_cache(None)
+    def best(mask):
+        idx = [i for i in range(m) if mask >> i & 1]
+        lo = np.min([boxes[i][0] for i in idx], 0)
+        hi = np.max([boxes[i][1] for i in idx], 0)
+        d = hi - lo
+        a = 2 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+        if len(idx) == 1:
+            return a
+        low = mask & -mask
+        out = min(best(q) + best(mask ^ q) for q in range(1, mask) if q & mask == q and q & low)
+        return 1.2 * a + out
+
+    return best((1 << m) - 1)
+
+
+@pytest.mark.parametrize("seed", [41, 42, 43, 44])
+def test_treelet_finds_the_cheapest_tree_of_seven_leaves(seed):
+    # with 7 triangles the root's treelet is the whole hierarchy, so the
+    # restructured SAH cost is the least over all 10,395 binary trees
+    pos = soup(7, seed, spread=4.0, size=1.5)
+    r = oracle_lib.lbvh(pos)
+    tri = pos.astype(np.float64)
+    boxes = [(t.min(0), t.max(0)) for t in tri]
+    d = r["hi"][0].astype(np.float64) - r["lo"][0].astype(np.float64)
+    root_area = 2 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+    want = all_tree_costs(boxes) / root_area
+    assert abs(sah(r, 7) - want) <= 1e-5 * want
+    assert sah(oracle_lib.lbvh(pos, rounds=0), 7) >= want * (1 - 1e-5)
 
 
 def rays_for(pos, n, seed):
