@@ -8,9 +8,10 @@
 //      LDS histograms, one scan, stable scatter ranked with wave64 ballots.
 //   4. Karras 2012 hierarchy, one thread per internal node; equal codes are
 //      split on the sorted position (augmented key).
-//   5. bottom-up AABB refit with per-node arrival counters; each finished box
-//      is also written into its parent's 64-byte child-pair record used by
-//      traversal.
+//   5. bottom-up passes, one kernel per tree depth (depths by pointer
+//      jumping): the refit, or rounds of treelet restructuring, then the
+//      relabelling into Karras's layout of the new leaf order.
+//   6. the traversal's 32-byte compressed node records (k_quantize).
 #include <algorithm>
 #include "common.h"
 
@@ -144,6 +145,54 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, int64_t m)
     }
 }
 
+// A longer exclusive scan in three steps: tile sums, a scan of those (one
+// block), then each tile scanned in LDS from its offset.  A tile is 2048
+// values, eight consecutive per thread.
+constexpr int kScanTile = 2048;
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ data, int64_t m, uint32_t* tsum) {
+    __shared__ uint32_t red[256 / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanTile / 256; j++) {
+        const int64_t i = base + j * 256 + threadIdx.x;
+        if (i < m) s += data[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) tsum[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(uint32_t* data, int64_t m, const uint32_t* __restrict__ toff) {
+    __shared__ uint32_t wsum[256 / 64];
+    constexpr int kPer = kScanTile / 256;
+    const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kPer;
+    uint32_t v[kPer], t = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        v[j] = b + j < m ? data[b + j] : 0u;
+        t += v[j];
+    }
+    // inclusive scan of the thread totals over the wave, then over the waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = t;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)inc, o);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t run = toff[blockIdx.x] + inc - t;
+    for (int k = 0; k < wave; k++) run += wsum[k];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        if (b + j < m) data[b + j] = run;
+        run += v[j];
+    }
+}
+
 __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin, int64_t n, int shift,
                                                                const uint32_t* __restrict__ offs, int nblocks,
@@ -232,60 +281,24 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     parent[(R & kLeafBit) ? ni + (R & ~kLeafBit) : R] = (uint32_t)i;
 }
 
-// One thread per leaf: leaf box, sorted triangle record, then climb while
-// this thread is the second to reach a node.
-__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ pos, int64_t n,
-                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ left,
-                                                const uint32_t* __restrict__ right, const uint32_t* __restrict__ parent,
-                                                uint32_t* flags, float* lo, float* hi, float4* tris, uint32_t* error) {
-    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// One thread per leaf: its box and its triangle record in sorted order.
+// The internal boxes follow level by level (k_tree_level).
+__global__ __launch_bounds__(256) void k_leaf_boxes(const float* __restrict__ pos, int64_t n,
+                                                     const uint32_t* __restrict__ order, float* lo, float* hi,
+                                                     float4* tris) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const int64_t ni = n - 1;
-    uint32_t tid = order[k];
+    const uint32_t tid = order[k];
     const float* p = pos + 9 * (size_t)tid;
-    float bl[3], bh[3];
+    const int64_t node = (n - 1) + k;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        bl[a] = fminf(fminf(p[a], p[3 + a]), p[6 + a]);
-        bh[a] = fmaxf(fmaxf(p[a], p[3 + a]), p[6 + a]);
+        lo[3 * node + a] = fminf(fminf(p[a], p[3 + a]), p[6 + a]);
+        hi[3 * node + a] = fmaxf(fmaxf(p[a], p[3 + a]), p[6 + a]);
     }
     tris[3 * k + 0] = make_float4(p[0], p[1], p[2], __uint_as_float(tid));
     tris[3 * k + 1] = make_float4(p[3], p[4], p[5], 0.0f);
     tris[3 * k + 2] = make_float4(p[6], p[7], p[8], 0.0f);
-    int64_t node = ni + k;
-#pragma unroll
-    for (int a = 0; a < 3; a++) { lo[3 * node + a] = bl[a]; hi[3 * node + a] = bh[a]; }
-    if (ni == 0) return;
-    uint32_t par = parent[node];
-    for (int depth = 0;; depth++) {
-        if (depth > 256 || par >= (uint32_t)ni) {  // LBVH depth <= 30 + log2(n): a malformed tree
-            atomicOr(error, 1u);
-            return;
-        }
-        __threadfence();
-        if (atomicAdd(&flags[par], 1u) == 0u) return;  // sibling not finished yet
-        __threadfence();
-        uint32_t cl = left[par], cr = right[par];
-        int64_t il = (cl & kLeafBit) ? ni + (cl & ~kLeafBit) : cl;
-        int64_t ir = (cr & kLeafBit) ? ni + (cr & ~kLeafBit) : cr;
-        float nl[3], nh[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            float l0 = __hip_atomic_load(&lo[3 * il + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float l1 = __hip_atomic_load(&lo[3 * ir + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float h0 = __hip_atomic_load(&hi[3 * il + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float h1 = __hip_atomic_load(&hi[3 * ir + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            nl[a] = fminf(l0, l1);
-            nh[a] = fmaxf(h0, h1);
-        }
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            __hip_atomic_store(&lo[3 * (size_t)par + a], nl[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&hi[3 * (size_t)par + a], nh[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (par == 0) return;
-        par = parent[par];
-    }
 }
 
 // exact 2^e (|e| < 1023)
@@ -340,135 +353,292 @@ __global__ __launch_bounds__(256) void k_quantize(int64_t ni, const uint32_t* __
     qnodes[2 * v + 1] = make_uint4(w4, q[0], q[1], q[2]);
 }
 
-// ---- treelet restructuring (Karras & Aila 2013), then Karras relabelling ----
-// Each round climbs the tree bottom up (the second thread to reach a node
-// goes on, as in the refit); at a node it recomputes box, leaf count and SAH
-// cost from the children and, if at least kTreeletLeaves << round leaves lie
-// below it, optimises its treelet: the treelet leaf (internal) of largest
-// area is replaced by its two children until there are seven, a dynamic
-// program over the subsets of those finds the binary tree of least SAH cost,
-// and if it is cheaper the treelet's internal nodes are rewired to it (the
-// root keeps its id; the others are reused in the order they were expanded,
-// depth first, left part first).  The tree is then relabelled as Karras's
-// layout of its own leaf order: every node's leaves are consecutive, internal
-// ids follow the endpoint rule (a left child takes the last position of its
+// ---- bottom-up passes: refit and treelet restructuring (Karras & Aila
+// 2013), then Karras relabelling ----
+// A bottom-up pass runs one kernel per tree depth, deepest first: a node's
+// children are finished by the kernels before its own, and the nodes of one
+// depth have disjoint subtrees.  (A climb with per-node arrival counters
+// would need a device-scope fence per node, and on this chip a device-scope
+// fence writes back and invalidates the XCD's L2.)  Depths come from
+// pointer jumping over the parent links.  At each node the pass recomputes
+// box, leaf count and SAH cost from the children and, in a restructuring
+// round, if at least kTreeletLeaves << round leaves lie below it, one wave
+// optimises its treelet: the treelet leaf (internal) of largest area is
+// replaced by its two children until there are seven, a dynamic program over
+// the subsets of those (in LDS, a lane per subset, subsets by size) finds the
+// binary tree of least SAH cost, and if it is cheaper the treelet's internal
+// nodes are rewired to it (the root keeps its id; the others are reused in
+// the order they were expanded, depth first, left part first).  Ties go to
+// the first candidate in a fixed order, so the tree is the one a serial
+// post-order pass builds.  The tree is then relabelled as Karras's layout of
+// its own leaf order: every node's leaves are consecutive, internal ids
+// follow the endpoint rule (a left child takes the last position of its
 // range, a right child the first), so a node's children are again gamma and
-// gamma + 1.  Reads and writes of the tree during the climb are relaxed
-// device-scope atomics, so a thread sees what the threads below it wrote.
+// gamma + 1.
 constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletSubsets = 1 << kTreeletLeaves;
+constexpr int kDepthJumps = 13;   // pointer-jumping steps: depths below 2^13
 constexpr float kCostNode = 1.2f, kCostTri = 1.0f;
-
-template <typename T>
-__device__ __forceinline__ T ald(const T* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-template <typename T>
-__device__ __forceinline__ void ast(T* p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 __device__ __forceinline__ float box_area(const float lo[3], const float hi[3]) {
     const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
     return 2.0f * ((dx * dy + dy * dz) + dz * dx);
 }
 
+// The tree as the passes see it.  Every node a kernel reads was written by
+// an earlier kernel of the pass (its children lie deeper), so plain loads
+// and stores suffice.
 struct TreeDev {
     int64_t ni;
     uint32_t *left, *right, *parent, *nl;
     float *lo, *hi, *cost;
     __device__ int64_t index(uint32_t ref) const { return (ref & kLeafBit) ? ni + (ref & ~kLeafBit) : ref; }
-    __device__ void box(int64_t x, float l[3], float h[3]) const {
-#pragma unroll
-        for (int k = 0; k < 3; k++) { l[k] = ald(&lo[3 * x + k]); h[k] = ald(&hi[3 * x + k]); }
-    }
     // internal node v from its two children: box, leaf count and SAH cost
     __device__ void join(int64_t v) const {
-        const int64_t a = index(ald(&left[v])), c = index(ald(&right[v]));
-        float la[3], ha[3], lc[3], hc[3], l[3], h[3];
-        box(a, la, ha);
-        box(c, lc, hc);
+        const int64_t a = index(left[v]), c = index(right[v]);
+        float l[3], h[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            l[k] = fminf(la[k], lc[k]);
-            h[k] = fmaxf(ha[k], hc[k]);
-            ast(&lo[3 * v + k], l[k]);
-            ast(&hi[3 * v + k], h[k]);
+            l[k] = fminf(lo[3 * a + k], lo[3 * c + k]);
+            h[k] = fmaxf(hi[3 * a + k], hi[3 * c + k]);
+            lo[3 * v + k] = l[k];
+            hi[3 * v + k] = h[k];
         }
-        ast(&nl[v], ald(&nl[a]) + ald(&nl[c]));
-        ast(&cost[v], kCostNode * box_area(l, h) + ald(&cost[a]) + ald(&cost[c]));
+        nl[v] = nl[a] + nl[c];
+        cost[v] = kCostNode * box_area(l, h) + cost[a] + cost[c];
     }
 };
 
-__device__ void treelet(const TreeDev& T, int64_t v) {
-    uint32_t ref[kTreeletLeaves], inner[kTreeletLeaves];
-    int m = 2, ninner = 0;
-    ref[0] = ald(&T.left[v]);
-    ref[1] = ald(&T.right[v]);
-    while (m < kTreeletLeaves) {
-        int best = -1;
-        float best_area = -1.0f;
-        for (int i = 0; i < m; i++) {
-            if (ref[i] & kLeafBit) continue;
-            float l[3], h[3];
-            T.box(ref[i], l, h);
-            const float a = box_area(l, h);
-            if (a > best_area) { best_area = a; best = i; }
-        }
-        if (best < 0) break;
-        const uint32_t c = ref[best];
-        inner[ninner++] = c;
-        ref[best] = ald(&T.left[c]);
-        ref[m++] = ald(&T.right[c]);
+// One wave's treelet (LDS).  Slots hold the treelet leaves; subsets of them
+// are bit masks over the slots.
+struct TreeletLds {
+    float cost[kTreeletSubsets];         // the dynamic program's best SAH cost per subset
+    float sarea[kTreeletSubsets];        // each subset's box area (slot boxes folded in ascending order)
+    float jbox[kTreeletSubsets][6];      // the rewired nodes' boxes, by their leaf subset
+    float jcost[kTreeletSubsets];        // ... their SAH costs
+    uint32_t jnl[kTreeletSubsets];       // ... their leaf counts
+    float lb[kTreeletLeaves][6];         // slot boxes (lo.xyz, hi.xyz)
+    float area[kTreeletLeaves];          // slot box areas
+    float scost[kTreeletLeaves];         // slot SAH costs
+    uint32_t snl[kTreeletLeaves];        // slot leaf counts
+    uint32_t ref[kTreeletLeaves];        // slot node refs
+    uint32_t cl[kTreeletLeaves], cr[kTreeletLeaves];  // an internal slot's children
+    uint32_t inner[kTreeletLeaves];      // the treelet's internal nodes below its root, in expansion order
+    int64_t st_node[kTreeletLeaves], order_node[kTreeletLeaves];
+    int st_set[kTreeletLeaves], order_set[kTreeletLeaves];
+    uint8_t part[kTreeletSubsets];       // the subset's best left part
+    int m;
+    float root_cost;
+};
+
+// The wave's LDS writes before this point are seen by all its lanes after it.
+__device__ __forceinline__ void wave_lds_sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The subsets of the seven slots ordered by size, then value: those of
+// size k are mask[start[k]] .. mask[start[k + 1] - 1].
+struct SubsetTable {
+    uint8_t mask[kTreeletSubsets];
+    uint8_t start[kTreeletLeaves + 2];
+};
+constexpr SubsetTable make_subset_table() {
+    SubsetTable t{};
+    int o = 0;
+    for (int k = 0; k <= kTreeletLeaves; k++) {
+        t.start[k] = (uint8_t)o;
+        for (int s = 0; s < kTreeletSubsets; s++)
+            if (__builtin_popcount(s) == k) t.mask[o++] = (uint8_t)s;
     }
+    t.start[kTreeletLeaves + 1] = (uint8_t)o;
+    return t;
+}
+__constant__ SubsetTable kSubsets = make_subset_table();
+struct SubsetStart {
+    int v[kTreeletLeaves + 2];
+};
+constexpr SubsetStart make_subset_start() {
+    SubsetStart r{};
+    const SubsetTable t = make_subset_table();
+    for (int k = 0; k < kTreeletLeaves + 2; k++) r.v[k] = t.start[k];
+    return r;
+}
+constexpr SubsetStart kSubsetStart = make_subset_start();
+
+// the bits of u placed, in order, on the set bits of r
+__device__ __forceinline__ int deposit_bits(int u, int r) {
+    int d = 0;
+    for (int b = 0; r; b++) {
+        const int low = r & -r;
+        if ((u >> b) & 1) d |= low;
+        r ^= low;
+    }
+    return d;
+}
+
+// slot i of the treelet := node ref r (its box, area, leaf count, cost, and
+// for an internal node its children, read with the rest so that expanding
+// the slot needs no further dependent load)
+__device__ __forceinline__ void treelet_slot(const TreeDev& T, TreeletLds& L, int i, uint32_t r) {
+    const int64_t x = T.index(r);
+    if (!(r & kLeafBit)) {
+        L.cl[i] = T.left[x];
+        L.cr[i] = T.right[x];
+    }
+    float l[3], h[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        l[k] = T.lo[3 * x + k];
+        h[k] = T.hi[3 * x + k];
+        L.lb[i][k] = l[k];
+        L.lb[i][3 + k] = h[k];
+    }
+    L.ref[i] = r;
+    L.area[i] = box_area(l, h);
+    L.snl[i] = T.nl[x];
+    L.scost[i] = T.cost[x];
+}
+
+// Treelet of internal node v, by the whole wave (every lane calls it).
+__device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const uint8_t* smask, int lane) {
+    if (lane == 0) {
+        // formation: expand the internal slot of largest area (the first of
+        // equal ones) until there are seven slots
+        L.root_cost = T.cost[v];
+        treelet_slot(T, L, 0, T.left[v]);
+        treelet_slot(T, L, 1, T.right[v]);
+        int m = 2, ninner = 0;
+        while (m < kTreeletLeaves) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int i = 0; i < m; i++) {
+                if (L.ref[i] & kLeafBit) continue;
+                if (L.area[i] > best_area) { best_area = L.area[i]; best = i; }
+            }
+            if (best < 0) break;
+            const uint32_t c = L.ref[best];
+            L.inner[ninner++] = c;
+            const uint32_t cl = L.cl[best], cr = L.cr[best];
+            treelet_slot(T, L, best, cl);
+            treelet_slot(T, L, m++, cr);
+        }
+        L.m = m;
+    }
+    wave_lds_sync();
+    const int m = L.m;
     if (m < 3) return;
     const int full = (1 << m) - 1;
-    float slo[1 << kTreeletLeaves][3], shi[1 << kTreeletLeaves][3], scost[1 << kTreeletLeaves];
-    uint8_t part[1 << kTreeletLeaves];
-    for (int s = 1; s <= full; s++) {
-        if ((s & (s - 1)) == 0) {
-            const int64_t x = T.index(ref[__builtin_ctz(s)]);
-            T.box(x, slo[s], shi[s]);
-            scost[s] = ald(&T.cost[x]);
-            continue;
-        }
-        const int top = 31 - __builtin_clz((unsigned)s), rest = s & ~(1 << top);
-        float l[3], h[3];
-        T.box(T.index(ref[top]), l, h);
+    if (lane < m) L.cost[1 << lane] = L.scost[lane];
+    // this lane's subsets s = lane + 1 and lane + 65: box (the slot boxes
+    // folded in ascending order) and area
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            slo[s][k] = fminf(slo[rest][k], l[k]);
-            shi[s][k] = fmaxf(shi[rest][k], h[k]);
+    for (int j = 0; j < 2; j++) {
+        const int s = lane + 1 + 64 * j;
+        if (s > full || (s & (s - 1)) == 0) continue;
+        int b = __builtin_ctz(s);
+        float l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) { l[k] = L.lb[b][k]; h[k] = L.lb[b][3 + k]; }
+        for (int t = s & (s - 1); t; t &= t - 1) {
+            b = __builtin_ctz(t);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                l[k] = fminf(l[k], L.lb[b][k]);
+                h[k] = fmaxf(h[k], L.lb[b][3 + k]);
+            }
         }
-        const int low = s & -s;
-        float best = __builtin_inff();
-        int bp = 0;
-        for (int q = (s - 1) & s; q > 0; q = (q - 1) & s) {
-            if (!(q & low)) continue;
-            const float c = scost[q] + scost[s ^ q];
-            if (c < best) { best = c; bp = q; }
-        }
-        scost[s] = kCostNode * box_area(slo[s], shi[s]) + best;
-        part[s] = (uint8_t)bp;
+        L.sarea[s] = box_area(l, h);
     }
-    if (!(scost[full] < ald(&T.cost[v]))) return;
-    int next = 0;
-    int64_t st_node[kTreeletLeaves], order_nodes[kTreeletLeaves];
-    int st_set[kTreeletLeaves];
-    int sp = 0, nn = 0;
-    st_node[sp] = v;
-    st_set[sp++] = full;
+    wave_lds_sync();
+    // subsets by size: each one's best split into a left part holding its
+    // lowest slot and a right part.  A subset of size k has 2^(k-1) - 1
+    // splits; candidate t is the t-th of them in descending order of the
+    // left part (the left part is the lowest slot plus the subset of the
+    // other slots numbered 2^(k-1) - 2 - t).  A segment of 2^(k-1) lanes takes
+    // one subset, a lane a candidate, and the best is the least finite cost,
+    // the lowest t among equal ones: the first found by a scan in that order
+    // keeping a candidate only when strictly cheaper than the best so far.
+#pragma unroll
+    for (int size = 2; size <= kTreeletLeaves; size++) {
+        if (size > m) break;
+        const int W = 1 << (size - 1), per = 64 / W;
+        const int first = kSubsetStart.v[size], cnt = kSubsetStart.v[size + 1] - first;
+        const int g = lane / W, t = lane & (W - 1);
+        for (int b0 = 0; b0 < cnt; b0 += per) {
+            const int idx = b0 + g;
+            const int s = idx < cnt ? smask[first + idx] : 0;
+            const bool live = idx < cnt && s <= full;
+            const int low = s & -s, r = s ^ low;
+            float bc = __builtin_inff();
+            int bt = W;  // none
+            if (live && t < W - 1) {
+                const int q = deposit_bits(W - 2 - t, r) | low;
+                const float c = L.cost[q] + L.cost[s ^ q];
+                if (c < __builtin_inff()) { bc = c; bt = t; }
+            }
+            for (int o = 1; o < W; o <<= 1) {
+                const float oc = __shfl_xor(bc, o);
+                const int ot = __shfl_xor(bt, o);
+                if (ot < W && (bt == W || oc < bc || (oc == bc && ot < bt))) { bc = oc; bt = ot; }
+            }
+            if (live && t == 0) {
+                L.cost[s] = kCostNode * L.sarea[s] + bc;
+                L.part[s] = (uint8_t)(bt < W ? (deposit_bits(W - 2 - bt, r) | low) : 0);
+            }
+        }
+        wave_lds_sync();
+    }
+    if (lane != 0 || !(L.cost[full] < L.root_cost)) return;
+    // rewire: pre-order over the chosen splits, left part first
+    int next = 0, sp = 0, nn = 0;
+    L.st_node[sp] = v;
+    L.st_set[sp++] = full;
     while (sp > 0) {
         --sp;
-        const int64_t node = st_node[sp];
-        const int set = st_set[sp];
-        const int pl = part[set], pr = set ^ pl;
-        const uint32_t rl = (pl & (pl - 1)) == 0 ? ref[__builtin_ctz(pl)] : inner[next++];
-        const uint32_t rr = (pr & (pr - 1)) == 0 ? ref[__builtin_ctz(pr)] : inner[next++];
-        ast(&T.left[node], rl);
-        ast(&T.right[node], rr);
-        ast(&T.parent[T.index(rl)], (uint32_t)node);
-        ast(&T.parent[T.index(rr)], (uint32_t)node);
-        order_nodes[nn++] = node;
-        if ((pr & (pr - 1)) != 0) { st_node[sp] = rr; st_set[sp++] = pr; }
-        if ((pl & (pl - 1)) != 0) { st_node[sp] = rl; st_set[sp++] = pl; }
+        const int64_t node = L.st_node[sp];
+        const int set = L.st_set[sp];
+        const int pl = L.part[set], pr = set ^ pl;
+        const uint32_t rl = (pl & (pl - 1)) == 0 ? L.ref[__builtin_ctz(pl)] : L.inner[next++];
+        const uint32_t rr = (pr & (pr - 1)) == 0 ? L.ref[__builtin_ctz(pr)] : L.inner[next++];
+        T.left[node] = rl;
+        T.right[node] = rr;
+        T.parent[T.index(rl)] = (uint32_t)node;
+        T.parent[T.index(rr)] = (uint32_t)node;
+        L.order_node[nn] = node;
+        L.order_set[nn++] = set;
+        if ((pr & (pr - 1)) != 0) { L.st_node[sp] = rr; L.st_set[sp++] = pr; }
+        if ((pl & (pl - 1)) != 0) { L.st_node[sp] = rl; L.st_set[sp++] = pl; }
     }
-    for (int i = nn - 1; i >= 0; i--) T.join(order_nodes[i]);
+    // the rewired nodes, children first, joined from the slots' and each
+    // other's values as the join of each node would compute them
+    for (int i = nn - 1; i >= 0; i--) {
+        const int set = L.order_set[i];
+        const int pl = L.part[set], pr = set ^ pl;
+        const bool sa = (pl & (pl - 1)) == 0, sc = (pr & (pr - 1)) == 0;
+        const float* ba = sa ? L.lb[__builtin_ctz(pl)] : L.jbox[pl];
+        const float* bc = sc ? L.lb[__builtin_ctz(pr)] : L.jbox[pr];
+        const float ca = sa ? L.scost[__builtin_ctz(pl)] : L.jcost[pl];
+        const float cc = sc ? L.scost[__builtin_ctz(pr)] : L.jcost[pr];
+        const uint32_t na = sa ? L.snl[__builtin_ctz(pl)] : L.jnl[pl];
+        const uint32_t nc = sc ? L.snl[__builtin_ctz(pr)] : L.jnl[pr];
+        const int64_t node = L.order_node[i];
+        float l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l[k] = fminf(ba[k], bc[k]);
+            h[k] = fmaxf(ba[3 + k], bc[3 + k]);
+            L.jbox[set][k] = l[k];
+            L.jbox[set][3 + k] = h[k];
+            T.lo[3 * node + k] = l[k];
+            T.hi[3 * node + k] = h[k];
+        }
+        const float cst = kCostNode * box_area(l, h) + ca + cc;
+        L.jcost[set] = cst;
+        L.jnl[set] = na + nc;
+        T.nl[node] = na + nc;
+        T.cost[node] = cst;
+    }
 }
 
 // leaves: leaf count 1 and SAH cost kCostTri * area
@@ -483,26 +653,102 @@ __global__ void k_tree_leaves(int64_t n, TreeDev T) {
     T.cost[x] = kCostTri * box_area(l, h);
 }
 
-// one round: thread per leaf climbs; the second arrival joins the node and,
-// with gamma leaves or more below it (gamma 0: join only), optimises its treelet
-__global__ __launch_bounds__(256) void k_treelet_round(int64_t n, TreeDev T, uint32_t gamma, uint32_t* flags,
-                                                       uint32_t* error) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    uint32_t node = ald(&T.parent[T.ni + k]);
-    for (int depth = 0;; depth++) {
-        if (depth > 4096 || node >= (uint32_t)T.ni) {
+// Depths of the internal nodes by pointer jumping: anc = an ancestor (the
+// root 0 its own), dep = the number of links to it.  A parent link out of
+// range, or a node whose ancestor is not the root after kDepthJumps steps
+// (a cycle or a depth of 2^13 or more), marks the hierarchy malformed.
+__global__ void k_depth_init(int64_t ni, const uint32_t* __restrict__ parent, uint32_t* anc, uint32_t* dep,
+                             uint32_t* error) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    uint32_t p = 0;
+    if (i > 0) {
+        p = parent[i];
+        if (p >= (uint32_t)ni) {
             atomicOr(error, 1u);
-            return;
+            p = 0;
         }
-        __threadfence();
-        if (atomicAdd(&flags[node], 1u) == 0u) return;  // the other child's thread is not done yet
-        __threadfence();
-        T.join(node);
-        if (gamma > 0 && ald(&T.nl[node]) >= gamma) treelet(T, node);
-        __threadfence();
-        if (node == 0) return;
-        node = ald(&T.parent[node]);
+    }
+    anc[i] = p;
+    dep[i] = i > 0 ? 1u : 0u;
+}
+__global__ void k_depth_jump(int64_t ni, const uint32_t* __restrict__ anc, const uint32_t* __restrict__ dep,
+                             uint32_t* anc2, uint32_t* dep2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const uint32_t a = anc[i];
+    anc2[i] = anc[a];
+    dep2[i] = dep[i] + dep[a];
+}
+__global__ __launch_bounds__(256) void k_depth_max(int64_t ni, const uint32_t* __restrict__ anc,
+                                                   const uint32_t* __restrict__ dep, uint32_t* maxd, uint32_t* error) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint32_t d = 0;
+    if (i < ni) {
+        if (anc[i] != 0) atomicOr(error, 1u);
+        d = dep[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&maxd[blockIdx.x % 64], d);  // 64 partial maxima
+}
+
+// One depth of a bottom-up pass: the nodes at depth d join their children;
+// with gamma > 0 those with gamma leaves or more below them are listed for
+// treelet optimisation.  The list is kSubLists sub-lists of cap entries, one
+// per block residue (block b appends to sub-list b % kSubLists with one
+// atomic per block), so that no counter takes every block's atomic.
+constexpr int kSubLists = 64;
+
+__global__ __launch_bounds__(256) void k_tree_level(TreeDev T, const uint32_t* __restrict__ dep, uint32_t d,
+                                                    uint32_t gamma, uint32_t* list, int64_t cap, uint32_t* count) {
+    __shared__ uint32_t wcount[256 / 64], wbase[256 / 64];
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool need = false;
+    if (i < T.ni && dep[i] == d) {
+        T.join(i);
+        need = gamma > 0 && T.nl[i] >= gamma;
+    }
+    if (gamma == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t b = __ballot(need);
+    if (lane == 0) wcount[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = (wcount[0] + wcount[1]) + (wcount[2] + wcount[3]);
+        uint32_t base = total ? atomicAdd(&count[blockIdx.x % kSubLists], total) : 0u;
+        for (int k = 0; k < 256 / 64; k++) {
+            wbase[k] = base;
+            base += wcount[k];
+        }
+    }
+    __syncthreads();
+    if (need)
+        list[(int64_t)(blockIdx.x % kSubLists) * cap + wbase[wave] + __popcll(b & ((1ull << lane) - 1))] =
+            (uint32_t)i;
+}
+
+// The listed treelets of one depth, a wave each.
+__global__ __launch_bounds__(256) void k_treelets(TreeDev T, const uint32_t* __restrict__ list, int64_t cap,
+                                                  const uint32_t* __restrict__ count) {
+    __shared__ TreeletLds lds[256 / 64];
+    __shared__ uint8_t smask[kTreeletSubsets];
+    if (threadIdx.x < kTreeletSubsets) smask[threadIdx.x] = kSubsets.mask[threadIdx.x];
+    __syncthreads();
+    TreeletLds& L = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    static_assert(kSubLists == 64, "a lane per sub-list");
+    const uint32_t c = count[lane];
+    uint32_t inc = c;  // inclusive scan of the sub-list lengths
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)inc, o);
+        if (lane >= o) inc += u;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)inc, 63);
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < total; j += waves) {
+        const int sub = __builtin_ctzll(__ballot(inc > j));         // the sub-list holding entry j
+        const uint32_t start = (uint32_t)__shfl((int)(inc - c), sub);
+        treelet_wave(T, list[(int64_t)sub * cap + (j - start)], L, smask, lane);
     }
 }
 
@@ -576,16 +822,26 @@ __global__ void k_scatter_leaves(int64_t n, const float* __restrict__ lo, const 
     for (int a = 0; a < 3; a++) { lo2[3 * (ni + q) + a] = lo[3 * (ni + k) + a]; hi2[3 * (ni + q) + a] = hi[3 * (ni + k) + a]; }
 }
 
+__global__ void k_zero_words(uint32_t* w, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) w[i] = 0;
+}
+
+// entries per treelet sub-list: the nodes of the blocks of one residue
+inline int64_t sublist_cap(int64_t n) { return ((n / 256 + 1) / kSubLists + 1) * 256; }
+
 inline int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 // Word offsets of the restructuring's arrays in the build scratch, after the
 // sort's (each aligned to 64 words).
 struct TreeScratch {
-    size_t cost, nl, first, newid, L2, R2, P2, lo2, hi2, order2, tris2, end;
+    size_t tsum, cost, nl, first, newid, L2, R2, P2, lo2, hi2, order2, tris2, anc, dep, anc2, dep2, tlist, tcount, maxd, end;
     explicit TreeScratch(int64_t n) {
         const int64_t nb = (n + kSortTile - 1) / kSortTile;
         auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
-        size_t o = al((size_t)(4 * n + (int64_t)kRadix * nb + 8 + n + 64));
+        const int64_t tiles = ((int64_t)kRadix * nb + kScanTile - 1) / kScanTile;
+        size_t o = al((size_t)(4 * n + (int64_t)kRadix * nb + 8));
+        tsum = o; o = al(o + tiles);
         cost = o; o = al(o + 2 * n);
         nl = o; o = al(o + 2 * n);
         first = o; o = al(o + 2 * n);
@@ -597,6 +853,13 @@ struct TreeScratch {
         hi2 = o; o = al(o + 6 * n);
         order2 = o; o = al(o + n);
         tris2 = o; o = al(o + 12 * n);
+        anc = o; o = al(o + n);
+        dep = o; o = al(o + n);
+        anc2 = o; o = al(o + n);
+        dep2 = o; o = al(o + n);
+        tlist = o; o = al(o + (size_t)kSubLists * sublist_cap(n));
+        tcount = o; o = al(o + ((size_t)kSubLists << kDepthJumps));
+        maxd = o; o = al(o + 64);
         end = o;
     }
 };
@@ -609,8 +872,8 @@ uint32_t* bvh_error_word(void* scratch, int64_t n) {
 }
 
 size_t bvh_scratch_bytes(int64_t n) {
-    // keys/vals ping-pong (4n words) + histogram + bounds + flags, then the
-    // restructuring's costs, leaf counts and relabelled copies
+    // keys/vals ping-pong (4n words) + histogram + bounds, then the tree
+    // passes' costs, leaf counts, relabelled copies and depth arrays
     return TreeScratch(n).end * sizeof(uint32_t);
 }
 
@@ -624,16 +887,19 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
     int nb = (int)((n + kSortTile - 1) / kSortTile);
     uint32_t* hist = v1 + n;
     uint32_t* bounds = hist + (size_t)kRadix * nb;
-    uint32_t* flags = bounds + 8;
 
     k_init_bounds<<<1, 64, 0, s>>>(bounds);
     int gb = std::min(blocks_for(n, 256), 4096);
     k_centroid_bounds<<<gb, 256, 0, s>>>(pos, n, bounds);
     k_morton<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, bounds, k0, v0);
     uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    const int64_t hm = (int64_t)kRadix * nb, tiles = (hm + kScanTile - 1) / kScanTile;
+    uint32_t* tsum = w + TreeScratch(n).tsum;
     for (int shift = 0; shift < 30; shift += kRadixBits) {
         k_radix_hist<<<nb, kSortBlock, 0, s>>>(ki, n, shift, hist, nb);
-        k_scan_single<<<1, 1024, 0, s>>>(hist, (int64_t)kRadix * nb);
+        k_scan_reduce<<<(int)tiles, 256, 0, s>>>(hist, hm, tsum);
+        k_scan_single<<<1, 1024, 0, s>>>(tsum, tiles);
+        k_scan_apply<<<(int)tiles, 256, 0, s>>>(hist, hm, tsum);
         k_radix_scatter<<<nb, kSortBlock, 0, s>>>(ki, vi, n, shift, hist, nb, ko, vo);
         std::swap(ki, ko);
         std::swap(vi, vo);
@@ -642,23 +908,51 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(o.order, vi, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
+    if (n > 1) k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(o.codes, n, o.left, o.right, o.parent);
+    k_leaf_boxes<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.lo, o.hi, o.tris);
     if (n > 1) {
-        k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(o.codes, n, o.left, o.right, o.parent);
-        e = hipMemsetAsync(flags, 0, (size_t)(n - 1) * sizeof(uint32_t), s);
-        if (e != hipSuccess) return e;
-    }
-    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(pos, n, o.order, o.left, o.right, o.parent, flags, o.lo, o.hi,
-                                               o.tris, bounds + 6);
-    if (n > 2 && rounds > 0) {
         const TreeScratch ts(n);
+        const int64_t ni = n - 1;
         float* cost = reinterpret_cast<float*>(w + ts.cost);
         uint32_t* nl = w + ts.nl;
-        TreeDev T{n - 1, o.left, o.right, o.parent, nl, o.lo, o.hi, cost};
+        uint32_t* error = bounds + 6;
+        uint32_t* maxd = w + ts.maxd;
+        TreeDev T{ni, o.left, o.right, o.parent, nl, o.lo, o.hi, cost};
         k_tree_leaves<<<blocks_for(n, 256), 256, 0, s>>>(n, T);
-        for (int r = 0; r < rounds; r++) {
-            if ((e = hipMemsetAsync(flags, 0, (size_t)(n - 1) * sizeof(uint32_t), s)) != hipSuccess) return e;
-            k_treelet_round<<<blocks_for(n, 256), 256, 0, s>>>(n, T, (uint32_t)kTreeletLeaves << r, flags, bounds + 6);
+        // a refit pass (rounds 0), else one pass per restructuring round
+        const int passes = (n > 2 && rounds > 0) ? rounds : 1;
+        for (int r = 0; r < passes; r++) {
+            uint32_t *anc = w + ts.anc, *dep = w + ts.dep, *anc2 = w + ts.anc2, *dep2 = w + ts.dep2;
+            k_depth_init<<<blocks_for(ni, 256), 256, 0, s>>>(ni, o.parent, anc, dep, error);
+            for (int j = 0; j < kDepthJumps; j++) {
+                k_depth_jump<<<blocks_for(ni, 256), 256, 0, s>>>(ni, anc, dep, anc2, dep2);
+                std::swap(anc, anc2);
+                std::swap(dep, dep2);
+            }
+            k_zero_words<<<1, 64, 0, s>>>(maxd, 64);
+            k_depth_max<<<blocks_for(ni, 256), 256, 0, s>>>(ni, anc, dep, maxd, error);
+            uint32_t host[65] = {};   // error, then the 64 partial maxima
+            if ((e = hipMemcpyAsync(host, error, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                (e = hipMemcpyAsync(host + 1, maxd, 4 * 64, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                (e = hipStreamSynchronize(s)) != hipSuccess)
+                return e;
+            if (host[0] != 0) return hipSuccess;   // malformed: the caller reads the error word
+            const int64_t depth = *std::max_element(host + 1, host + 65);
+            const uint32_t gamma = (n > 2 && rounds > 0) ? (uint32_t)kTreeletLeaves << r : 0u;
+            const int64_t cap = sublist_cap(n);
+            uint32_t *tlist = w + ts.tlist, *tcount = w + ts.tcount;
+            if (gamma > 0) k_zero_words<<<blocks_for((depth + 1) * kSubLists, 256), 256, 0, s>>>(tcount, (depth + 1) * kSubLists);
+            for (int64_t d = depth; d >= 0; d--) {
+                uint32_t* cnt = tcount + d * kSubLists;
+                k_tree_level<<<blocks_for(ni, 256), 256, 0, s>>>(T, dep, (uint32_t)d, gamma, tlist, cap, cnt);
+                if (gamma > 0)
+                    k_treelets<<<std::min(blocks_for(ni, 256), 2048), 256, 0, s>>>(T, tlist, cap, cnt);
+            }
         }
+    }
+    if (n > 2 && rounds > 0) {
+        const TreeScratch ts(n);
+        uint32_t* nl = w + ts.nl;
         uint32_t *first = w + ts.first, *newid = w + ts.newid, *L2 = w + ts.L2, *R2 = w + ts.R2, *P2 = w + ts.P2,
                  *order2 = w + ts.order2;
         float *lo2 = reinterpret_cast<float*>(w + ts.lo2), *hi2 = reinterpret_cast<float*>(w + ts.hi2);
