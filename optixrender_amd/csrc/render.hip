@@ -1188,16 +1188,19 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
     q.depth[pix] = dz;
 }
 
-// The scene's cover set (one thread): starting from the root's two
-// children, replace every internal node by its children, level by level,
-// while the list stays within kCoverBoxes.  Entries are (lo.xyz, ref),
-// (hi.xyz, 0) with the exact boxes of the Karras layout.  Expanding from the
-// back writes each entry at or after the one being read.
-__global__ void k_cover_set(int64_t nt, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
-                            float4* boxes, int* count) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The scene's cover set (one wave): starting from the root's two children,
+// replace every internal node by its children, level by level, while the
+// list stays within kCoverBoxes.  Entries are (lo.xyz, ref), (hi.xyz, 0) with
+// the exact boxes of the Karras layout; an expanded entry's children take its
+// place in the list (left, then right), so entry i of a level lands at i
+// plus the number of internal entries before it.  The levels are built in
+// LDS and the last one copied out.
+__global__ __launch_bounds__(64) void k_cover_set(int64_t nt, const uint32_t* left, const uint32_t* right,
+                                                  const float* lo, const float* hi, float4* boxes, int* count) {
+    __shared__ float4 lv[2][2 * kCoverBoxes];
+    const int lane = threadIdx.x;
     if (nt <= 0) {
-        *count = 0;
+        if (lane == 0) *count = 0;
         return;
     }
     const int64_t ni = nt - 1;
@@ -1207,34 +1210,44 @@ __global__ void k_cover_set(int64_t nt, const uint32_t* left, const uint32_t* ri
         out[1] = make_float4(hi[3 * idx], hi[3 * idx + 1], hi[3 * idx + 2], 0.0f);
     };
     if (nt == 1) {
-        put(kLeafBit, boxes);
-        *count = 1;
+        if (lane == 0) {
+            put(kLeafBit, boxes);
+            *count = 1;
+        }
         return;
     }
-    put(left[0], boxes);
-    put(right[0], boxes + 2);
-    int n = 2;
+    if (lane == 0) put(left[0], lv[0]);
+    if (lane == 1) put(right[0], lv[0] + 2);
+    int n = 2, cur = 0;
     while (true) {
+        __syncthreads();
         int internal = 0;
-        for (int i = 0; i < n; i++) internal += (__float_as_uint(boxes[2 * i].w) & kLeafBit) ? 0 : 1;
+        for (int i = lane; i < n; i += 64) internal += (__float_as_uint(lv[cur][2 * i].w) & kLeafBit) ? 0 : 1;
+        for (int o = 32; o > 0; o >>= 1) internal += __shfl_xor(internal, o);
         if (internal == 0 || n + internal > kCoverBoxes) break;
-        int m = n + internal;
-        for (int i = n - 1; i >= 0; i--) {
-            const float4 blo = boxes[2 * i], bhi = boxes[2 * i + 1];
+        int carry = 0;  // internal entries before this chunk
+        for (int i0 = 0; i0 < n; i0 += 64) {
+            const int i = i0 + lane;
+            const bool in = i < n;
+            const float4 blo = in ? lv[cur][2 * i] : make_float4(0, 0, 0, 0), bhi = in ? lv[cur][2 * i + 1] : blo;
             const uint32_t ref = __float_as_uint(blo.w);
-            if (ref & kLeafBit) {
-                m--;
-                boxes[2 * m] = blo;
-                boxes[2 * m + 1] = bhi;
-            } else {
-                m -= 2;
-                put(left[ref], boxes + 2 * m);
-                put(right[ref], boxes + 2 * m + 2);
+            const bool inner = in && !(ref & kLeafBit);
+            const uint64_t b = __ballot(inner);
+            const int at = i + carry + __popcll(b & ((1ull << lane) - 1));
+            if (inner) {
+                put(left[ref], lv[cur ^ 1] + 2 * at);
+                put(right[ref], lv[cur ^ 1] + 2 * at + 2);
+            } else if (in) {
+                lv[cur ^ 1][2 * at] = blo;
+                lv[cur ^ 1][2 * at + 1] = bhi;
             }
+            carry += __popcll(b);
         }
         n += internal;
+        cur ^= 1;
     }
-    *count = n;
+    for (int i = lane; i < 2 * n; i += 64) boxes[i] = lv[cur][i];
+    if (lane == 0) *count = n;
 }
 
 // Cover mask of one request (one block).  A camera ray of pixel (ix, iy)
