@@ -50,11 +50,12 @@ def bvh_cases():
     }
 
 
-@pytest.mark.parametrize("rounds", [0, -1])
+@pytest.mark.parametrize("rounds", [0, -1, 2, 4])
 @pytest.mark.parametrize("name", list(bvh_cases().keys()))
 def test_lbvh_bit_exact(ctx, name, rounds):
     # rounds 0: the plain LBVH; -1: the default build (treelet restructuring
-    # and relabelling on top of it)
+    # and relabelling on top of it); 2 and 4: later rounds restructure a
+    # restructured tree, whose depths are computed again
     pos = bvh_cases()[name]
     print(f"[lbvh] case={name} n={pos.shape[0]} rounds={rounds}", flush=True)
     sid = ctx.upload(soup_scene(pos))
