@@ -168,6 +168,10 @@ def roofline(counted, trace_ms, launches, traffic_file, workload, tuning, strict
         "launches_per_step": int(launches), "bytes_per_step": int(alg),
         "bytes_basis": f"algorithmic: {NODE_BYTES_DOC}, node and triangle counts from a counting render",
         "per_ray_bytes": {"extension": round(per_ext, 1), "shadow": round(per_sh, 1)},
+        # active lanes per lane slot of the traversal loop's iterations
+        "simd_use": round(counted.trace_lane_visits / max(counted.trace_wave_slots, 1), 4),
+        "simd_use_basis": "node visits of active lanes / (64 x the most visits of any lane of each 64-ray chunk), "
+                          "from the counting render, which also traces the camera rays the cover mask resolves",
     }
     # the same counts priced at 32 B per child box, the unit of uncompressed
     # (lo, ref, hi) records: how fast the traversal covers the tree
