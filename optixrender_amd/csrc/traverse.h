@@ -181,103 +181,138 @@ struct Stack {
     }
 };
 
+// One lane's traversal between two node visits: the ray, the best hit so far
+// and the record of the node to test next (its loads issued by the visit
+// before).  traverse() runs a walk to its end; the trace kernel also keeps
+// walks of many rays in flight in one wave, a node per lane at a time.
+struct Walk {
+    RayPre r;
+    Hit h;
+    uint4 w0, w1;
+    float tmin, tmax;
+    int budget;
+};
+
+// The triangle of sorted leaf k against the walk's ray; true when it is taken
+// (any hit: it ends the walk).
+template <bool ANY, bool COUNT>
+OXR_D bool walk_leaf(const BvhView& b, Walk& w, uint32_t k, uint32_t& nt, int* overflow) {
+    if (k >= (uint64_t)b.n) { *overflow = 2; return false; }
+    const float4* tr = b.tris + 3 * (size_t)k;
+    float4 a = tr[0], bb = tr[1], c = tr[2];
+    if (COUNT) nt++;
+    float t;
+    if (!tri_hit_t(w.r, a, bb, c, w.tmin, t)) return false;
+    uint32_t id = __float_as_uint(a.w);
+    bool take = ANY ? (t < w.tmax) : (t < w.h.t || (t == w.h.t && w.h.leaf != kNoHit && id < w.h.id));
+    if (take) { w.h.t = t; w.h.leaf = k; w.h.id = id; }
+    return take;
+}
+
+// Starts a walk at the root; false when there is none to take (fewer than two
+// triangles: the hit is then already final).
+template <bool ANY, bool COUNT, class StackT>
+OXR_D bool walk_start(const BvhView& b, Walk& w, f3 o, f3 d, float tmin, float tmax, StackT& st, uint32_t& nt,
+                      int* overflow) {
+    w.h.t = tmax;
+    w.h.leaf = kNoHit;
+    w.h.id = kNoHit;
+    w.h.u = w.h.v = 0.0f;
+    w.tmin = tmin;
+    w.tmax = tmax;
+    if (b.n <= 0) return false;
+    w.r = prep_ray(o, d);
+    if (b.n == 1) {
+        walk_leaf<ANY, COUNT>(b, w, 0, nt, overflow);
+        return false;
+    }
+    st.sp = 0;
+    // A tree traversal visits each internal node at most once; the cap only
+    // guards against a malformed hierarchy.
+    w.budget = (int)b.n + 64 - 1;  // the root's visit
+    w.w0 = b.qnodes[0];
+    w.w1 = b.qnodes[1];
+    return true;
+}
+
+// One node of a walk: tests the record in w0/w1, picks the next node and
+// issues its record's loads before the triangle tests of this one's leaf
+// children, so that the two fetches overlap; the nodes visited, their order
+// and every hit are those of testing first and loading after.  False when
+// the walk has ended.
+template <bool ANY, bool COUNT, class StackT>
+OXR_D bool walk_step(const BvhView& b, Walk& w, StackT& st, uint32_t& nn, uint32_t& nt, int* overflow) {
+    if (COUNT) nn += 2;
+    const uint4 w0 = w.w0, w1 = w.w1;
+    const uint32_t g = w0.w & kQGamma;
+    const uint32_t c0 = (w0.w & kQLeft) ? (kLeafBit | g) : g;
+    const uint32_t c1 = (w0.w & kQRight) ? (kLeafBit | (g + 1)) : g + 1;
+    float tn0, tn1;
+    bool h0, h1;
+    qnode_test(w.r, w0, w1, w.tmin, w.h.t, h0, h1, tn0, tn1);
+    const bool l0 = h0 && (c0 & kLeafBit), l1 = h1 && (c1 & kLeafBit);
+    h0 = h0 && !l0;
+    h1 = h1 && !l1;
+    // the next node: the nearer internal child (any hit: the farther -- a
+    // shadow ray leaving a surface is mostly blocked by the far side of the
+    // object it starts on, not by the boxes around its origin, about 40%
+    // fewer node fetches on sun shadow rays; the answer does not depend on
+    // the order), the only one, or the top of the stack
+    uint32_t next = 0, far = 0;
+    bool more = true, push = false;
+    if (h0 && h1) {
+        bool second = tn1 < tn0;
+        if (ANY) second = !second;
+        next = second ? c1 : c0;
+        far = second ? c0 : c1;
+        push = true;
+    } else if (h0) {
+        next = c0;
+    } else if (h1) {
+        next = c1;
+    } else if (st.sp == 0) {
+        more = false;
+    } else {
+        next = st.pop();
+    }
+    const bool bad = more && (--w.budget < 0 || next >= (uint32_t)(b.n - 1));
+    if (more && !bad) {
+        w.w0 = b.qnodes[2 * (size_t)next];
+        w.w1 = b.qnodes[2 * (size_t)next + 1];
+    }
+    // Leaf children: child 0's triangle, then child 1's.  One block serves
+    // every lane with a leaf on either side and a second only the lanes with
+    // two, so a wave whose lanes reach leaves on different sides runs the
+    // triangle test once instead of twice.
+    if (l0 || l1) {
+        if (walk_leaf<ANY, COUNT>(b, w, (l0 ? c0 : c1) & ~kLeafBit, nt, overflow) && ANY) return false;
+        if (l0 && l1 && walk_leaf<ANY, COUNT>(b, w, c1 & ~kLeafBit, nt, overflow) && ANY) return false;
+    }
+    if (!more) return false;
+    if (push && !st.push(far)) { *overflow = 1; return false; }
+    if (bad) { *overflow = 2; return false; }
+    return true;
+}
+
+// The ended walk's barycentrics of the closest hit, recomputed from its
+// triangle record.
+template <bool ANY>
+OXR_D void walk_finish(const BvhView& b, Walk& w) {
+    if (!ANY && w.h.leaf != kNoHit) {
+        const float4* tr = b.tris + 3 * (size_t)w.h.leaf;
+        tri_uv(w.r, tr[0], tr[1], tr[2], w.h.u, w.h.v);
+    }
+}
+
 template <bool ANY, bool COUNT, class StackT>
 OXR_D Hit traverse(const BvhView& b, f3 o, f3 d, float tmin, float tmax, StackT& st, uint32_t& nn, uint32_t& nt,
                    int* overflow) {
-    Hit h;
-    h.t = tmax;
-    h.leaf = kNoHit;
-    h.id = kNoHit;
-    h.u = h.v = 0.0f;
-    if (b.n <= 0) return h;
-    RayPre r = prep_ray(o, d);
-    auto leaf_test = [&](uint32_t k) -> bool {
-        if (k >= (uint64_t)b.n) { *overflow = 2; return false; }
-        const float4* tr = b.tris + 3 * (size_t)k;
-        float4 a = tr[0], bb = tr[1], c = tr[2];
-        if (COUNT) nt++;
-        float t;
-        if (!tri_hit_t(r, a, bb, c, tmin, t)) return false;
-        uint32_t id = __float_as_uint(a.w);
-        bool take = ANY ? (t < tmax) : (t < h.t || (t == h.t && h.leaf != kNoHit && id < h.id));
-        if (take) { h.t = t; h.leaf = k; h.id = id; }
-        return take;
-    };
-    // barycentrics of the closest hit, recomputed from its triangle record
-    auto finish = [&]() {
-        if (!ANY && h.leaf != kNoHit) {
-            const float4* tr = b.tris + 3 * (size_t)h.leaf;
-            tri_uv(r, tr[0], tr[1], tr[2], h.u, h.v);
+    Walk w;
+    if (walk_start<ANY, COUNT>(b, w, o, d, tmin, tmax, st, nt, overflow))
+        while (walk_step<ANY, COUNT>(b, w, st, nn, nt, overflow)) {
         }
-    };
-    if (b.n == 1) {
-        leaf_test(0);
-        finish();
-        return h;
-    }
-    st.sp = 0;
-    uint32_t node = 0;
-    // A tree traversal visits each internal node at most once; the cap only
-    // guards against a malformed hierarchy.
-    int budget = (int)b.n + 64 - 1;  // the root's visit
-    uint4 w0 = b.qnodes[0], w1 = b.qnodes[1];
-    // Each iteration tests the record in w0/w1, picks the next node and
-    // issues its record's loads before the triangle tests of this one's leaf
-    // children, so that the two fetches overlap; the nodes visited, their
-    // order and every hit are those of testing first and loading after.
-    while (true) {
-        if (COUNT) nn += 2;
-        const uint32_t g = w0.w & kQGamma;
-        const uint32_t c0 = (w0.w & kQLeft) ? (kLeafBit | g) : g;
-        const uint32_t c1 = (w0.w & kQRight) ? (kLeafBit | (g + 1)) : g + 1;
-        float tn0, tn1;
-        bool h0, h1;
-        qnode_test(r, w0, w1, tmin, h.t, h0, h1, tn0, tn1);
-        const bool l0 = h0 && (c0 & kLeafBit), l1 = h1 && (c1 & kLeafBit);
-        h0 = h0 && !l0;
-        h1 = h1 && !l1;
-        // the next node: the nearer internal child (any hit: the farther --
-        // a shadow ray leaving a surface is mostly blocked by the far side of
-        // the object it starts on, not by the boxes around its origin, about
-        // 40% fewer node fetches on sun shadow rays; the answer does not
-        // depend on the order), the only one, or the top of the stack
-        uint32_t next = 0, far = 0;
-        bool more = true, push = false;
-        if (h0 && h1) {
-            bool second = tn1 < tn0;
-            if (ANY) second = !second;
-            next = second ? c1 : c0;
-            far = second ? c0 : c1;
-            push = true;
-        } else if (h0) {
-            next = c0;
-        } else if (h1) {
-            next = c1;
-        } else if (st.sp == 0) {
-            more = false;
-        } else {
-            next = st.pop();
-        }
-        const bool bad = more && (--budget < 0 || next >= (uint32_t)(b.n - 1));
-        if (more && !bad) {
-            w0 = b.qnodes[2 * (size_t)next];
-            w1 = b.qnodes[2 * (size_t)next + 1];
-        }
-        // Leaf children: child 0's triangle, then child 1's.  One block
-        // serves every lane with a leaf on either side and a second only the
-        // lanes with two, so a wave whose lanes reach leaves on different
-        // sides runs the triangle test once instead of twice.
-        if (l0 || l1) {
-            if (leaf_test((l0 ? c0 : c1) & ~kLeafBit) && ANY) break;
-            if (l0 && l1 && leaf_test(c1 & ~kLeafBit) && ANY) break;
-        }
-        if (!more) break;
-        if (push && !st.push(far)) { *overflow = 1; break; }
-        if (bad) { *overflow = 2; break; }
-        node = next;
-    }
-    finish();
-    return h;
+    walk_finish<ANY>(b, w);
+    return w.h;
 }
 
 }  // namespace oxr
