@@ -337,7 +337,14 @@ OXR_D int sky_blocks(int k, int64_t total) {
     return nb > k ? (int)((nb - k + kShards - 1) / kShards) : 0;
 }
 
+// The first pass keeps each block's mask of samples to trace in LDS (up to
+// kSkyMasks blocks per wave), so that the second pass, which writes the
+// list, does not classify them again.
+constexpr int kSkyMasks = 512;
+
 __global__ __launch_bounds__(256) void k_sky_list(WaveParams p) {
+    __shared__ uint64_t masks[256 / 64][kSkyMasks];
+    uint64_t* mine = masks[threadIdx.x >> 6];
     const int k = wave_shard();
     const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int stride = (int)((gridDim.x * (blockDim.x >> 6)) / kShards);  // waves per shard
@@ -345,19 +352,27 @@ __global__ __launch_bounds__(256) void k_sky_list(WaveParams p) {
     const int blocks_k = sky_blocks(k, total);
     const int first = wave / kShards;
     int n_trace = 0;
-    for (int j = first; j < blocks_k; j += stride) {
+    for (int j = first, i = 0; j < blocks_k; j += stride, i++) {
         int64_t g;
-        n_trace += (int)__popcll(__ballot(sky_class(p, k, j, total, g) == 1));
+        const uint64_t mt = __ballot(sky_class(p, k, j, total, g) == 1);
+        if (i < kSkyMasks && lane_id() == 0) mine[i] = mt;
+        n_trace += (int)__popcll(mt);
     }
     int base = 0;
     if (lane_id() == 0 && n_trace) base = atomicAdd(ctr_at(p.ctr, kCtrShardSamples + k), n_trace);
     base = __shfl(base, 0);
     uint32_t* list = p.samples + (size_t)k * p.shard_cap;
-    for (int j = first; j < blocks_k; j += stride) {
-        int64_t g;
-        const bool tr = sky_class(p, k, j, total, g) == 1;
-        const uint64_t mt = __ballot(tr);
-        if (tr) list[base + (int)__popcll(mt & lanes_below())] = (uint32_t)g;
+    for (int j = first, i = 0; j < blocks_k; j += stride, i++) {
+        // the first pass's mask, or the classification again past the LDS array
+        uint64_t mt;
+        if (i < kSkyMasks) {
+            mt = mine[i];
+        } else {
+            int64_t g;
+            mt = __ballot(sky_class(p, k, j, total, g) == 1);
+        }
+        if ((mt >> lane_id()) & 1)
+            list[base + (int)__popcll(mt & lanes_below())] = (uint32_t)((((int64_t)j * kShards + k) << 6) + lane_id());
         base += (int)__popcll(mt);
     }
 }
