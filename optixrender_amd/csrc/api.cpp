@@ -815,6 +815,8 @@ int oxr_render(oxr_ctx* c, const oxr_render_params* prm) {
     rq.path_radiance = keep ? c->path_radiance : nullptr;
     rq.cover = cull ? c->cover : nullptr;
     rq.tiles_w = cover_tiles(W);
+    rq.by_pixels = make_divider((uint32_t)W * (uint32_t)H);
+    rq.by_width = make_divider((uint32_t)W);
     OXR_HIP(c, hipMemcpy(c->reqs, &rq, sizeof(rq), hipMemcpyHostToDevice));
     WaveParams p;
     fill_scene_params(p, s, c->lut_bsdf, c->lut_albedo);

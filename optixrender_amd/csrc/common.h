@@ -98,6 +98,24 @@ hipError_t gather_sorted_attributes(const float* d_normals, const uint32_t* d_ma
 // One render request's share of a wavefront window.  A window holds
 // `frames` frames of every pixel of each request, laid out request after
 // request: sample id = path_begin + s * width * height + pixel.
+// Exact unsigned 32-bit division by a divisor fixed ahead of time, as a
+// multiply-high and two shifts (Granlund & Montgomery 1994, the round-up
+// method): with L = ceil(log2 d) and m = floor(2^32 (2^L - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(L, 1))) >> max(L - 1, 0) where t = mulhi(m, n).
+struct Divider {
+    uint32_t m, s1, s2;
+};
+inline Divider make_divider(uint32_t d) {
+    uint32_t L = 0;
+    while (L < 32 && (1ull << L) < d) L++;
+    const uint64_t m = ((((uint64_t)1 << L) - d) << 32) / d + 1;
+    return Divider{(uint32_t)m, L < 1 ? L : 1u, L > 1 ? L - 1 : 0u};
+}
+__device__ __forceinline__ uint32_t udiv(uint32_t n, Divider d) {
+    const uint32_t t = __umulhi(d.m, n);
+    return (t + ((n - t) >> d.s1)) >> d.s2;
+}
+
 struct ReqDesc {
     int64_t path_begin;    // first sample id of this request in the window
     int64_t pix_begin;     // first pixel of this request among the window's pixels
@@ -116,6 +134,7 @@ struct ReqDesc {
     // hit the scene, so its samples skip the first trace.  nullptr: none.
     const uint8_t* cover;
     int tiles_w;
+    Divider by_pixels, by_width;  // division by width * height and by width
 };
 
 // Camera-ray culling (build_cover_mask): the boxes of the BVH's top levels,

@@ -238,8 +238,8 @@ OXR_D void camera_ray(const WaveParams& p, int64_t g, f3& org, f3& dir, uint32_t
     // a window holds fewer than 2^31 samples: 32-bit division is exact
     const uint32_t local = (uint32_t)(g - q.path_begin);
     const uint32_t npix = (uint32_t)(q.width * q.height), w = (uint32_t)q.width;
-    const uint32_t s = local / npix, pix = local - s * npix;
-    const uint32_t iyu = pix / w;
+    const uint32_t s = udiv(local, q.by_pixels), pix = local - s * npix;
+    const uint32_t iyu = udiv(pix, q.by_width);
     const int ix = (int)(pix - iyu * w), iy = (int)iyu;
     Rng rng{lcg_seed((uint32_t)(ix + q.width * iy), (uint32_t)q.frame0 + s)};
     float j0 = rng(), j1 = rng();
@@ -263,9 +263,9 @@ OXR_D bool camera_ray_misses(const WaveParams& p, int64_t g) {
     const ReqDesc& q = p.reqs[find_req<false>(p.reqs, p.nreq, g)];
     if (!q.cover) return false;
     const uint32_t local = (uint32_t)(g - q.path_begin);  // a window holds fewer than 2^31 samples
-    const uint32_t w = (uint32_t)q.width;
-    const uint32_t pix = local % (w * (uint32_t)q.height);
-    const uint32_t iy = pix / w, ix = pix - iy * w;
+    const uint32_t w = (uint32_t)q.width, npix = w * (uint32_t)q.height;
+    const uint32_t pix = local - udiv(local, q.by_pixels) * npix;
+    const uint32_t iy = udiv(pix, q.by_width), ix = pix - iy * w;
     return q.cover[(iy / kCoverTile) * (uint32_t)q.tiles_w + ix / kCoverTile] == 0;
 }
 

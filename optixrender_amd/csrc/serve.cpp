@@ -229,6 +229,8 @@ hipError_t run_tick(oxr_server* s, Runner& rn, std::vector<Request*>& group, std
         d.depth = q->depth;
         d.cover = q->cover;
         d.tiles_w = cover_tiles(q->r.width);
+        d.by_pixels = make_divider((uint32_t)q->r.width * (uint32_t)q->r.height);
+        d.by_width = make_divider((uint32_t)q->r.width);
         const int64_t np = (int64_t)q->r.width * q->r.height;
         path += np * frames[i];
         pix += np;
