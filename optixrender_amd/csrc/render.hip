@@ -1187,10 +1187,24 @@ __global__ void k_accumulate(WaveParams p, int64_t npix_total) {
         }
         if (frame > 0) {
             float f = (float)frame, g = (float)frame + 1.0f;
-            x.x = (x.x + f * c.x) / g; x.y = (x.y + f * c.y) / g; x.z = (x.z + f * c.z) / g; x.w = (x.w + f * c.w) / g;
-            al.x = (al.x + f * a.x) / g; al.y = (al.y + f * a.y) / g; al.z = (al.z + f * a.z) / g; al.w = (al.w + f * a.w) / g;
-            no.x = (no.x + f * nr.x) / g; no.y = (no.y + f * nr.y) / g; no.z = (no.z + f * nr.z) / g; no.w = (no.w + f * nr.w) / g;
-            z = (z + f * dz) / g;
+            x.x = (x.x + f * c.x) / g; x.y = (x.y + f * c.y) / g; x.z = (x.z + f * c.z) / g;
+            // Where sample and mean are both 0 or both 1 the quotient is
+            // known exactly -- (0 + f 0) / g = +0, and (1 + f 1) / g = 1, the
+            // numerator and g being the same sum rounded alike -- so those
+            // divisions are skipped: the colour's w (always 1), and the AOVs
+            // of a default-AOV sample (albedo and normal (0, 0, 0, 1), depth
+            // 1) on default means, as every sky pixel has them.
+            if (!(x.w == 1.0f && c.w == 1.0f)) x.w = (x.w + f * c.w) / g;
+            const bool unit = a.x == 0.0f && a.y == 0.0f && a.z == 0.0f && a.w == 1.0f && nr.x == 0.0f &&
+                              nr.y == 0.0f && nr.z == 0.0f && nr.w == 1.0f && dz == 1.0f;
+            // a sample with default AOVs that was not dropped (a dropped
+            // one has depth 0): al, no and z are the constants
+            const bool dflt = r.w != 0.0f && z == 1.0f;
+            if (!(unit && dflt)) {
+                al.x = (al.x + f * a.x) / g; al.y = (al.y + f * a.y) / g; al.z = (al.z + f * a.z) / g; al.w = (al.w + f * a.w) / g;
+                no.x = (no.x + f * nr.x) / g; no.y = (no.y + f * nr.y) / g; no.z = (no.z + f * nr.z) / g; no.w = (no.w + f * nr.w) / g;
+                z = (z + f * dz) / g;
+            }
         }
         c = x;
         a = al;
