@@ -333,7 +333,7 @@ int oxr_create(int device, oxr_ctx** out) {
     c->tuning.iters_per_poll = 2;
     c->tuning.pool_paths = 1 << 24;
     c->tuning.window_samples = 1 << 27;
-    c->tuning.tail_paths = 1 << 18;
+    c->tuning.tail_paths = 1 << 19;
     apply_tuning(c);
     if (dalloc(&c->reqs, 1) != hipSuccess) return bail(OXR_ERR_OOM);
     if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
