@@ -319,13 +319,14 @@ def denoiser(ctx, args, scene, frames=8):
 
 
 HF_W, HF_H, HF_SPP, HF_DEPTH = 1920, 1080, 32, 32
+HF_PROBE_SPP = 8
 
 
 def heightfield_leg(ctx, args, steps=3):
     """The batched configuration: 1920x1080, 32 spp, depth 32 over the
     20,016,980-triangle fBm heightfield with the homogeneous-medium box and
     the dielectric sphere, per-triangle interleaved materials.  The launch
-    tuning comes from oxr_autotune on a 4-spp probe; the timed steps replay
+    tuning comes from oxr_autotune on an 8-spp probe; the timed steps replay
     the captured iteration graphs.  Its own counting render gives the
     traversal roofline, and the CPU oracle is timed on a sample of it."""
     from optixrender_amd import abi, scenes
@@ -342,9 +343,11 @@ def heightfield_leg(ctx, args, steps=3):
         ctx.set_camera(sc.camera)
         print(f"[bench] heightfield: {sc.n_tris} tris generated in {gen_s:.1f}s, BVH build {build_ms:.1f} ms",
               file=sys.stderr, flush=True)
-        tuned, probe_ms = ctx.autotune(spp=4, max_depth=HF_DEPTH, rr_start=0)
+        # an 8-spp probe: 16.6 M samples fill the largest pool, so that pool
+        # sizes are compared on a full pool as in the 32-spp step
+        tuned, probe_ms = ctx.autotune(spp=HF_PROBE_SPP, max_depth=HF_DEPTH, rr_start=0)
         ctx.set_camera(sc.camera)
-        print(f"[bench] heightfield: autotune {tuned.as_dict()} ({probe_ms:.2f} ms per 4-spp probe)",
+        print(f"[bench] heightfield: autotune {tuned.as_dict()} ({probe_ms:.2f} ms per {HF_PROBE_SPP}-spp probe)",
               file=sys.stderr, flush=True)
 
         def step(flags):

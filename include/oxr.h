@@ -330,8 +330,9 @@ int oxr_set_tuning(oxr_ctx* ctx, const oxr_tuning* t);
 /* Coordinate-descent search over oxr_tuning: renders `probe` on the selected
    scene for each candidate value of one knob at a time (LDS stack entries,
    trace and shade blocks per CU, pool slots, tail_paths, iterations per
-   poll), keeps the fastest, applies it and returns it with its render time.
-   Resets frameID. */
+   poll), each timed as the best of three renders after a warm-up, moves to
+   a candidate only when it is at least 1% faster, applies the result and
+   returns it with its render time.  Resets frameID. */
 int oxr_autotune(oxr_ctx* ctx, const oxr_render_params* probe, oxr_tuning* best, double* best_ms);
 
 /* ---- building blocks exposed for parity tests ---- */

@@ -975,13 +975,14 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
     auto time_it = [&](const oxr_tuning& t, double& ms) -> int {
         int rc = oxr_set_tuning(c, &t);
         if (rc != OXR_OK) return rc;
-        double tot = 0.0;
-        for (int rep = 0; rep < 3; rep++) {  // first run warms up (graph capture, allocation)
+        // the first run warms up (graph capture, allocation); the best of
+        // the next three is the candidate's time, the least disturbed one
+        ms = 1e30;
+        for (int rep = 0; rep < 4; rep++) {
             rc = oxr_render(c, &pr);
             if (rc != OXR_OK) return rc;
-            if (rep > 0) tot += c->stats.render_ms;
+            if (rep > 0) ms = std::min(ms, c->stats.render_ms);
         }
-        ms = tot / 2.0;
         return OXR_OK;
     };
     double cur_ms = 0.0;
@@ -1006,7 +1007,8 @@ int oxr_autotune(oxr_ctx* c, const oxr_render_params* probe, oxr_tuning* best, d
             double ms = 0.0;
             rc = time_it(t, ms);
             if (rc != OXR_OK) return rc;
-            if (ms < cur_ms) { cur_ms = ms; cur = t; }
+            // a candidate must win by 1%, so that timing noise alone does not move the tuning
+            if (ms < 0.99 * cur_ms) { cur_ms = ms; cur = t; }
         }
     }
     rc = oxr_set_tuning(c, &cur);
