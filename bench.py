@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--profile-steps", action="store_true", help="only run the timed steps (for rocprofv3)")
     ap.add_argument("--tuning", default="",
                     help="comma-separated oxr_tuning fields to set before the steps, e.g. stack_lds=16,pool_paths=4194304")
-    ap.add_argument("--autotune", action="store_true", help="run oxr_autotune on a 4-spp probe before the steps")
+    ap.add_argument("--autotune", action="store_true", help="run oxr_autotune on an 8-spp probe before the steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (the known-size reference for HBM counters)")
     return ap.parse_args()
@@ -319,7 +319,8 @@ def denoiser(ctx, args, scene, frames=8):
 
 
 HF_W, HF_H, HF_SPP, HF_DEPTH = 1920, 1080, 32, 32
-HF_PROBE_SPP = 8
+# autotune probes: 8 spp of 1080p (16.6 M samples) fill the largest pool
+PROBE_SPP = HF_PROBE_SPP = 8
 
 
 def heightfield_leg(ctx, args, steps=3):
@@ -446,9 +447,9 @@ def main():
             setattr(t, k.strip(), int(v))
         ctx.set_tuning(t)
     if args.autotune:
-        t, ms = ctx.autotune(spp=4, max_depth=depth, rr_start=0)
+        t, ms = ctx.autotune(spp=PROBE_SPP, max_depth=depth, rr_start=0)
         ctx.set_camera(scene.camera)
-        print(f"[bench] autotune: {t.as_dict()} ({ms:.2f} ms per 4-spp probe)", file=sys.stderr, flush=True)
+        print(f"[bench] autotune: {t.as_dict()} ({ms:.2f} ms per {PROBE_SPP}-spp probe)", file=sys.stderr, flush=True)
 
     if args.profile_denoiser:
         print(json.dumps({"denoiser": denoiser(ctx, args, scene)}), flush=True)
