@@ -555,16 +555,18 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
     // lowest slot and a right part.  A subset of size k has 2^(k-1) - 1
     // splits; candidate t is the t-th of them in descending order of the
     // left part (the left part is the lowest slot plus the subset of the
-    // other slots numbered 2^(k-1) - 2 - t).  A segment of 2^(k-1) lanes takes
-    // one subset, a lane a candidate, and the best is the least finite cost,
-    // the lowest t among equal ones: the first found by a scan in that order
-    // keeping a candidate only when strictly cheaper than the best so far.
+    // other slots numbered 2^(k-1) - 2 - t).  A segment of 2^(k-3) lanes (at
+    // least one) takes one subset, a lane four consecutive candidates, and
+    // the best is the least finite cost, the lowest t among equal ones: the
+    // first found by a scan in that order keeping a candidate only when
+    // strictly cheaper than the best so far.
 #pragma unroll
     for (int size = 2; size <= kTreeletLeaves; size++) {
         if (size > m) break;
-        const int W = 1 << (size - 1), per = 64 / W;
+        const int W = 1 << (size - 1);
+        const int SL = W >= 4 ? W / 4 : 1, per = 64 / SL;  // lanes per subset, subsets per pass
         const int first = kSubsetStart.v[size], cnt = kSubsetStart.v[size + 1] - first;
-        const int g = lane / W, t = lane & (W - 1);
+        const int g = lane / SL, u = lane & (SL - 1);
         for (int b0 = 0; b0 < cnt; b0 += per) {
             const int idx = b0 + g;
             const int s = idx < cnt ? smask[first + idx] : 0;
@@ -572,17 +574,24 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
             const int low = s & -s, r = s ^ low;
             float bc = __builtin_inff();
             int bt = W;  // none
-            if (live && t < W - 1) {
-                const int q = deposit_bits(W - 2 - t, r) | low;
-                const float c = L.cost[q] + L.cost[s ^ q];
-                if (c < __builtin_inff()) { bc = c; bt = t; }
+            if (live) {
+                // candidates 4u .. 4u + 3 in order: a later one only when strictly cheaper
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int t = 4 * u + v;
+                    if (t < W - 1) {
+                        const int q = deposit_bits(W - 2 - t, r) | low;
+                        const float c = L.cost[q] + L.cost[s ^ q];
+                        if (c < __builtin_inff() && (bt == W || c < bc)) { bc = c; bt = t; }
+                    }
+                }
             }
-            for (int o = 1; o < W; o <<= 1) {
+            for (int o = 1; o < SL; o <<= 1) {
                 const float oc = __shfl_xor(bc, o);
                 const int ot = __shfl_xor(bt, o);
                 if (ot < W && (bt == W || oc < bc || (oc == bc && ot < bt))) { bc = oc; bt = ot; }
             }
-            if (live && t == 0) {
+            if (live && u == 0) {
                 L.cost[s] = kCostNode * L.sarea[s] + bc;
                 L.part[s] = (uint8_t)(bt < W ? (deposit_bits(W - 2 - bt, r) | low) : 0);
             }
