@@ -689,16 +689,25 @@ __global__ void k_depth_jump(int64_t ni, const uint32_t* __restrict__ anc, const
     anc2[i] = anc[a];
     dep2[i] = dep[i] + dep[a];
 }
+// The deepest depth as 64 partial maxima, one atomic per block.  A node
+// whose ancestor is not yet the root adds bit 31 to its depth, so a maximum
+// with that bit set asks for more jumps; after the last jumps (last != 0)
+// it marks the hierarchy malformed.
 __global__ __launch_bounds__(256) void k_depth_max(int64_t ni, const uint32_t* __restrict__ anc,
-                                                   const uint32_t* __restrict__ dep, uint32_t* maxd, uint32_t* error) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+                                                   const uint32_t* __restrict__ dep, uint32_t* maxd, uint32_t* error,
+                                                   int last) {
+    __shared__ uint32_t wmax[256 / 64];
     uint32_t d = 0;
-    if (i < ni) {
-        if (anc[i] != 0) atomicOr(error, 1u);
-        d = dep[i];
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ni; i += (int64_t)gridDim.x * blockDim.x)
+        d = max(d, dep[i] | (anc[i] != 0 ? 0x80000000u : 0u));
     for (int o = 32; o > 0; o >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(&maxd[blockIdx.x % 64], d);  // 64 partial maxima
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        atomicMax(&maxd[blockIdx.x % 64], d);
+        if (last && (d >> 31)) atomicOr(error, 1u);
+    }
 }
 
 // One depth of a bottom-up pass: the nodes at depth d join their children;
@@ -933,20 +942,29 @@ hipError_t bvh_build(const float* pos, int64_t n, BvhBuffers& o, void* scratch, 
         for (int r = 0; r < passes; r++) {
             uint32_t *anc = w + ts.anc, *dep = w + ts.dep, *anc2 = w + ts.anc2, *dep2 = w + ts.dep2;
             k_depth_init<<<blocks_for(ni, 256), 256, 0, s>>>(ni, o.parent, anc, dep, error);
-            for (int j = 0; j < kDepthJumps; j++) {
-                k_depth_jump<<<blocks_for(ni, 256), 256, 0, s>>>(ni, anc, dep, anc2, dep2);
-                std::swap(anc, anc2);
-                std::swap(dep, dep2);
+            // six jumps reach the root from depths below 64; the other seven
+            // run only when some node has not reached it
+            constexpr int kFirstJumps = 6;
+            int64_t depth = 0;
+            for (int j0 = 0; j0 < kDepthJumps;) {
+                const int j1 = j0 == 0 ? kFirstJumps : kDepthJumps;
+                for (; j0 < j1; j0++) {
+                    k_depth_jump<<<blocks_for(ni, 256), 256, 0, s>>>(ni, anc, dep, anc2, dep2);
+                    std::swap(anc, anc2);
+                    std::swap(dep, dep2);
+                }
+                k_zero_words<<<1, 64, 0, s>>>(maxd, 64);
+                k_depth_max<<<std::min(blocks_for(ni, 256), 2048), 256, 0, s>>>(ni, anc, dep, maxd, error,
+                                                                                j0 == kDepthJumps);
+                uint32_t host[65] = {};   // error, then the 64 partial maxima
+                if ((e = hipMemcpyAsync(host, error, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                    (e = hipMemcpyAsync(host + 1, maxd, 4 * 64, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                    (e = hipStreamSynchronize(s)) != hipSuccess)
+                    return e;
+                if (host[0] != 0) return hipSuccess;   // malformed: the caller reads the error word
+                depth = *std::max_element(host + 1, host + 65);
+                if (!(depth >> 31)) break;
             }
-            k_zero_words<<<1, 64, 0, s>>>(maxd, 64);
-            k_depth_max<<<blocks_for(ni, 256), 256, 0, s>>>(ni, anc, dep, maxd, error);
-            uint32_t host[65] = {};   // error, then the 64 partial maxima
-            if ((e = hipMemcpyAsync(host, error, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-                (e = hipMemcpyAsync(host + 1, maxd, 4 * 64, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-                (e = hipStreamSynchronize(s)) != hipSuccess)
-                return e;
-            if (host[0] != 0) return hipSuccess;   // malformed: the caller reads the error word
-            const int64_t depth = *std::max_element(host + 1, host + 65);
             const uint32_t gamma = (n > 2 && rounds > 0) ? (uint32_t)kTreeletLeaves << r : 0u;
             const int64_t cap = sublist_cap(n);
             uint32_t *tlist = w + ts.tlist, *tcount = w + ts.tcount;
