@@ -414,9 +414,10 @@ struct TreeDev {
 struct TreeletLds {
     float cost[kTreeletSubsets];         // the dynamic program's best SAH cost per subset
     float sarea[kTreeletSubsets];        // each subset's box area (slot boxes folded in ascending order)
-    float jbox[kTreeletSubsets][6];      // the rewired nodes' boxes, by their leaf subset
-    float jcost[kTreeletSubsets];        // ... their SAH costs
-    uint32_t jnl[kTreeletSubsets];       // ... their leaf counts
+    float jbox[kTreeletLeaves][6];       // the rewired nodes' boxes, by join position
+    float jcost[kTreeletLeaves];         // ... their SAH costs
+    uint32_t jnl[kTreeletLeaves];        // ... their leaf counts
+    uint8_t jat[kTreeletSubsets];        // a rewired node's join position, by its leaf subset
     float lb[kTreeletLeaves][6];         // slot boxes (lo.xyz, hi.xyz)
     float area[kTreeletLeaves];          // slot box areas
     float scost[kTreeletLeaves];         // slot SAH costs
@@ -625,26 +626,28 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
         const int set = L.order_set[i];
         const int pl = L.part[set], pr = set ^ pl;
         const bool sa = (pl & (pl - 1)) == 0, sc = (pr & (pr - 1)) == 0;
-        const float* ba = sa ? L.lb[__builtin_ctz(pl)] : L.jbox[pl];
-        const float* bc = sc ? L.lb[__builtin_ctz(pr)] : L.jbox[pr];
-        const float ca = sa ? L.scost[__builtin_ctz(pl)] : L.jcost[pl];
-        const float cc = sc ? L.scost[__builtin_ctz(pr)] : L.jcost[pr];
-        const uint32_t na = sa ? L.snl[__builtin_ctz(pl)] : L.jnl[pl];
-        const uint32_t nc = sc ? L.snl[__builtin_ctz(pr)] : L.jnl[pr];
+        const int ja = sa ? 0 : L.jat[pl], jc = sc ? 0 : L.jat[pr];
+        const float* ba = sa ? L.lb[__builtin_ctz(pl)] : L.jbox[ja];
+        const float* bc = sc ? L.lb[__builtin_ctz(pr)] : L.jbox[jc];
+        const float ca = sa ? L.scost[__builtin_ctz(pl)] : L.jcost[ja];
+        const float cc = sc ? L.scost[__builtin_ctz(pr)] : L.jcost[jc];
+        const uint32_t na = sa ? L.snl[__builtin_ctz(pl)] : L.jnl[ja];
+        const uint32_t nc = sc ? L.snl[__builtin_ctz(pr)] : L.jnl[jc];
         const int64_t node = L.order_node[i];
         float l[3], h[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             l[k] = fminf(ba[k], bc[k]);
             h[k] = fmaxf(ba[3 + k], bc[3 + k]);
-            L.jbox[set][k] = l[k];
-            L.jbox[set][3 + k] = h[k];
+            L.jbox[i][k] = l[k];
+            L.jbox[i][3 + k] = h[k];
             T.lo[3 * node + k] = l[k];
             T.hi[3 * node + k] = h[k];
         }
         const float cst = kCostNode * box_area(l, h) + ca + cc;
-        L.jcost[set] = cst;
-        L.jnl[set] = na + nc;
+        L.jcost[i] = cst;
+        L.jnl[i] = na + nc;
+        L.jat[set] = (uint8_t)i;
         T.nl[node] = na + nc;
         T.cost[node] = cst;
     }
@@ -746,8 +749,8 @@ __global__ __launch_bounds__(256) void k_tree_level(TreeDev T, const uint32_t* _
 }
 
 // The listed treelets of one depth, a wave each.
-__global__ __launch_bounds__(256) void k_treelets(TreeDev T, const uint32_t* __restrict__ list, int64_t cap,
-                                                  const uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(256, 8) void k_treelets(TreeDev T, const uint32_t* __restrict__ list, int64_t cap,
+                                                     const uint32_t* __restrict__ count) {
     __shared__ TreeletLds lds[256 / 64];
     __shared__ uint8_t smask[kTreeletSubsets];
     if (threadIdx.x < kTreeletSubsets) smask[threadIdx.x] = kSubsets.mask[threadIdx.x];
