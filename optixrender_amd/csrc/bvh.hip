@@ -576,12 +576,15 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
             float bc = __builtin_inff();
             int bt = W;  // none
             if (live) {
-                // candidates 4u .. 4u + 3 in order: a later one only when strictly cheaper
+                // candidates 4u .. 4u + 3 in order: a later one only when
+                // strictly cheaper.  The other slots of candidate t + 1 are
+                // the next smaller subset of r: (sub - 1) & r.
+                int sub = deposit_bits(W - 2 - 4 * u, r);
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
+                for (int v = 0; v < 4; v++, sub = (sub - 1) & r) {
                     const int t = 4 * u + v;
                     if (t < W - 1) {
-                        const int q = deposit_bits(W - 2 - t, r) | low;
+                        const int q = sub | low;
                         const float c = L.cost[q] + L.cost[s ^ q];
                         if (c < __builtin_inff() && (bt == W || c < bc)) { bc = c; bt = t; }
                     }
