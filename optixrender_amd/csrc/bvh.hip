@@ -456,6 +456,27 @@ constexpr SubsetTable make_subset_table() {
     return t;
 }
 __constant__ SubsetTable kSubsets = make_subset_table();
+
+// The deposit table: at (r >> 1) * 64 + u, the bits of u placed, in order,
+// on the set bits of r, for every r without bit 0 and every u < 64 (the
+// subset of r numbered u)
+struct alignas(16) DepositTable {
+    uint8_t d[64 * 64];
+};
+constexpr DepositTable make_deposit_table() {
+    DepositTable t{};
+    for (int k = 0; k < 64 * 64; k++) {
+        int u = k & 63, r = (k >> 6) << 1, d = 0;
+        for (int b = 0; r; b++) {
+            const int low = r & -r;
+            if ((u >> b) & 1) d |= low;
+            r ^= low;
+        }
+        t.d[k] = (uint8_t)d;
+    }
+    return t;
+}
+__constant__ DepositTable kDeposit = make_deposit_table();
 struct SubsetStart {
     int v[kTreeletLeaves + 2];
 };
@@ -467,16 +488,6 @@ constexpr SubsetStart make_subset_start() {
 }
 constexpr SubsetStart kSubsetStart = make_subset_start();
 
-// the bits of u placed, in order, on the set bits of r
-__device__ __forceinline__ int deposit_bits(int u, int r) {
-    int d = 0;
-    for (int b = 0; r; b++) {
-        const int low = r & -r;
-        if ((u >> b) & 1) d |= low;
-        r ^= low;
-    }
-    return d;
-}
 
 // slot i of the treelet := node ref r (its box, area, leaf count, cost, and
 // for an internal node its children, read with the rest so that expanding
@@ -501,8 +512,10 @@ __device__ __forceinline__ void treelet_slot(const TreeDev& T, TreeletLds& L, in
     L.scost[i] = T.cost[x];
 }
 
-// Treelet of internal node v, by the whole wave (every lane calls it).
-__device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const uint8_t* smask, int lane) {
+// Treelet of internal node v, by the whole wave (every lane calls it);
+// smask is the subset table and dtab the deposit table in LDS.
+__device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const uint8_t* smask, const uint8_t* dtab,
+                             int lane) {
     if (lane == 0) {
         // formation: expand the internal slot of largest area (the first of
         // equal ones) until there are seven slots
@@ -579,7 +592,7 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
                 // candidates 4u .. 4u + 3 in order: a later one only when
                 // strictly cheaper.  The other slots of candidate t + 1 are
                 // the next smaller subset of r: (sub - 1) & r.
-                int sub = deposit_bits(W - 2 - 4 * u, r);
+                int sub = dtab[(r >> 1) * 64 + (W - 2 - 4 * u)];
 #pragma unroll
                 for (int v = 0; v < 4; v++, sub = (sub - 1) & r) {
                     const int t = 4 * u + v;
@@ -597,7 +610,7 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
             }
             if (live && u == 0) {
                 L.cost[s] = kCostNode * L.sarea[s] + bc;
-                L.part[s] = (uint8_t)(bt < W ? (deposit_bits(W - 2 - bt, r) | low) : 0);
+                L.part[s] = (uint8_t)(bt < W ? (dtab[(r >> 1) * 64 + (W - 2 - bt)] | low) : 0);
             }
         }
         wave_lds_sync();
@@ -756,7 +769,10 @@ __global__ __launch_bounds__(256, 8) void k_treelets(TreeDev T, const uint32_t* 
                                                      const uint32_t* __restrict__ count) {
     __shared__ TreeletLds lds[256 / 64];
     __shared__ uint8_t smask[kTreeletSubsets];
+    __shared__ __align__(16) uint8_t dtab[64 * 64];   // the deposit table
     if (threadIdx.x < kTreeletSubsets) smask[threadIdx.x] = kSubsets.mask[threadIdx.x];
+    static_assert(sizeof(DepositTable) == 16 * 256, "a 16-byte piece per thread");
+    reinterpret_cast<uint4*>(dtab)[threadIdx.x] = reinterpret_cast<const uint4*>(kDeposit.d)[threadIdx.x];
     __syncthreads();
     TreeletLds& L = lds[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
@@ -772,7 +788,7 @@ __global__ __launch_bounds__(256, 8) void k_treelets(TreeDev T, const uint32_t* 
     for (uint32_t j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < total; j += waves) {
         const int sub = __builtin_ctzll(__ballot(inc > j));         // the sub-list holding entry j
         const uint32_t start = (uint32_t)__shfl((int)(inc - c), sub);
-        treelet_wave(T, list[(int64_t)sub * cap + (j - start)], L, smask, lane);
+        treelet_wave(T, list[(int64_t)sub * cap + (j - start)], L, smask, dtab, lane);
     }
 }
 
