@@ -419,6 +419,7 @@ struct TreeletLds {
     uint32_t jnl[kTreeletLeaves];        // ... their leaf counts
     uint8_t jat[kTreeletSubsets];        // a rewired node's join position, by its leaf subset
     float lb[kTreeletLeaves][6];         // slot boxes (lo.xyz, hi.xyz)
+    float pbox[8 + 16][6];               // boxes of the subsets of slots 0..2, then of slots 3..6
     float area[kTreeletLeaves];          // slot box areas
     float scost[kTreeletLeaves];         // slot SAH costs
     uint32_t snl[kTreeletLeaves];        // slot leaf counts
@@ -544,23 +545,44 @@ __device__ void treelet_wave(const TreeDev& T, int64_t v, TreeletLds& L, const u
     if (m < 3) return;
     const int full = (1 << m) - 1;
     if (lane < m) L.cost[1 << lane] = L.scost[lane];
-    // this lane's subsets s = lane + 1 and lane + 65: box (the slot boxes
-    // folded in ascending order) and area
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int s = lane + 1 + 64 * j;
-        if (s > full || (s & (s - 1)) == 0) continue;
-        int b = __builtin_ctz(s);
+    // subset boxes in two steps: lanes 0..7 fold the boxes of the subsets
+    // of slots 0..2, lanes 8..23 those of slots 3..6; then a subset's box is
+    // the union of its two parts.  Minima and maxima are exact, so the
+    // grouping changes no box.
+    if ((lane >= 1 && lane < 8) || (lane >= 9 && lane < 24)) {   // the non-empty parts
+        const int base = lane < 8 ? 0 : 3, bits = lane < 8 ? lane : lane - 8;
+        const int b0 = base + __builtin_ctz(bits);
         float l[3], h[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) { l[k] = L.lb[b][k]; h[k] = L.lb[b][3 + k]; }
-        for (int t = s & (s - 1); t; t &= t - 1) {
-            b = __builtin_ctz(t);
+        for (int k = 0; k < 3; k++) { l[k] = L.lb[b0][k]; h[k] = L.lb[b0][3 + k]; }
+        for (int t = bits & (bits - 1); t; t &= t - 1) {
+            const int b = base + __builtin_ctz(t);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 l[k] = fminf(l[k], L.lb[b][k]);
                 h[k] = fmaxf(h[k], L.lb[b][3 + k]);
             }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            L.pbox[lane][k] = l[k];
+            L.pbox[lane][3 + k] = h[k];
+        }
+    }
+    wave_lds_sync();
+    // this lane's subsets s = lane + 1 and lane + 65: box and area
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int s = lane + 1 + 64 * j;
+        if (s > full || (s & (s - 1)) == 0) continue;
+        const int sa = s & 7, sb = s >> 3;   // one of them may be empty
+        const float* a = L.pbox[sa ? sa : 8 + sb];
+        const float* b = L.pbox[sb ? 8 + sb : sa];
+        float l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l[k] = fminf(a[k], b[k]);
+            h[k] = fmaxf(a[3 + k], b[3 + k]);
         }
         L.sarea[s] = box_area(l, h);
     }
